@@ -1,0 +1,318 @@
+"""Python front end of libbhx.so (the gfx950 exact-diagonalisation engine), over its C ABI.
+
+Mirrors the reference's surface for the exact path:
+    Neighbours / lattice()         Neighbours::chain_neighbours, square_neighbours (src/neighbours.cpp:21-71)
+    Basis  (fixed_set_basis)       BH::fixed_set_basis (src/hamiltonian.cpp:143-149)
+    Hamiltonian                    BH::fixed_bosons_hamiltonian, J and U terms (src/hamiltonian.cpp:170-256)
+    Hamiltonian.spmv               SparseGenMatProd::perform_op (Spectra MatOp/SparseGenMatProd.h:81-86)
+    Hamiltonian.lanczos / IRLM_eigen  Op::IRLM_eigen (src/operator.cpp:22-33)
+Errors raise BHError with the library's message.  There is no CPU fallback: without libbhx.so or a
+GPU every compute call fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "libbhx.so")
+
+OK, EINVAL, ENOMEM, EBUDGET, ENOCONV, EHIP, ERANGE, EAGAIN = 0, 1, 2, 3, 4, 5, 6, 7
+CONV_REF_HOP = 1
+CONV_REF_INTERACTION = 2
+ORDER_REFERENCE = 0
+ORDER_ASCENDING = 1
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+_STATUS = {OK: "ok", EINVAL: "invalid argument", ENOMEM: "out of memory", EBUDGET: "over HBM budget",
+           ENOCONV: "not converged", EHIP: "HIP error", ERANGE: "out of range", EAGAIN: "not ready"}
+
+
+class BHError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class LanczosOpts(C.Structure):
+    _fields_ = [("k", C.c_int), ("prec", C.c_int), ("max_iter", C.c_int), ("chunk", C.c_int),
+                ("tol", C.c_double), ("dedup_rel", C.c_double), ("accept_rel", C.c_double),
+                ("seed", C.c_uint64), ("fallback", C.c_int), ("want_vectors", C.c_int)]
+
+
+class LanczosResult(C.Structure):
+    _fields_ = [("evals", C.c_void_p), ("resid", C.c_void_p), ("errest", C.c_void_p), ("iters", C.c_void_p),
+                ("used_fp64", C.c_void_p), ("status", C.c_void_p)]
+
+
+_lib = None
+_vp = C.c_void_p
+
+
+def lib():
+    """Loads libbhx.so (raises when it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BHError(EHIP, f"libbhx.so not built at {LIB_PATH}")
+        L = C.CDLL(LIB_PATH)
+        sig = {
+            "bh_ctx_create": (C.c_int, [C.c_int, C.c_size_t, C.POINTER(_vp)]),
+            "bh_ctx_destroy": (C.c_int, [_vp]),
+            "bh_last_error": (C.c_char_p, [_vp]),
+            "bh_ctx_memory": (C.c_int, [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+            "bh_ctx_set_timing": (C.c_int, [_vp, C.c_int]),
+            "bh_ctx_kernel_stats": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.c_int]),
+            "bh_device_name": (C.c_int, [C.c_int, C.c_char_p, C.c_int]),
+            "bh_lattice_neighbours": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp]),
+            "bh_basis_build": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
+            "bh_basis_destroy": (C.c_int, [_vp]),
+            "bh_basis_dim": (C.c_int64, [_vp]),
+            "bh_basis_copy_states": (C.c_int, [_vp, _vp]),
+            "bh_basis_copy_tags": (C.c_int, [_vp, _vp]),
+            "bh_basis_copy_perm": (C.c_int, [_vp, _vp, _vp]),
+            "bh_hamiltonian_build": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint, C.POINTER(_vp)]),
+            "bh_ham_destroy": (C.c_int, [_vp]),
+            "bh_ham_dim": (C.c_int64, [_vp]),
+            "bh_ham_nnz": (C.c_int64, [_vp]),
+            "bh_ham_device_bytes": (C.c_int64, [_vp]),
+            "bh_ham_copy_csr": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+            "bh_ham_copy_packed": (C.c_int, [_vp, _vp, C.POINTER(C.c_int)]),
+            "bh_ham_prepare": (C.c_int, [_vp, C.c_int]),
+            "bh_spmv": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]),
+            "bh_spmv_host": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
+            "bh_spmv_bench": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+        _bind_optional(L)
+    return _lib
+
+
+def _bind_optional(L):
+    opt = {
+        "bh_lanczos_default_opts": (None, [C.POINTER(LanczosOpts)]),
+        "bh_lanczos": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, C.POINTER(LanczosOpts), C.POINTER(LanczosResult)]),
+        "bh_lanczos_copy_vectors": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
+        "bh_tridiag_eigvals": (C.c_int, [C.c_int, _vp, _vp, _vp]),
+        "bh_server_create": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
+    }
+    for name, (res, args) in opt.items():
+        if hasattr(L, name):
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def lattice(geometry: str, m: int = 0, Lx: int = 0, Ly: int = 0, periodic: bool = True):
+    """Neighbour lists as (off[m+1], sites) for a chain of m sites or an Lx x Ly square lattice."""
+    if geometry == "chain":
+        kind, a, b, msites = 0, m, 1, m
+    elif geometry == "square":
+        kind, a, b, msites = 1, Lx, Ly, Lx * Ly
+    else:
+        raise ValueError(geometry)
+    off = np.zeros(msites + 1, np.int32)
+    sites = np.zeros(4 * msites + 4, np.int32)
+    t = lib().bh_lattice_neighbours(kind, a, b, int(periodic), _ptr(off), _ptr(sites))
+    if t < 0:
+        raise BHError(-t, "invalid lattice")
+    return off, sites[:t].copy()
+
+
+class Context:
+    """A device context (one HIP stream, an HBM budget)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        self.h = _vp()
+        rc = lib().bh_ctx_create(device, hbm_budget, C.byref(self.h))
+        if rc != OK:
+            raise BHError(rc, f"bh_ctx_create(device={device}) failed")
+
+    def check(self, rc):
+        if rc != OK:
+            raise BHError(rc, lib().bh_last_error(self.h).decode())
+        return rc
+
+    def close(self):
+        if self.h:
+            lib().bh_ctx_destroy(self.h)
+            self.h = _vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def memory(self):
+        u, b = C.c_size_t(), C.c_size_t()
+        self.check(lib().bh_ctx_memory(self.h, C.byref(u), C.byref(b)))
+        return u.value, b.value
+
+    def set_timing(self, on: bool):
+        self.check(lib().bh_ctx_set_timing(self.h, int(on)))
+
+    def kernel_stats(self, name: str, reset: bool = False):
+        c, t = C.c_int64(), C.c_double()
+        self.check(lib().bh_ctx_kernel_stats(self.h, name.encode(), C.byref(c), C.byref(t), int(reset)))
+        return c.value, t.value
+
+    def basis(self, m: int, n: int, order: int = ORDER_REFERENCE) -> "Basis":
+        return Basis(self, m, n, order)
+
+
+class Basis:
+    def __init__(self, ctx: Context, m: int, n: int, order: int = ORDER_REFERENCE):
+        self.ctx, self.m, self.n, self.order = ctx, m, n, order
+        self.h = _vp()
+        ctx.check(lib().bh_basis_build(ctx.h, m, n, order, C.byref(self.h)))
+        self.D = int(lib().bh_basis_dim(self.h))
+
+    def states(self):
+        out = np.zeros((self.D, self.m), np.uint8)
+        self.ctx.check(lib().bh_basis_copy_states(self.h, _ptr(out)))
+        return out
+
+    def tags(self):
+        out = np.zeros(self.D, np.float64)
+        self.ctx.check(lib().bh_basis_copy_tags(self.h, _ptr(out)))
+        return out
+
+    def perm(self):
+        a = np.zeros(self.D, np.int32)
+        b = np.zeros(self.D, np.int32)
+        self.ctx.check(lib().bh_basis_copy_perm(self.h, _ptr(a), _ptr(b)))
+        return a, b
+
+    def close(self):
+        if self.h:
+            lib().bh_basis_destroy(self.h)
+            self.h = _vp()
+
+    def hamiltonian(self, off, sites, flags: int = 0) -> "Hamiltonian":
+        return Hamiltonian(self, off, sites, flags)
+
+
+@dataclass
+class LanczosOut:
+    evals: np.ndarray
+    resid: np.ndarray
+    errest: np.ndarray
+    iters: np.ndarray
+    used_fp64: np.ndarray
+    status: np.ndarray
+
+
+class Hamiltonian:
+    def __init__(self, basis: Basis, off, sites, flags: int = 0):
+        self.basis, self.ctx = basis, basis.ctx
+        off = np.ascontiguousarray(off, np.int32)
+        sites = np.ascontiguousarray(sites, np.int32)
+        if sites.size == 0:
+            sites = np.zeros(1, np.int32)
+        self.h = _vp()
+        self.ctx.check(lib().bh_hamiltonian_build(self.ctx.h, basis.h, _ptr(off), _ptr(sites), flags, C.byref(self.h)))
+        self.D = int(lib().bh_ham_dim(self.h))
+        self.nnz = int(lib().bh_ham_nnz(self.h))
+
+    def close(self):
+        if self.h:
+            lib().bh_ham_destroy(self.h)
+            self.h = _vp()
+
+    def device_bytes(self):
+        return int(lib().bh_ham_device_bytes(self.h))
+
+    def csr(self):
+        rp = np.zeros(self.D + 1, np.int64)
+        col = np.zeros(max(self.nnz, 1), np.int32)
+        val = np.zeros(max(self.nnz, 1), np.float64)
+        diag = np.zeros(self.D, np.float64)
+        self.ctx.check(lib().bh_ham_copy_csr(self.h, _ptr(rp), _ptr(col), _ptr(val), _ptr(diag)))
+        return rp, col[: self.nnz], val[: self.nnz], diag
+
+    def packed(self):
+        w = np.zeros(max(self.nnz, 1), np.uint32)
+        cb = C.c_int()
+        self.ctx.check(lib().bh_ham_copy_packed(self.h, _ptr(w), C.byref(cb)))
+        return w[: self.nnz], cb.value
+
+    def spmv(self, J, U, x, prec: int = PREC_FP64):
+        """y_b = J_b T x_b + U_b V x_b for x of shape (D,) or (D, B)."""
+        x = np.asarray(x, np.float64)
+        one = x.ndim == 1
+        X = np.ascontiguousarray(x.reshape(self.D, -1))
+        B = X.shape[1]
+        Jv = np.ascontiguousarray(np.broadcast_to(np.asarray(J, np.float64), (B,)))
+        Uv = np.ascontiguousarray(np.broadcast_to(np.asarray(U, np.float64), (B,)))
+        Y = np.zeros_like(X)
+        self.ctx.check(lib().bh_spmv_host(self.ctx.h, self.h, prec, B, _ptr(Jv), _ptr(Uv), _ptr(X), _ptr(Y)))
+        return Y[:, 0] if one else Y
+
+    def spmv_bench(self, prec: int, B: int = 1, reps: int = 20):
+        ms, by = C.c_double(), C.c_double()
+        self.ctx.check(lib().bh_spmv_bench(self.ctx.h, self.h, prec, B, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
+    def lanczos(self, J, U, k: int = 4, prec: int = PREC_MIXED, **kw) -> LanczosOut:
+        J = np.atleast_1d(np.asarray(J, np.float64))
+        U = np.atleast_1d(np.asarray(U, np.float64))
+        B = max(J.size, U.size)
+        J = np.ascontiguousarray(np.broadcast_to(J, (B,)))
+        U = np.ascontiguousarray(np.broadcast_to(U, (B,)))
+        o = LanczosOpts()
+        lib().bh_lanczos_default_opts(C.byref(o))
+        o.k, o.prec = k, prec
+        for key, v in kw.items():
+            setattr(o, key, v)
+        ev = np.full((B, k), np.nan)
+        rs = np.full((B, k), np.nan)
+        ee = np.full((B, k), np.nan)
+        it = np.zeros(B, np.int32)
+        uf = np.zeros(B, np.int32)
+        st = np.zeros(B, np.int32)
+        r = LanczosResult(_ptr(ev).value, _ptr(rs).value, _ptr(ee).value, _ptr(it).value, _ptr(uf).value,
+                          _ptr(st).value)
+        rc = lib().bh_lanczos(self.ctx.h, self.h, B, _ptr(J), _ptr(U), C.byref(o), C.byref(r))
+        if rc not in (OK, ENOCONV):
+            self.ctx.check(rc)
+        return LanczosOut(ev, rs, ee, it, uf, st)
+
+    def lanczos_vectors(self, B: int, k: int):
+        v = np.zeros((B, k, self.D))
+        self.ctx.check(lib().bh_lanczos_copy_vectors(self.ctx.h, B, k, _ptr(v)))
+        return v
+
+
+def device_name(device: int = 0) -> str:
+    buf = C.create_string_buffer(256)
+    rc = lib().bh_device_name(device, buf, 256)
+    if rc != OK:
+        raise BHError(rc, "bh_device_name failed")
+    return buf.value.decode()
+
+
+def gap_ratio(levels) -> float:
+    """Mean of min(d_i, d_{i+1}) / max(d_i, d_{i+1}) over ascending levels, 0 when the max is 0
+    (Analysis::gap_ratios, src/analysis.cpp:433-454, mean at :317-318)."""
+    ev = np.sort(np.asarray(levels, np.float64))
+    if ev.size < 3:
+        return 0.0
+    d = np.diff(ev)
+    a, b = d[:-1], d[1:]
+    mn, mx = np.minimum(a, b), np.maximum(a, b)
+    r = np.where(mx != 0, mn / np.where(mx != 0, mx, 1.0), 0.0)
+    return float(r.mean())

@@ -1,0 +1,101 @@
+// bh_kernels.hpp -- device building blocks of the operator application and of the Lanczos step.
+//
+// Layout: B vectors are stored row-interleaved, x[r * B + b], so that the gather x[c * B + b] of one
+// column for all B vectors is one contiguous B-element segment.  In a workgroup, lane l works on
+// vector b = l % B, on the g = (l / B) % G-th strided part of its row, and on row slot l / (G * B).
+#pragma once
+
+#include "bh_internal.hpp"
+
+constexpr int BH_FMT_PACKED = 0;  // 32-bit word: column << code_bits | code, value -sqrt(code)
+constexpr int BH_FMT_CSR64 = 1;   // int32 column + FP64 value
+
+constexpr int BH_WG = 256;  // workgroup size of the operator kernels (4 waves)
+
+template <typename XT>
+__device__ __forceinline__ double to_d(XT v) {
+    return static_cast<double>(v);
+}
+
+// Partial sum of one lane over its strided part of row r: sum_k value_k * x[col_k * B + b].
+// For the packed format the table holds +sqrt(code); the caller negates.
+template <typename XT, int B, int G, int FMT>
+__device__ __forceinline__ double bh_row_partial(int32_t rs, int32_t re, int g, int b, const uint32_t* __restrict__ words,
+                                                 const int32_t* __restrict__ col64, const double* __restrict__ val64,
+                                                 const double* __restrict__ tab, int cb, const XT* __restrict__ x) {
+    double acc = 0.0;
+    if constexpr (FMT == BH_FMT_PACKED) {
+        const uint32_t mask = (1u << cb) - 1u;
+        for (int32_t k = rs + g; k < re; k += G) {
+            const uint32_t w = words[k];
+            acc += tab[w & mask] * to_d(x[static_cast<int64_t>(w >> cb) * B + b]);
+        }
+    } else {
+        for (int32_t k = rs + g; k < re; k += G) acc += val64[k] * to_d(x[static_cast<int64_t>(col64[k]) * B + b]);
+    }
+    return acc;
+}
+
+// sum over the G lanes that share (row slot, b); every lane of the wave must call it
+template <int B, int G>
+__device__ __forceinline__ double bh_group_sum(double v) {
+#pragma unroll
+    for (int o = B; o < B * G; o <<= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sums NV values per vector b over the workgroup.  Each lane contributes v[0..NV-1] for its
+// b = lane % B.  On return, out[q * B + b] (LDS, NV*B doubles) holds the workgroup totals; all lanes
+// must call it.  scratch: (BH_WG/64) * B * NV doubles of LDS.
+template <int B, int NV>
+__device__ __forceinline__ void bh_block_sum_b(double (&v)[NV], double* scratch, double* out) {
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double t = v[q];
+#pragma unroll
+        for (int o = B; o < 64; o <<= 1) t += __shfl_xor(t, o, 64);
+        if (lane < B) scratch[(wid * NV + q) * B + lane] = t;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < NV * B; i += BH_WG) {
+        const int q = i / B, b = i % B;
+        double t = 0.0;
+        for (int w = 0; w < BH_WG / 64; ++w) t += scratch[(w * NV + q) * B + b];
+        out[i] = t;
+    }
+    __syncthreads();
+}
+
+// true in exactly one workgroup: the last one to arrive.  Partials written before the call by any
+// workgroup are visible to the last one after it returns.  The counter is reset by the caller.
+__device__ __forceinline__ bool bh_last_block(unsigned* counter) {
+    __shared__ int amlast;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        amlast = (prev == gridDim.x - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (amlast) __threadfence();
+    return amlast != 0;
+}
+
+__device__ __forceinline__ double bh_coherent_load(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void bh_reset_counter(unsigned* counter) {
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Row tile of a workgroup with XCD-aware placement.  Workgroups are dispatched round-robin over the
+// 8 XCDs of the chip (workgroup w runs on XCD w % 8); tile ids are renumbered so that each XCD walks
+// one contiguous eighth of the rows, keeping the x gathers of neighbouring rows in one L2.
+__device__ __forceinline__ int64_t bh_xcd_tile(int64_t wg, int64_t nwg) {
+    const int64_t per = nwg / 8;
+    if (per == 0 || wg >= per * 8) return wg;  // tail workgroups keep their natural index
+    return (wg % 8) * per + wg / 8;
+}

@@ -1,0 +1,200 @@
+// bh_spmv.hip -- application of H_b = J_b T + U_b V to B row-interleaved vectors.
+//
+// Replaces the Eigen CSC x dense product behind Spectra's SparseGenMatProd::perform_op
+// (external/spectra/include/Spectra/MatOp/SparseGenMatProd.h:81-86 ->
+// include/Eigen/src/SparseCore/SparseDenseProduct.h:86-107), a serial column scatter in the
+// reference.  Here: CSR rows, G lanes per row and B vectors per CSR pass, FP64 accumulation.
+// The kernel is HBM-bound: per pass it reads the matrix once (4 B per entry in the packed format,
+// 12 B in FP64 CSR) plus the row pointers and the diagonal, and B * (x gather + y write).
+#include <cmath>
+
+#include "bh_kernels.hpp"
+#include "bh_spmv.hpp"
+
+template <typename XT, int B, int G, int FMT>
+__global__ void __launch_bounds__(BH_WG) k_spmv(SpmvArgs<XT> a) {
+    constexpr int GB = G * B;
+    constexpr int SLOTS = BH_WG / GB;
+    extern __shared__ double tab[];  // (1 << cb) entries, packed format only
+    if constexpr (FMT == BH_FMT_PACKED) {
+        for (int i = threadIdx.x; i < (1 << a.cb); i += BH_WG) tab[i] = a.table[i];
+        __syncthreads();
+    }
+    const int lane = threadIdx.x;
+    const int slot = lane / GB;
+    const int g = (lane % GB) / B;
+    const int b = lane % B;
+    const int64_t ntile = (a.D + SLOTS - 1) / SLOTS;
+    const int64_t tile = a.xcd ? bh_xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+    if (tile >= ntile) return;  // whole workgroup leaves together (tile is uniform)
+    const int64_t r = tile * SLOTS + slot;
+    double acc = 0.0;
+    if (r < a.D) acc = bh_row_partial<XT, B, G, FMT>(a.row_ptr[r], a.row_ptr[r + 1], g, b, a.words, a.col64, a.val64, tab,
+                                                       a.cb, a.x);
+    acc = bh_group_sum<B, G>(acc);
+    if (g == 0 && r < a.D) {
+        const double hx = (FMT == BH_FMT_PACKED) ? -acc : acc;
+        const double d = a.diag32 ? static_cast<double>(a.diag32[r]) : a.diag64[r];
+        const double xr = to_d(a.x[r * B + b]);
+        a.y[r * B + b] = static_cast<XT>(a.J[b] * hx + a.U[b] * d * xr);
+    }
+}
+
+template <typename XT, int FMT>
+static void launch_spmv_t(const SpmvArgs<XT>& a, int B, int G, hipStream_t s) {
+    const int GB = G * B;
+    const int64_t slots = BH_WG / GB;
+    const unsigned grid = static_cast<unsigned>((a.D + slots - 1) / slots);
+    const size_t shm = FMT == BH_FMT_PACKED ? (static_cast<size_t>(1) << a.cb) * sizeof(double) : 0;
+#define L(BB, GG)                                                                               \
+    if (B == BB && G == GG) {                                                                  \
+        hipLaunchKernelGGL((k_spmv<XT, BB, GG, FMT>), dim3(grid), dim3(BH_WG), shm, s, a);     \
+        return;                                                                                \
+    }
+    L(1, 1) L(1, 2) L(1, 4) L(1, 8) L(1, 16) L(1, 32) L(1, 64)
+    L(2, 1) L(2, 2) L(2, 4) L(2, 8) L(2, 16) L(2, 32)
+    L(4, 1) L(4, 2) L(4, 4) L(4, 8) L(4, 16)
+    L(8, 1) L(8, 2) L(8, 4) L(8, 8)
+    L(16, 1) L(16, 2) L(16, 4)
+    L(32, 1) L(32, 2)
+    L(64, 1)
+#undef L
+}
+
+void bh_launch_spmv(const SpmvArgs<float>& a, int B, int G, hipStream_t s) { launch_spmv_t<float, BH_FMT_PACKED>(a, B, G, s); }
+void bh_launch_spmv(const SpmvArgs<double>& a, int B, int G, int fmt, hipStream_t s) {
+    if (fmt == BH_FMT_PACKED) launch_spmv_t<double, BH_FMT_PACKED>(a, B, G, s);
+    else launch_spmv_t<double, BH_FMT_CSR64>(a, B, G, s);
+}
+
+int bh_pick_lanes(const bh_ham* h, int B) {
+    // lanes per row: about half the mean row length, as a power of two, G*B <= 64
+    const double avg = h->D ? static_cast<double>(h->nnz) / static_cast<double>(h->D) : 1.0;
+    int G = 1;
+    while (G < 64 && G * 2 <= avg) G <<= 1;
+    while (G * B > 64 && G > 1) G >>= 1;
+    return G;
+}
+
+double bh_spmv_bytes(const bh_ham* h, int prec, int B) {
+    const double D = static_cast<double>(h->D), nnz = static_cast<double>(h->nnz);
+    if (prec == BH_PREC_FP64) return nnz * 12.0 + 4.0 * (D + 1) + 8.0 * D + B * 2.0 * 8.0 * D;
+    return nnz * 4.0 + 4.0 * (D + 1) + 4.0 * D + B * 2.0 * 4.0 * D;
+}
+
+int bh_is_pow2_le64(int B) { return B >= 1 && B <= 64 && (B & (B - 1)) == 0; }
+
+extern "C" {
+
+int bh_spmv(bh_ctx* ctx, bh_ham* h, int prec, int B, const double* J, const double* U, const void* x, void* y,
+            void* stream) {
+    if (!ctx || !h || !J || !U || !x || !y) return BH_EINVAL;
+    if (!bh_is_pow2_le64(B)) return bh_set_error(ctx, BH_EINVAL, "spmv: B must be a power of two <= 64");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    BH_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    BH_TRY(bh_ham_prepare_locked(h, prec));
+    DevArray<double> dJ, dU;
+    BH_TRY(dJ.alloc(ctx, B));
+    BH_TRY(dU.alloc(ctx, B));
+    BH_HIP(ctx, hipMemcpyAsync(dJ.p, J, B * sizeof(double), hipMemcpyHostToDevice, s));
+    BH_HIP(ctx, hipMemcpyAsync(dU.p, U, B * sizeof(double), hipMemcpyHostToDevice, s));
+    const int G = bh_pick_lanes(h, B);
+    {
+        KTimer kt(ctx, "spmv", s);
+        if (prec == BH_PREC_FP64) {
+            SpmvArgs<double> a{h->D, h->row_ptr, h->words, h->col64, h->val64, nullptr, h->diag64, h->table, h->code_bits,
+                               dJ.p, dU.p, static_cast<const double*>(x), static_cast<double*>(y), 1};
+            bh_launch_spmv(a, B, G, BH_FMT_CSR64, s);
+        } else {
+            SpmvArgs<float> a{h->D, h->row_ptr, h->words, nullptr, nullptr, h->diag, nullptr, h->table, h->code_bits,
+                              dJ.p, dU.p, static_cast<const float*>(x), static_cast<float*>(y), 1};
+            bh_launch_spmv(a, B, G, s);
+        }
+    }
+    BH_HIP(ctx, hipGetLastError());
+    BH_HIP(ctx, hipStreamSynchronize(s));  // dJ/dU are freed on return
+    return BH_OK;
+}
+
+int bh_spmv_host(bh_ctx* ctx, bh_ham* h, int prec, int B, const double* J, const double* U, const double* x,
+                 double* y) {
+    if (!ctx || !h || !x || !y) return BH_EINVAL;
+    if (!bh_is_pow2_le64(B)) return bh_set_error(ctx, BH_EINVAL, "spmv: B must be a power of two <= 64");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    BH_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const size_t n = static_cast<size_t>(h->D) * B;
+    if (prec == BH_PREC_FP64) {
+        DevArray<double> dx, dy;
+        BH_TRY(dx.alloc(ctx, n));
+        BH_TRY(dy.alloc(ctx, n));
+        BH_HIP(ctx, hipMemcpyAsync(dx.p, x, n * 8, hipMemcpyHostToDevice, s));
+        BH_TRY(bh_spmv(ctx, h, prec, B, J, U, dx.p, dy.p, nullptr));
+        BH_HIP(ctx, hipMemcpyAsync(y, dy.p, n * 8, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipStreamSynchronize(s));
+    } else {
+        std::vector<float> xf(n), yf(n);
+        for (size_t i = 0; i < n; ++i) xf[i] = static_cast<float>(x[i]);
+        DevArray<float> dx, dy;
+        BH_TRY(dx.alloc(ctx, n));
+        BH_TRY(dy.alloc(ctx, n));
+        BH_HIP(ctx, hipMemcpyAsync(dx.p, xf.data(), n * 4, hipMemcpyHostToDevice, s));
+        BH_TRY(bh_spmv(ctx, h, prec, B, J, U, dx.p, dy.p, nullptr));
+        BH_HIP(ctx, hipMemcpyAsync(yf.data(), dy.p, n * 4, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipStreamSynchronize(s));
+        for (size_t i = 0; i < n; ++i) y[i] = yf[i];
+    }
+    return BH_OK;
+}
+
+int bh_spmv_bench(bh_ctx* ctx, bh_ham* h, int prec, int B, int reps, double* avg_ms, double* bytes) {
+    if (!ctx || !h || reps < 1) return BH_EINVAL;
+    if (!bh_is_pow2_le64(B)) return bh_set_error(ctx, BH_EINVAL, "spmv: B must be a power of two <= 64");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    BH_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    BH_TRY(bh_ham_prepare_locked(h, prec));
+    const size_t n = static_cast<size_t>(h->D) * B;
+    const size_t es = prec == BH_PREC_FP64 ? 8 : 4;
+    DevArray<uint8_t> dx, dy;
+    DevArray<double> dJ, dU;
+    BH_TRY(dx.alloc(ctx, n * es));
+    BH_TRY(dy.alloc(ctx, n * es));
+    BH_TRY(dJ.alloc(ctx, B));
+    BH_TRY(dU.alloc(ctx, B));
+    std::vector<double> ones(B, 1.0);
+    BH_HIP(ctx, hipMemcpyAsync(dJ.p, ones.data(), B * 8, hipMemcpyHostToDevice, s));
+    BH_HIP(ctx, hipMemcpyAsync(dU.p, ones.data(), B * 8, hipMemcpyHostToDevice, s));
+    BH_HIP(ctx, hipMemsetAsync(dx.p, 0, n * es, s));
+    const int G = bh_pick_lanes(h, B);
+    auto launch = [&]() {
+        if (prec == BH_PREC_FP64) {
+            SpmvArgs<double> a{h->D, h->row_ptr, h->words, h->col64, h->val64, nullptr, h->diag64, h->table, h->code_bits,
+                               dJ.p, dU.p, reinterpret_cast<const double*>(dx.p), reinterpret_cast<double*>(dy.p), 1};
+            bh_launch_spmv(a, B, G, BH_FMT_CSR64, s);
+        } else {
+            SpmvArgs<float> a{h->D, h->row_ptr, h->words, nullptr, nullptr, h->diag, nullptr, h->table, h->code_bits,
+                              dJ.p, dU.p, reinterpret_cast<const float*>(dx.p), reinterpret_cast<float*>(dy.p), 1};
+            bh_launch_spmv(a, B, G, s);
+        }
+    };
+    launch();  // warm-up
+    hipEvent_t e0, e1;
+    BH_HIP(ctx, hipEventCreate(&e0));
+    BH_HIP(ctx, hipEventCreate(&e1));
+    BH_HIP(ctx, hipEventRecord(e0, s));
+    for (int i = 0; i < reps; ++i) launch();
+    BH_HIP(ctx, hipEventRecord(e1, s));
+    BH_HIP(ctx, hipEventSynchronize(e1));
+    float ms = 0.f;
+    BH_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    BH_HIP(ctx, hipGetLastError());
+    if (avg_ms) *avg_ms = ms / reps;
+    if (bytes) *bytes = bh_spmv_bytes(h, prec, B);
+    return BH_OK;
+}
+
+}  // extern "C"

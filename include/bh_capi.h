@@ -1,0 +1,177 @@
+/*
+ * bh_capi.h -- C ABI of libbhx.so, the MI355X (gfx950) exact-diagonalisation engine for the
+ * fixed-N Bose-Hubbard Hamiltonian  H = J T + U V  (T: hopping over a neighbour list, V: on-site
+ * interaction; the chemical term -mu N is a constant in a fixed-N sector and is added by callers).
+ *
+ * The reference project has no FFI; its hot path sits behind C++ functions with Eigen types.
+ * Each entry point below names the reference interface it replaces (paths relative to the
+ * reference repository).  The C++ host layer (BH::, Op::, Analysis::, Neighbours, the CLI) is built
+ * on these calls.
+ *
+ * Conventions of the ABI:
+ *   - every function returns an int status (BH_OK == 0); no exception crosses the boundary;
+ *     bh_last_error(ctx) gives a message for the last failure on that context;
+ *   - handles are opaque and owned by the library; host arrays are owned by the caller;
+ *     device memory is owned by the library and released by the matching _destroy;
+ *   - a context serialises its calls with an internal mutex; use one context per host thread
+ *     for concurrency;
+ *   - the basis order is the reference's (see BH_ORDER_REFERENCE).
+ */
+#ifndef BH_CAPI_H
+#define BH_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ----------------------------------------------------------------------------- */
+#define BH_OK 0
+#define BH_EINVAL 1   /* invalid argument (sizes, neighbour list, options) */
+#define BH_ENOMEM 2   /* device or host allocation failed */
+#define BH_EBUDGET 3  /* request exceeds the context's HBM budget */
+#define BH_ENOCONV 4  /* eigensolver did not converge within max_iter */
+#define BH_EHIP 5     /* HIP runtime error */
+#define BH_ERANGE 6   /* sector too large for an exact 128-bit ordering key or the 32-bit index formats */
+#define BH_EAGAIN 7   /* serving: result not ready yet */
+
+/* ---- Hamiltonian conventions (flags of bh_hamiltonian_build) ------------------------------------ */
+#define BH_CONV_REF_HOP 1u         /* hop amplitude x2: each bond summed twice (src/hamiltonian.cpp:184-185,190) */
+#define BH_CONV_REF_INTERACTION 2u /* V = sum n(n+1) (src/hamiltonian.cpp:206) instead of sum n(n-1)/2 */
+
+/* ---- basis orders ------------------------------------------------------------------------------- */
+/* BH_ORDER_REFERENCE: the order BH::fixed_set_basis actually produces.  The states are enumerated in
+ * descending lexicographic order (src/hamiltonian.cpp:60-85), argsorted by tag = sum n_i ln p_i
+ * (:91-114), and the argsort is applied by a cycle-swap loop (:115-122) that realises the inverse
+ * permutation: the state of lexicographic rank p lands at position sortperm[p].
+ * BH_ORDER_ASCENDING: ascending tag (= ascending prod p_i^n_i), the order the code intended. */
+#define BH_ORDER_REFERENCE 0
+#define BH_ORDER_ASCENDING 1
+
+/* ---- precisions --------------------------------------------------------------------------------- */
+/* BH_PREC_FP64: FP64 CSR (double value + int32 column) and FP64 vectors -- the plain comparator.
+ * BH_PREC_MIXED: one 32-bit word per entry (column << code_bits | code), value = -sqrt(code) read
+ *   from an exact FP64 table, FP32-stored vectors, FP64 accumulation and reductions, FP64 finish. */
+#define BH_PREC_FP64 0
+#define BH_PREC_MIXED 1
+
+typedef struct bh_ctx bh_ctx;
+typedef struct bh_basis bh_basis;
+typedef struct bh_ham bh_ham;
+typedef struct bh_server bh_server;
+
+/* ---- context -------------------------------------------------------------------------------------- */
+/* hbm_budget: bytes this context may allocate on the device (0 = 90% of free device memory).
+ * Replaces the memory-budgeted thread cap Resource::set_omp_threads (src/resource.cpp:77-82). */
+int bh_ctx_create(int device, size_t hbm_budget, bh_ctx** out);
+int bh_ctx_destroy(bh_ctx* ctx);
+const char* bh_last_error(const bh_ctx* ctx);
+int bh_ctx_memory(const bh_ctx* ctx, size_t* used, size_t* budget);
+/* Kernel timing with HIP events on the library's stream (0 = off).  bh_ctx_kernel_stats returns
+ * the launch count and total milliseconds recorded for a kernel name since the last reset. */
+int bh_ctx_set_timing(bh_ctx* ctx, int on);
+int bh_ctx_kernel_stats(bh_ctx* ctx, const char* name, int64_t* count, double* total_ms, int reset);
+int bh_device_name(int device, char* buf, int len);
+
+/* ---- lattice ------------------------------------------------------------------------------------ */
+/* Neighbour lists as CSR: site i's neighbours are sites[off[i] .. off[i+1]-1].
+ * kind 0: chain of Lx sites (Neighbours::chain_neighbours, src/neighbours.cpp:21-34);
+ * kind 1: Lx x Ly square lattice, site = row*Lx + col, neighbours left/right/top/bottom
+ *         (Neighbours::square_neighbours, src/neighbours.cpp:40-71, whose shadowing local at :45
+ *         leaves the reference's lists empty; here they are filled and Lx may differ from Ly).
+ * off needs m+1 entries, sites 4m+4.  Returns the number of entries written, or < 0 on error. */
+int bh_lattice_neighbours(int kind, int Lx, int Ly, int periodic, int* off, int* sites);
+
+/* ---- basis ------------------------------------------------------------------------------------ */
+/* Replaces BH::fixed_set_basis(int m, int n) (include/hamiltonian.hpp:64, src/hamiltonian.cpp:143-149).
+ * Built on the device: unranking of lexicographic ranks, exact integer keys prod p_i^n_i, LSD radix
+ * sort, permutation and inverse permutation.  m <= 32, n <= 255, D = C(m+n-1, n) < 2^31. */
+int bh_basis_build(bh_ctx* ctx, int m, int n, int order, bh_basis** out);
+int bh_basis_destroy(bh_basis* b);
+int64_t bh_basis_dim(const bh_basis* b);
+/* states: D*m bytes (state k at k*m), the reference's column k of the basis matrix */
+int bh_basis_copy_states(const bh_basis* b, uint8_t* states);
+/* tags: D doubles, sum_i n_i * log(p_i) accumulated in site order (the reference's tags vector) */
+int bh_basis_copy_tags(const bh_basis* b, double* tags);
+/* lex_of_pos[k]: lexicographic rank of the state at position k; pos_of_lex: the inverse */
+int bh_basis_copy_perm(const bh_basis* b, int32_t* lex_of_pos, int32_t* pos_of_lex);
+
+/* ---- Hamiltonian ------------------------------------------------------------------------------- */
+/* Replaces BH::fixed_bosons_hamiltonian(nbr, basis, tags, m, n, J, U, mu) (include/hamiltonian.hpp:87,
+ * src/hamiltonian.cpp:170-256) for the J and U terms: two-pass count / exclusive scan / fill CSR
+ * build of T (off-diagonal, columns ascending per row, value -sqrt((n_i+1) n_j) per neighbour entry,
+ * repeated entries summed) and the diagonal V.  The neighbour list must be symmetric as a multiset
+ * and free of self-loops.  flags: BH_CONV_*. */
+int bh_hamiltonian_build(bh_ctx* ctx, const bh_basis* b, const int* nbr_off, const int* nbr_sites, unsigned flags,
+                         bh_ham** out);
+int bh_ham_destroy(bh_ham* h);
+int64_t bh_ham_dim(const bh_ham* h);
+int64_t bh_ham_nnz(const bh_ham* h);
+/* bytes the Hamiltonian occupies on the device */
+int64_t bh_ham_device_bytes(const bh_ham* h);
+/* host copy: row_ptr (D+1), col (nnz), val (nnz, T at J=1), diag (D, V at U=1); any may be NULL */
+int bh_ham_copy_csr(const bh_ham* h, int64_t* row_ptr, int32_t* col, double* val, double* diag);
+/* the packed words (column << code_bits | code) and code_bits of the mixed format */
+int bh_ham_copy_packed(const bh_ham* h, uint32_t* words, int* code_bits);
+/* Materialise the FP64 CSR (needed once before BH_PREC_FP64 use; done lazily otherwise). */
+int bh_ham_prepare(bh_ham* h, int prec);
+
+/* ---- operator application ---------------------------------------------------------------------- */
+/* Replaces the sparse product of SparseGenMatProd::perform_op
+ * (external/spectra/include/Spectra/MatOp/SparseGenMatProd.h:81-86): y_b = J_b T x_b + U_b V x_b for
+ * B vectors stored row-interleaved ([D][B]).  x, y are device pointers: double for BH_PREC_FP64,
+ * float for BH_PREC_MIXED (FP64 accumulation).  stream: a hipStream_t or NULL for the library's. */
+int bh_spmv(bh_ctx* ctx, bh_ham* h, int prec, int B, const double* J, const double* U, const void* x, void* y,
+            void* stream);
+/* Host-array convenience: x, y are host doubles [D][B]; FP64 accumulation in both precisions. */
+int bh_spmv_host(bh_ctx* ctx, bh_ham* h, int prec, int B, const double* J, const double* U, const double* x,
+                 double* y);
+/* Times `reps` back-to-back launches of the SpMV kernel used by bh_spmv (device-resident data,
+ * HIP events on the library stream).  avg_ms: average per launch; bytes: algorithmic bytes per launch. */
+int bh_spmv_bench(bh_ctx* ctx, bh_ham* h, int prec, int B, int reps, double* avg_ms, double* bytes);
+
+/* ---- Lanczos eigensolver ------------------------------------------------------------------------ */
+/* Replaces Op::IRLM_eigen(SparseMatrix O, int nb_eigen, MatrixXcd& evecs) (include/operator.hpp:34,
+ * src/operator.cpp:22-33, Spectra GenEigs nev/ncv=2nev+1/SmallestReal/tol 1e-10).
+ * Lanczos with selective orthogonalisation against converged Ritz vectors, FP64 alpha/beta
+ * reductions, FP64 tridiagonal analysis; B independent solves (J_b, U_b) share one CSR pass.
+ * Results are the lowest k distinct eigenvalues of H_b (a single start vector meets each
+ * degenerate level once; values within dedup_rel*||T|| count as one level). */
+typedef struct bh_lanczos_opts {
+    int k;             /* wanted eigenvalues (distinct levels), >= 1 */
+    int prec;          /* BH_PREC_FP64 or BH_PREC_MIXED */
+    int max_iter;      /* Krylov dimension cap (0: 1000) */
+    int chunk;         /* steps between convergence checks (0: 16) */
+    double tol;        /* Spectra rule: |beta s_last| < tol * max(eps^(2/3), |theta|) (0: 1e-10) */
+    double dedup_rel;  /* level merge tolerance relative to ||T|| (0: 1e-9) */
+    double accept_rel; /* mixed: FP64-finish acceptance of E0, ||r||^2/gap <= accept_rel*|E0| (0: 1e-9) */
+    uint64_t seed;     /* Park-Miller start vector seed (0 -> 1, Spectra's convention) */
+    int fallback;      /* mixed: rerun failing solves in FP64 (1, default) or report them (-1) */
+    int want_vectors;  /* 1: keep the k Ritz vectors (bh_lanczos_copy_vectors) */
+} bh_lanczos_opts;
+
+typedef struct bh_lanczos_result {
+    double* evals;   /* [B][k] eigenvalues (FP64-finished Rayleigh quotients), NaN if missing */
+    double* resid;   /* [B][k] FP64 residual norms ||H y - theta y|| (may be NULL) */
+    double* errest;  /* [B][k] a posteriori eigenvalue error bound ||r||^2 / gap (may be NULL) */
+    int* iters;      /* [B] Lanczos steps taken (may be NULL) */
+    int* used_fp64;  /* [B] 1 when the FP64 fallback produced the result (may be NULL) */
+    int* status;     /* [B] per-solve BH_OK / BH_ENOCONV (may be NULL) */
+} bh_lanczos_result;
+
+void bh_lanczos_default_opts(bh_lanczos_opts* o);
+int bh_lanczos(bh_ctx* ctx, bh_ham* h, int B, const double* J, const double* U, const bh_lanczos_opts* opts,
+               bh_lanczos_result* res);
+/* Ritz vectors of the last bh_lanczos call with want_vectors (host [B][k][D] doubles). */
+int bh_lanczos_copy_vectors(bh_ctx* ctx, int B, int k, double* vecs);
+
+/* Dense FP64 eigenvalues of a symmetric tridiagonal matrix (QL), exposed for tests of the host
+ * analysis: d[n] diagonal, e[n-1] off-diagonal; evals ascending. */
+int bh_tridiag_eigvals(int n, const double* d, const double* e, double* evals);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BH_CAPI_H */

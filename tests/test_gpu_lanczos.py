@@ -1,0 +1,149 @@
+"""GPU Lanczos against the oracle (dense FP64 eigh of the oracle's H, the oracle's FP64 Lanczos),
+the reference's own Spectra solver run on the same H (oracle/_ref), and closed forms.
+Tolerances: FP64 path 1e-9 relative on every level; mixed path 1e-9 relative on E0 (the acceptance
+target) and 1e-7 relative on the other levels."""
+import math
+
+import numpy as np
+import pytest
+
+import bhx
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = bhx.Context(0)
+    yield c
+    c.close()
+
+
+def build(ctx, geom, n):
+    if geom[0] == "chain":
+        m = geom[1]
+        off, sites = O.chain_neighbours(m, geom[2])
+    else:
+        m = geom[1] * geom[2]
+        off, sites = O.square_neighbours(geom[1], geom[2], geom[3])
+    b = ctx.basis(m, n)
+    return m, off, sites, b, b.hamiltonian(off, sites)
+
+
+def rel(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)) / np.maximum(np.abs(b), 1e-300)
+
+
+@pytest.mark.parametrize("prec", [bhx.PREC_FP64, bhx.PREC_MIXED])
+@pytest.mark.parametrize("geom,n,J,U,k", [(("chain", 8, True), 8, 1.0, 1.0, 10), (("chain", 6, True), 6, 0.1, 1.0, 4),
+                                          (("chain", 7, False), 5, 0.3, 1.0, 6), (("square", 3, 3, True), 6, 0.2, 1.0, 4),
+                                          (("chain", 10, True), 9, 0.05, 1.0, 4)])
+def test_lanczos_matches_dense(ctx, geom, n, J, U, k, prec):
+    print(f"lanczos dense {geom} n={n} J={J} prec={prec}", flush=True)
+    m, off, sites, b, h = build(ctx, geom, n)
+    ho = O.hamiltonian(m, n, off, sites)
+    if ho.D <= 7000:
+        want = O.distinct_levels(np.linalg.eigvalsh(ho.dense(J, U)), k)
+    else:
+        rc, want, _, _ = O.lanczos(ho, J, U, k)
+        assert rc == 0
+    r = h.lanczos(J, U, k=k, prec=prec)
+    assert r.status[0] == 0
+    e = rel(r.evals[0], want)
+    assert e[0] <= 1e-9
+    assert np.all(e <= (1e-9 if prec == bhx.PREC_FP64 else 1e-7)), (r.evals[0], want)
+    h.close()
+    b.close()
+
+
+@pytest.mark.parametrize("prec", [bhx.PREC_FP64, bhx.PREC_MIXED])
+def test_batched_equals_individual(ctx, prec):
+    m, off, sites, b, h = build(ctx, ("chain", 8, True), 8)
+    ho = O.hamiltonian(m, n := 8, off, sites)
+    Js = np.array([0.02, 0.07, 0.15, 0.3, 0.6])
+    r = h.lanczos(Js, 1.0, k=4, prec=prec)
+    for i, J in enumerate(Js):
+        want = O.distinct_levels(np.linalg.eigvalsh(ho.dense(J, 1.0)), 4)
+        assert rel(r.evals[i, 0], want[0]) <= 1e-9
+        assert np.all(rel(r.evals[i], want) <= 1e-7)
+        single = h.lanczos(J, 1.0, k=4, prec=prec)
+        assert rel(single.evals[0, 0], r.evals[i, 0]) <= 1e-9
+    h.close()
+    b.close()
+
+
+def test_closed_forms(ctx):
+    # U = 0, periodic chain: E0 = -2 J N (all bosons in the k = 0 orbital)
+    m, off, sites, b, h = build(ctx, ("chain", 10, True), 10)
+    r = h.lanczos(0.7, 0.0, k=2, prec=bhx.PREC_FP64)
+    assert abs(r.evals[0, 0] - (-2 * 0.7 * 10)) <= 1e-9 * 14
+    # U = 0: second level = one boson moved to k = +-2 pi / m
+    e1 = -2 * 0.7 * 9 - 2 * 0.7 * math.cos(2 * math.pi / 10)
+    assert abs(r.evals[0, 1] - e1) <= 1e-8
+    # J = 0, unit filling: levels 0, U, 2U, 3U (doubly occupied sites)
+    r = h.lanczos(0.0, 1.0, k=4, prec=bhx.PREC_MIXED)
+    assert np.allclose(r.evals[0], [0.0, 1.0, 2.0, 3.0], atol=1e-9)
+    h.close()
+    b.close()
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build oracle/_ref not present")
+@pytest.mark.parametrize("J", [1.0, 0.1])
+def test_matches_reference_spectra(ctx, J):
+    """The reference's Op::IRLM_eigen (Spectra GenEigs, nev=10) on the same Hamiltonian."""
+    m, n = 8, 8
+    _, off, sites, b, h = build(ctx, ("chain", m, True), n)
+    ho = O.hamiltonian(m, n, off, sites)
+    ir = np.zeros(10)
+    assert O.ref().ref_irlm_eigen(ho.D, ho.row_ptr, ho.col, ho.val, ho.diag, J, 1.0, 10, ir) == 0
+    want = O.distinct_levels(ir, 4)
+    for prec in (bhx.PREC_FP64, bhx.PREC_MIXED):
+        r = h.lanczos(J, 1.0, k=4, prec=prec)
+        assert rel(r.evals[0, 0], want[0]) <= 1e-9
+        assert np.all(rel(r.evals[0], want) <= 1e-7)
+    h.close()
+    b.close()
+
+
+def test_forced_fallback_and_nonconvergence(ctx):
+    m, off, sites, b, h = build(ctx, ("chain", 8, True), 8)
+    ref = h.lanczos(0.3, 1.0, k=4, prec=bhx.PREC_FP64)
+    # an impossible acceptance bound forces the FP64 rerun
+    r = h.lanczos(0.3, 1.0, k=4, prec=bhx.PREC_MIXED, accept_rel=1e-300)
+    assert r.used_fp64[0] == 1
+    assert np.allclose(r.evals[0], ref.evals[0], rtol=1e-12)
+    # fallback disabled: reported as is
+    r = h.lanczos(0.3, 1.0, k=4, prec=bhx.PREC_MIXED, accept_rel=1e-300, fallback=-1)
+    assert r.used_fp64[0] == 0
+    # too few steps: not converged status
+    r = h.lanczos(0.3, 1.0, k=4, prec=bhx.PREC_FP64, max_iter=12)
+    assert r.status[0] == bhx.ENOCONV
+    h.close()
+    b.close()
+
+
+def test_ritz_vectors(ctx):
+    m, off, sites, b, h = build(ctx, ("chain", 6, True), 6)
+    ho = O.hamiltonian(m, 6, off, sites)
+    r = h.lanczos(0.2, 1.0, k=3, prec=bhx.PREC_MIXED, want_vectors=1)
+    v = h.lanczos_vectors(1, 3)[0]
+    H = ho.dense(0.2, 1.0)
+    for i in range(3):
+        y = v[i]
+        assert abs(np.linalg.norm(y) - 1.0) < 1e-12
+        assert np.linalg.norm(H @ y - r.evals[0, i] * y) < 1e-5
+
+
+@pytest.mark.parametrize("geom,n,J", [(("chain", 12, True), 12, 0.1)])
+def test_full_size_mixed_vs_fp64(ctx, geom, n, J):
+    """The single large solve of the benchmark: FP32-mixed E0 within 1e-9 of the FP64 run."""
+    print(f"full size {geom} n={n}", flush=True)
+    m, off, sites, b, h = build(ctx, geom, n)
+    r64 = h.lanczos(J, 1.0, k=4, prec=bhx.PREC_FP64)
+    r32 = h.lanczos(J, 1.0, k=4, prec=bhx.PREC_MIXED)
+    assert r64.status[0] == 0 and r32.status[0] == 0
+    assert rel(r32.evals[0, 0], r64.evals[0, 0]) <= 1e-9
+    print("fp64", r64.evals[0], r64.iters, "mixed", r32.evals[0], r32.iters, r32.used_fp64, flush=True)
+    h.close()
+    b.close()

@@ -1,0 +1,250 @@
+"""CPU tests of the oracle (the checker), pinned against the reference's own compiled functions
+(oracle/_ref, when built), brute force and closed forms.  No GPU."""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+PRIMES = [2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71, 73, 79, 83, 89, 97]
+
+
+def brute_states(m, n):
+    """All occupations with sum n, descending lexicographic (src/hamiltonian.cpp:60-85 order)."""
+    out = [s for s in itertools.product(range(n, -1, -1), repeat=m) if sum(s) == n]
+    return out  # product over descending ranges is already descending lexicographic
+
+
+def key(s):
+    k = 1
+    for i, v in enumerate(s):
+        k *= PRIMES[i] ** v
+    return k
+
+
+@pytest.mark.parametrize("m,n", [(1, 0), (1, 4), (2, 3), (3, 3), (4, 2), (5, 5), (6, 3), (3, 7)])
+def test_dimension_and_enumeration(m, n):
+    lex = brute_states(m, n)
+    assert O.dimension(m, n) == len(lex) == math.comb(m + n - 1, n)
+    st, tags, lexrank = O.basis(m, n, O.ORDER_ASCENDING)
+    # ascending-key order = sorted by the exact integer key
+    want = sorted(lex, key=key)
+    assert [tuple(r) for r in st] == want
+    assert [lex[i] for i in lexrank] == want
+    # reference order: the state of lex rank p sits at position sortperm[p]
+    sortperm = sorted(range(len(lex)), key=lambda p: key(lex[p]))
+    ref = [None] * len(lex)
+    for p, q in enumerate(sortperm):
+        ref[q] = lex[p]
+    st0, tags0, _ = O.basis(m, n, O.ORDER_REFERENCE)
+    assert [tuple(r) for r in st0] == ref
+    # tags accumulate n_i * log(p_i) in site order
+    for r, t in zip(st0, tags0):
+        acc = 0.0
+        for i, v in enumerate(r):
+            acc += float(v) * math.log(PRIMES[i])
+        assert acc == t
+
+
+def test_log_tag_order_equals_integer_order():
+    # the reference sorts double tags; the integer key gives the same order wherever both are defined
+    for m, n in [(8, 8), (10, 9), (6, 12)]:
+        st, tags, _ = O.basis(m, n, O.ORDER_ASCENDING)
+        assert np.all(np.diff(tags) > 0)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build oracle/_ref not present")
+@pytest.mark.parametrize("m,n", [(2, 5), (3, 2), (5, 5), (8, 8), (6, 4), (10, 9), (4, 12)])
+def test_basis_matches_reference_build(m, n):
+    R = O.ref()
+    st, tags, _ = O.basis(m, n)
+    D = st.shape[0]
+    rt = np.zeros(D)
+    rs = np.zeros(D * m)
+    assert R.ref_fixed_set_basis(m, n, rt, rs) == D
+    assert np.array_equal(rs.reshape(D, m).astype(np.uint8), st)
+    assert np.array_equal(rt, tags)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build oracle/_ref not present")
+@pytest.mark.parametrize("m,n", [(3, 3), (5, 5), (8, 8), (7, 9)])
+def test_diagonal_terms_match_reference_build(m, n):
+    R = O.ref()
+    st, _, _ = O.basis(m, n)
+    D = st.shape[0]
+    h = O.hamiltonian(m, n, *O.chain_neighbours(m), flags=O.CONV_REF_INTERACTION, states=st)
+    d = np.zeros(D)
+    R.ref_diagonal_term(m, n, 1, d)
+    assert np.array_equal(d, h.diag)  # fill_interaction: sum n(n+1)
+    R.ref_diagonal_term(m, n, 2, d)
+    assert np.all(d == -float(n))  # fill_chemical: -N on the diagonal
+    hc = O.hamiltonian(m, n, *O.chain_neighbours(m), states=st)
+    # canonical V = sum n(n-1)/2 is an affine image of the reference's: sum n(n+1) = 2 V + 2 N
+    assert np.array_equal(h.diag, 2.0 * hc.diag + 2.0 * n)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build oracle/_ref not present")
+def test_neighbours_match_reference_build():
+    R = O.ref()
+    for m in (2, 3, 5, 8, 12):
+        for closed in (0, 1):
+            off = np.zeros(m + 1, np.int32)
+            s = np.zeros(2 * m + 2, np.int32)
+            t = R.ref_chain_neighbours(m, closed, off, s)
+            o2, s2 = O.chain_neighbours(m, bool(closed))
+            assert np.array_equal(off, o2) and np.array_equal(s[:t], s2)
+    # the reference's square lattice fills a shadowing local (src/neighbours.cpp:45): empty lists
+    off = np.zeros(17, np.int32)
+    s = np.zeros(80, np.int32)
+    assert R.ref_square_neighbours(16, 1, off, s) == 0
+    assert R.ref_square_neighbours(15, 1, off, s) == -1  # non-square m throws
+
+
+def test_square_neighbours_hand_written():
+    off, s = O.square_neighbours(3, 3, True)
+    lists = [sorted(s[off[i]:off[i + 1]]) for i in range(9)]
+    assert lists[0] == [1, 2, 3, 6] and lists[4] == [1, 3, 5, 7] and lists[8] == [2, 5, 6, 7]
+    off, s = O.square_neighbours(3, 3, False)
+    lists = [sorted(s[off[i]:off[i + 1]]) for i in range(9)]
+    assert lists[0] == [1, 3] and lists[4] == [1, 3, 5, 7]
+    off, s = O.square_neighbours(4, 3, True)  # 4 columns x 3 rows
+    lists = [sorted(s[off[i]:off[i + 1]]) for i in range(12)]
+    assert lists[0] == [1, 3, 4, 8] and lists[5] == [1, 4, 6, 9]
+
+
+def adjacency(m, off, sites):
+    A = np.zeros((m, m))
+    for i in range(m):
+        for a in range(off[i], off[i + 1]):
+            A[i, sites[a]] += 1
+    return A
+
+
+def free_boson_spectrum(eps, n):
+    """All many-body energies sum_k eps_k n_k with sum n_k = n (multiset sums)."""
+    return np.sort([sum(c) for c in itertools.combinations_with_replacement(eps, n)])
+
+
+@pytest.mark.parametrize("geom", [("chain", 5, True), ("chain", 6, False), ("square", (3, 3), True),
+                                  ("square", (2, 3), False), ("chain", 2, True)])
+def test_hopping_spectrum_closed_form(geom):
+    kind, size, per = geom
+    if kind == "chain":
+        m = size
+        off, sites = O.chain_neighbours(m, per)
+    else:
+        m = size[0] * size[1]
+        off, sites = O.square_neighbours(size[0], size[1], per)
+    n = 3
+    h = O.hamiltonian(m, n, off, sites)
+    H = h.dense(J=1.0, U=0.0)
+    assert np.array_equal(H, H.T)
+    eps = np.linalg.eigvalsh(-adjacency(m, off, sites))
+    want = free_boson_spectrum(eps, n)
+    got = np.linalg.eigvalsh(H)
+    assert np.allclose(got, want, atol=1e-10)
+    # reference hop convention doubles T
+    h2 = O.hamiltonian(m, n, off, sites, flags=O.CONV_REF_HOP)
+    assert np.array_equal(h2.val, 2.0 * h.val)
+
+
+def test_nnz_closed_form_and_traces():
+    for (m, n) in [(6, 6), (8, 8), (10, 9)]:
+        off, sites = O.chain_neighbours(m, True)
+        st, _, _ = O.basis(m, n)
+        h = O.hamiltonian(m, n, off, sites, states=st)
+        assert h.nnz == m * 2 * math.comb(n + m - 2, m - 1)
+        # Tr T^2 = sum over states and hops of (n_i + 1) n_j
+        tr2 = sum(float(v) ** 2 for v in h.val)
+        want = 0.0
+        for s in st:
+            for i in range(m):
+                for a in range(off[i], off[i + 1]):
+                    j = sites[a]
+                    if s[j] >= 1:
+                        want += (s[i] + 1) * s[j]
+        assert abs(tr2 - want) < 1e-9 * want
+    off, sites = O.square_neighbours(4, 4, True)
+    h = O.hamiltonian(16, 3, off, sites)
+    assert h.nnz == 16 * 4 * math.comb(3 + 16 - 2, 15)
+
+
+def test_single_particle_is_minus_adjacency():
+    m = 7
+    off, sites = O.chain_neighbours(m, True)
+    st, _, _ = O.basis(m, 1)
+    h = O.hamiltonian(m, 1, off, sites, states=st)
+    site = [int(np.argmax(s)) for s in st]
+    A = adjacency(m, off, sites)
+    H = h.dense(1.0, 0.0)
+    for r in range(m):
+        for c in range(m):
+            assert H[r, c] == -A[site[r], site[c]]
+
+
+def test_atomic_limit_spectrum():
+    m, n = 6, 6
+    h = O.hamiltonian(m, n, *O.chain_neighbours(m))
+    ev = np.linalg.eigvalsh(h.dense(J=0.0, U=1.0))
+    assert np.array_equal(ev, np.sort(h.diag))
+    assert ev[0] == 0.0 and ev[1] == 1.0  # unit filling: Mott state, first excitation U
+    assert np.sum(ev == 1.0) == m * (m - 1)
+
+
+@pytest.mark.parametrize("m,n,J,U", [(5, 5, 1.0, 1.0), (6, 6, 0.1, 1.0), (8, 8, 1.0, 1.0), (7, 5, 0.3, 1.0)])
+def test_oracle_lanczos_matches_dense(m, n, J, U):
+    h = O.hamiltonian(m, n, *O.chain_neighbours(m))
+    w = np.linalg.eigvalsh(h.dense(J, U))
+    k = 6
+    rc, ev, bd, it = O.lanczos(h, J, U, k)
+    assert rc == 0
+    want = O.distinct_levels(w, k)
+    assert np.allclose(ev, want, rtol=0, atol=1e-9 * max(1.0, abs(want[0])))
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build oracle/_ref not present")
+@pytest.mark.parametrize("m,n,J,U", [(5, 5, 1.0, 1.0), (8, 8, 1.0, 1.0), (8, 8, 0.1, 1.0)])
+def test_reference_spectra_and_eigen_on_oracle_h(m, n, J, U):
+    """The reference's own solvers (Spectra GenEigs via Op::IRLM_eigen, Eigen dense via
+    Op::exact_eigen) on the oracle's Hamiltonian agree with numpy and with the oracle Lanczos."""
+    R = O.ref()
+    h = O.hamiltonian(m, n, *O.chain_neighbours(m))
+    w = np.linalg.eigvalsh(h.dense(J, U))
+    if h.D <= 2000:
+        ex = np.zeros(h.D)
+        assert R.ref_exact_eigen(h.D, h.row_ptr, h.col, h.val, h.diag, J, U, ex) == 0
+        assert np.allclose(ex, w, atol=1e-10)
+    ir = np.zeros(10)
+    assert R.ref_irlm_eigen(h.D, h.row_ptr, h.col, h.val, h.diag, J, U, 10, ir) == 0
+    assert np.allclose(ir, w[:10], atol=1e-9)  # Spectra returns levels with multiplicity
+    rc, ev, _, _ = O.lanczos(h, J, U, 4)
+    assert np.allclose(ev, O.distinct_levels(ir, 4), atol=1e-9)
+
+
+def test_gap_ratio():
+    assert O.gap_ratio([0.0, 1.0, 3.0]) == pytest.approx(0.5)
+    assert O.gap_ratio([0.0, 1.0, 1.0, 2.0]) == pytest.approx(0.0)  # zero gaps give 0
+    ev = np.array([0.1, 0.7, 0.9, 2.0, 2.1, 3.5])
+    r = O.gap_ratio(ev)
+    assert O.gap_ratio(3.0 * ev - 7.0) == pytest.approx(r)  # positive affine invariance
+    assert O.gap_ratio(ev[::-1].copy()) == pytest.approx(r)  # sorted internally
+
+
+def test_start_vector_park_miller():
+    r = O.start_vector(3, 0)
+    # seed 0 -> 1; x1 = 16807, x2 = 16807^2 mod (2^31-1), x3 = 16807^3 mod (2^31-1)
+    p = 2 ** 31 - 1
+    xs = [16807 % p, 16807 ** 2 % p, 16807 ** 3 % p]
+    assert np.array_equal(r, np.array([x / p - 0.5 for x in xs]))
+
+
+def test_tridiag_ql():
+    rng = np.random.default_rng(1)
+    d = rng.standard_normal(30)
+    e = rng.standard_normal(29)
+    T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    w, z = O.tridiag_eig(d, e, vectors=True)
+    assert np.allclose(w, np.linalg.eigvalsh(T), atol=1e-12)
+    assert np.allclose(T @ z, z * w, atol=1e-10)
