@@ -2,17 +2,20 @@
 
 Workload (the single large solve of the benchmark configurations): periodic chain, m = 12 sites,
 n = 12 bosons (D = C(23, 11) = 1,352,078), H = -J sum (b_i^+ b_j + h.c.) + U/2 sum n(n-1) - mu N at
-J/U = 0.1, U = 1, mu = 0.5, lowest k = 4 eigenvalues, Spectra's seeded start vector.
+J/U = 0.1, U = 1, mu = 0.5, lowest k = 4 eigenvalues, seeded Park-Miller start vector.
 
-A step is one batched mixed-precision eigensolve: B solves of the sector (J/U = 0.1 and B-1 nearby
-points of a sweep, sharing one CSR pass per Lanczos step) on the resident CSR, including the FP64
-finish (Ritz vectors, Rayleigh quotients, residuals).  `value` = eigensolves per second.
-The plain FP64 GPU Lanczos (FP64 CSR and vectors, unbatched) is timed beside it, and the oracle's
-FP64 CPU Lanczos on the host cores gives the CPU baseline.
+A step is one batched mixed-precision eigensolve: B = 16 solves of the sector (J/U = 0.1 and 15
+neighbouring sweep points 0.1 * (1 + 0.01 b)) sharing one CSR pass per Lanczos step, on the resident
+CSR, including the FP64 finish (Ritz vectors, Rayleigh quotients with the exact matrix values,
+residuals).  value = eigensolves per second.  The solver layout is the lexicographic basis order
+(same spectrum as the reference order; the CSR is rebuilt in it, 3 ms).
+
+Beside it: the plain FP64 GPU Lanczos (FP64 CSR and FP64 vectors, one solve at a time, its fastest
+engine), the error of the mixed E0 against it, the dominant kernel's HBM roofline (HIP events on the
+library stream over the timed region), and the oracle's FP64 CPU Lanczos on the host cores.
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -25,31 +28,30 @@ import numpy as np  # noqa: E402
 
 import bhx  # noqa: E402
 
-HBM_PEAK = 8.0e12
+HBM_PEAK_GBS = 8000.0
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(m, n, J, U, k, seconds_target=15.0):
-    """The oracle's FP64 CPU Lanczos (OpenMP SpMV, full reorthogonalisation) on the same sector,
-    bounded: a fixed number of Lanczos steps, scaled to one full solve by the GPU step count."""
+def cpu_baseline(m, n, J, U, k, gpu_steps, sample_steps=80):
+    """The oracle's FP64 CPU Lanczos (full reorthogonalisation, OpenMP over this host's cores) on the
+    same sector: the first `sample_steps` steps are timed and the full solve is extrapolated linearly
+    to the step count of the FP64 GPU solve (linear is generous to the CPU: its per-step
+    reorthogonalisation grows with the step index)."""
     import oracle as O
 
     off, sites = O.chain_neighbours(m, True)
+    h = O.hamiltonian(m, n, off, sites, order=O.ORDER_ASCENDING)
     t0 = time.perf_counter()
-    h = O.hamiltonian(m, n, off, sites)
-    build_s = time.perf_counter() - t0
-    # time SpMVs (the per-step cost that dominates a Lanczos step at this size)
-    x = np.random.default_rng(0).standard_normal(h.D)
-    reps, t_sp = 0, 0.0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < seconds_target and reps < 200:
-        O.spmv(h, J, U, x)
-        reps += 1
-    t_sp = (time.perf_counter() - t0) / reps
-    return h, build_s, t_sp, reps
+    rc, ev, bd, it = O.lanczos(h, J, U, k, max_iter=sample_steps)
+    dt = time.perf_counter() - t0
+    per_solve = dt / it * gpu_steps
+    return {"value": 1.0 / per_solve, "unit": "eigensolves/s", "cores": os.cpu_count(), "kind": "port",
+            "sample": f"first {it} steps of the oracle's FP64 Lanczos (full reorthogonalisation, OpenMP CSR) on the "
+                      f"m=12 n=12 sector took {dt:.2f} s; extrapolated linearly to the {gpu_steps} steps of the FP64 "
+                      f"GPU solve"}
 
 
 def main():
@@ -57,34 +59,34 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (each solve is independent: "
-              "shard batches or requests across GPUs, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported. Solves are independent: a "
+              "multi-GPU run shards batches or requests across devices (see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
 
     m, n, J0, U, mu, k = 12, 12, 0.1, 1.0, 0.5, 4
     B = args.batch
-    Js = J0 * (1.0 + 0.02 * np.arange(B))  # J/U = 0.1 and neighbouring sweep points
+    Js = J0 * (1.0 + 0.01 * np.arange(B))
     ctx = bhx.Context(0)
     log("device:", bhx.device_name(0))
     off, sites = bhx.lattice("chain", m=m, periodic=True)
     t0 = time.perf_counter()
-    basis = ctx.basis(m, n)
+    basis = ctx.basis(m, n, bhx.ORDER_LEX)
     ham = basis.hamiltonian(off, sites)
     build_s = time.perf_counter() - t0
     log(f"build D={ham.D} nnz={ham.nnz} {build_s * 1e3:.1f} ms")
 
-    # FP64 reference solve (accuracy comparator and plain-FP64 throughput)
-    t0 = time.perf_counter()
+    # plain FP64 GPU Lanczos: FP64 CSR and vectors, unbatched (accuracy comparator and baseline rate)
     r64 = ham.lanczos(J0, U, k=k, prec=bhx.PREC_FP64)
-    fp64_s = time.perf_counter() - t0
+    reps64 = 3
     t0 = time.perf_counter()
-    r64 = ham.lanczos(J0, U, k=k, prec=bhx.PREC_FP64)
-    fp64_s = time.perf_counter() - t0
-    log(f"fp64 single solve {fp64_s * 1e3:.1f} ms iters={r64.iters[0]} E={r64.evals[0]}")
+    for _ in range(reps64):
+        r64 = ham.lanczos(J0, U, k=k, prec=bhx.PREC_FP64)
+    fp64_s = (time.perf_counter() - t0) / reps64
+    log(f"plain fp64: {fp64_s * 1e3:.1f} ms/solve, steps={r64.iters[0]}, E={r64.evals[0]}")
 
     for _ in range(args.warmup):
         ham.lanczos(Js, U, k=k, prec=bhx.PREC_MIXED)
@@ -96,33 +98,35 @@ def main():
         res = ham.lanczos(Js, U, k=k, prec=bhx.PREC_MIXED)
     dt = time.perf_counter() - t0
     ctx.set_timing(False)
-    cnt, tot = ctx.kernel_stats("lanczos_spmv")
+    k1_count, k1_total = ctx.kernel_stats("lanczos_spmv")
+    upd_count, upd_total = ctx.kernel_stats("lanczos_update")
     ms_step = dt / args.steps * 1e3
     value = B * args.steps / dt
     rel0 = abs(res.evals[0, 0] - r64.evals[0, 0]) / abs(r64.evals[0, 0])
-    log(f"mixed batched B={B}: {ms_step:.1f} ms/step, iters={res.iters.tolist()}, fp64 fallback={res.used_fp64.tolist()}, "
-        f"|dE0|/|E0|={rel0:.2e}")
+    log(f"mixed batched B={B}: {ms_step:.1f} ms/step ({value:.1f} solves/s), steps={res.iters.tolist()}, "
+        f"fp64 reruns={int(res.used_fp64.sum())}, |dE0|/|E0| vs fp64 = {rel0:.2e}")
 
-    # dominant kernel: the fused Lanczos CSR pass (K1).  Algorithmic bytes per launch (packed format):
-    # 4 B/entry + 4 B/row pointer + 4 B/row diagonal + B * (4 B x read + 8 B w write + 4 B v_{j-1} read)
+    # dominant kernel: the fused Lanczos operator pass K1 (row-lane engine at B = 16).  Algorithmic bytes per
+    # launch: packed matrix 4 B/entry + row pointers 4 B/row + diagonal 4 B/row, and per vector and row
+    # v_j read (4 B), v_{j-1} read (4 B), w written in FP64 (8 B).
     D, nnz = ham.D, ham.nnz
-    k1_bytes = 4.0 * nnz + 4.0 * (D + 1) + 4.0 * D + B * D * (4 + 8 + 4)
-    k1_ms = tot / max(cnt, 1)
+    k1_bytes = 4.0 * nnz + 4.0 * (D + 1) + 4.0 * D + B * D * (4 + 4 + 8)
+    k1_ms = k1_total / max(k1_count, 1)
     achieved = k1_bytes / (k1_ms * 1e-3) / 1e9
-    sp_ms, sp_bytes = ham.spmv_bench(bhx.PREC_MIXED, 1, 50)
-    sp64_ms, sp64_bytes = ham.spmv_bench(bhx.PREC_FP64, 1, 50)
-    log(f"K1 avg {k1_ms * 1e3:.1f} us -> {achieved:.0f} GB/s; SpMV mixed {sp_ms * 1e3:.1f} us "
-        f"({sp_bytes / sp_ms / 1e6:.0f} GB/s) fp64 {sp64_ms * 1e3:.1f} us ({sp64_bytes / sp64_ms / 1e6:.0f} GB/s)")
+    log(f"K1: {k1_count} launches, avg {k1_ms * 1e3:.1f} us, {k1_bytes / 1e6:.0f} MB -> {achieved:.0f} GB/s; "
+        f"K2 avg {upd_total / max(upd_count, 1) * 1e3:.1f} us")
+
+    spmv = {}
+    for prec, name in ((bhx.PREC_MIXED, "mixed"), (bhx.PREC_FP64, "fp64")):
+        for Bs in (1, B):
+            ms, by = ham.spmv_bench(prec, Bs, 30)
+            spmv[f"{name}_B{Bs}"] = {"us": ms * 1e3, "alg_GBs": by / ms / 1e6}
+    log("spmv:", json.dumps(spmv))
 
     cpu = None
     if not args.no_cpu:
-        h_o, cb_build, t_sp, reps = cpu_baseline(m, n, J0, U, k)
-        iters = int(r64.iters[0])
-        per_solve = iters * t_sp * 3.0  # SpMV + ~2 SpMV-equivalents of vector work per Lanczos step
-        cpu = {"value": 1.0 / per_solve, "unit": "eigensolves/s", "cores": os.cpu_count(), "kind": "port",
-               "sample": f"{reps} FP64 OpenMP CSR SpMVs of the m=12 n=12 sector ({t_sp * 1e3:.1f} ms each), scaled "
-                         f"to {iters} Lanczos steps x 3 SpMV-equivalents"}
-        log(f"cpu baseline: {cpu}")
+        cpu = cpu_baseline(m, n, J0, U, k, int(r64.iters[0]))
+        log("cpu baseline:", json.dumps(cpu))
 
     out = {
         "metric": "eigensolves/sec mixed-batched vs plain FP64 @|dE0|/|E0|<=1e-9",
@@ -136,14 +140,15 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32-stored/f64-reduced",
-        "data": "synthetic (seeded Park-Miller start vector; the Hamiltonian is exact)",
-        "config": {"workload": "chain m=12 n=12 periodic, J/U=0.1.., U=1, mu=0.5, k=4", "batch": B, "D": D,
-                   "nnz": nnz},
+        "data": "synthetic: exact Hamiltonian of the sector, seeded Park-Miller start vector",
+        "config": {"workload": "chain m=12 n=12 periodic, J/U=0.1..0.115 (16 points), U=1, mu=0.5, k=4",
+                   "batch": B, "D": D, "nnz": nnz, "layout": "lexicographic basis order"},
         "plain_fp64_eigensolves_per_s": 1.0 / fp64_s,
         "speedup_vs_plain_fp64": value * fp64_s,
-        "max_rel_err_E0_vs_fp64": rel0,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / (HBM_PEAK / 1e9), "traffic": None},
+        "rel_err_E0_vs_fp64": rel0,
+        "spmv": spmv,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -25,6 +25,7 @@ CONV_REF_HOP = 1
 CONV_REF_INTERACTION = 2
 ORDER_REFERENCE = 0
 ORDER_ASCENDING = 1
+ORDER_LEX = 2
 PREC_FP64 = 0
 PREC_MIXED = 1
 

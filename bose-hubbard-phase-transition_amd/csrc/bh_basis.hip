@@ -239,7 +239,10 @@ __global__ void k_basis_perm(int64_t D, const int32_t* __restrict__ sortperm, in
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= D) return;
     const int32_t v = sortperm[i];
-    if (order == BH_ORDER_REFERENCE) {
+    if (order == BH_ORDER_LEX) {
+        lex_of_pos[i] = static_cast<int32_t>(i);
+        pos_of_lex[i] = static_cast<int32_t>(i);
+    } else if (order == BH_ORDER_REFERENCE) {
         pos_of_lex[i] = v;
         lex_of_pos[v] = static_cast<int32_t>(i);
     } else {
@@ -276,7 +279,7 @@ int bh_basis_build(bh_ctx* ctx, int m, int n, int order, bh_basis** out) {
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     if (m < 1 || m > BH_MAX_SITES || n < 0 || n > 255)
         return bh_set_error(ctx, BH_EINVAL, "basis: need 1 <= m <= 32 and 0 <= n <= 255");
-    if (order != BH_ORDER_REFERENCE && order != BH_ORDER_ASCENDING)
+    if (order != BH_ORDER_REFERENCE && order != BH_ORDER_ASCENDING && order != BH_ORDER_LEX)
         return bh_set_error(ctx, BH_EINVAL, "basis: unknown order");
     const int64_t D = bh_binom_host(m + n - 1, n);
     if (D <= 0 || D > INT32_MAX) return bh_set_error(ctx, BH_ERANGE, "basis: dimension exceeds the int32 range");

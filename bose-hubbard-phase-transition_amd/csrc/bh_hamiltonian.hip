@@ -17,6 +17,9 @@
 #include <map>
 #include <memory>
 
+#include <cstdlib>
+#include <cstring>
+
 #include "bh_scan.hpp"
 
 constexpr int HB_BLOCK = 64;  // one wave per block: per-thread LDS row buffers stay small
@@ -129,9 +132,18 @@ __global__ void k_f32_to_f64(int64_t n, const float* __restrict__ a, double* __r
     if (k < n) b[k] = static_cast<double>(a[k]);
 }
 
-__global__ void k_row_max(int64_t D, const int32_t* __restrict__ row_ptr, int* __restrict__ out) {
+// out[0]: longest row; out[1]: largest row sum of |T| (FP32 bits, non-negative); out[2], out[3]: min, max of V
+__global__ void k_row_stats(int64_t D, const int32_t* __restrict__ row_ptr, const uint32_t* __restrict__ words, int cb,
+                            const double* __restrict__ table, const float* __restrict__ diag, int* __restrict__ out) {
     const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r < D) atomicMax(out, row_ptr[r + 1] - row_ptr[r]);
+    if (r >= D) return;
+    const uint32_t mask = (1u << cb) - 1u;
+    double s = 0.0;
+    for (int32_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k) s += table[words[k] & mask];
+    atomicMax(out, row_ptr[r + 1] - row_ptr[r]);
+    atomicMax(out + 1, __float_as_int(static_cast<float>(s) * 1.0001f));
+    atomicMin(out + 2, __float_as_int(diag[r]));
+    atomicMax(out + 3, __float_as_int(diag[r]));
 }
 
 #define BH_LAYOUT_DISPATCH(nw, bits, MACRO)             \
@@ -142,8 +154,56 @@ __global__ void k_row_max(int64_t D, const int32_t* __restrict__ row_ptr, int* _
         else MACRO(4, 8);                               \
     } while (0)
 
+int bh_ham_rowblocks(bh_ham* h, int B, const int32_t** rb, int64_t* nblocks) {
+    auto it = h->rowblocks.find(B);
+    if (it != h->rowblocks.end()) {
+        *rb = it->second.first;
+        *nblocks = it->second.second;
+        return BH_OK;
+    }
+    bh_ctx* ctx = h->ctx;
+    const int64_t cap = BH_STAGE_PRODUCTS_HOST / B;
+    if (static_cast<int64_t>(h->max_row) > cap) return bh_set_error(ctx, BH_ERANGE, "row longer than a staged block");
+    std::vector<int32_t> rp(static_cast<size_t>(h->D + 1));
+    BH_HIP(ctx, hipMemcpyAsync(rp.data(), h->row_ptr, rp.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+    BH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<int32_t> blocks;
+    blocks.push_back(0);
+    int64_t start = 0;
+    for (int64_t r = 0; r < h->D; ++r) {
+        if (rp[r + 1] - rp[start] > cap) {
+            blocks.push_back(static_cast<int32_t>(r));
+            start = r;
+        }
+    }
+    blocks.push_back(static_cast<int32_t>(h->D));
+    DevArray<int32_t> d;
+    BH_TRY(d.alloc(ctx, blocks.size()));
+    BH_HIP(ctx, hipMemcpyAsync(d.p, blocks.data(), blocks.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    BH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const int64_t nb = static_cast<int64_t>(blocks.size()) - 1;
+    h->bytes_rb += blocks.size() * 4;
+    h->rowblocks[B] = {d.release(), nb};
+    *rb = h->rowblocks[B].first;
+    *nblocks = nb;
+    return BH_OK;
+}
+
+int bh_engine_choice(int B, const bh_ham* h) {
+    static int env = [] {
+        const char* e = std::getenv("BHX_ENGINE");
+        return e ? std::atoi(e) : -1;
+    }();
+    const bool staged_ok = static_cast<int64_t>(h->max_row) * B <= BH_STAGE_PRODUCTS_HOST;
+    if (env == 0) return 0;
+    if (env == 1) return staged_ok ? 1 : 2;
+    if (env == 2) return 2;
+    return (B == 1 && staged_ok) ? 1 : 2;
+}
+
 static int free_ham(bh_ham* h) {
     bh_ctx* ctx = h->ctx;
+    for (auto& kv : h->rowblocks) bh_dev_free(ctx, kv.second.first, static_cast<size_t>(kv.second.second + 1) * 4);
     const size_t tab = (static_cast<size_t>(1) << h->code_bits);
     bh_dev_free(ctx, h->row_ptr, static_cast<size_t>(h->D + 1) * sizeof(int32_t));
     bh_dev_free(ctx, h->words, static_cast<size_t>(h->nnz ? h->nnz : 1) * sizeof(uint32_t));
@@ -262,7 +322,7 @@ int bh_hamiltonian_build(bh_ctx* ctx, const bh_basis* b, const int* nbr_off, con
     BH_TRY(diag.alloc(ctx, static_cast<size_t>(D)));
     BH_TRY(t64.alloc(ctx, tab));
     BH_TRY(t32.alloc(ctx, tab));
-    BH_TRY(dmax.alloc(ctx, 1));
+    BH_TRY(dmax.alloc(ctx, 4));
     if (E) {
         BH_HIP(ctx, hipMemcpyAsync(dei.p, ei.data(), E, hipMemcpyHostToDevice, s));
         BH_HIP(ctx, hipMemcpyAsync(dej.p, ej.data(), E, hipMemcpyHostToDevice, s));
@@ -296,13 +356,22 @@ int bh_hamiltonian_build(bh_ctx* ctx, const bh_basis* b, const int* nbr_off, con
 #undef LAUNCH_FILL
     }
     BH_HIP(ctx, hipGetLastError());
-    BH_HIP(ctx, hipMemsetAsync(dmax.p, 0, sizeof(int), s));
-    hipLaunchKernelGGL(k_row_max, dim3(static_cast<unsigned>((D + 255) / 256)), dim3(256), 0, s, D, row_ptr.p, dmax.p);
-    int mx = 0;
-    BH_HIP(ctx, hipMemcpyAsync(&mx, dmax.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    int stats[4] = {0, 0, 0x7f7fffff, 0};  // V >= 0, so FP32 bit patterns order like the values
+    BH_HIP(ctx, hipMemcpyAsync(dmax.p, stats, sizeof(stats), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_row_stats, dim3(static_cast<unsigned>((D + 255) / 256)), dim3(256), 0, s, D, row_ptr.p, words.p,
+                       cb, t64.p, diag.p, dmax.p);
+    BH_HIP(ctx, hipMemcpyAsync(stats, dmax.p, sizeof(stats), hipMemcpyDeviceToHost, s));
     BH_HIP(ctx, hipStreamSynchronize(s));
+    auto as_f = [](int v) {
+        float f;
+        std::memcpy(&f, &v, 4);
+        return static_cast<double>(f);
+    };
     h->nnz = nnz;
-    h->max_row = mx;
+    h->max_row = stats[0];
+    h->tmax = as_f(stats[1]);
+    h->vmin = as_f(stats[2]);
+    h->vmax = as_f(stats[3]);
     h->row_ptr = row_ptr.release();
     h->words = words.release();
     h->diag = diag.release();
@@ -325,7 +394,7 @@ int bh_ham_destroy(bh_ham* h) {
 
 int64_t bh_ham_dim(const bh_ham* h) { return h ? h->D : -1; }
 int64_t bh_ham_nnz(const bh_ham* h) { return h ? h->nnz : -1; }
-int64_t bh_ham_device_bytes(const bh_ham* h) { return h ? static_cast<int64_t>(h->bytes + h->bytes64) : -1; }
+int64_t bh_ham_device_bytes(const bh_ham* h) { return h ? static_cast<int64_t>(h->bytes + h->bytes64 + h->bytes_rb) : -1; }
 
 int bh_ham_prepare(bh_ham* h, int prec) {
     if (!h) return BH_EINVAL;

@@ -22,6 +22,13 @@ struct bh_ctx {
     std::recursive_mutex mu;
     int timing = 0;
     std::map<std::string, std::pair<int64_t, double>> kstats;
+    // grow-only device workspaces reused across calls (Krylov basis, locked vectors, scratch)
+    static constexpr int NWS = 8;
+    void* ws[NWS] = {};
+    size_t ws_bytes[NWS] = {};
+    // start vector cached in workspace slot 3 for (r0_D, r0_seed)
+    int64_t r0_D = -1;
+    uint64_t r0_seed = 0;
     // Ritz vectors kept by the last bh_lanczos(want_vectors=1): [B][k][D] doubles on the host
     std::vector<double> last_vecs;
     int last_vecs_B = 0, last_vecs_k = 0;
@@ -46,6 +53,8 @@ int bh_set_error(const bh_ctx* ctx, int code, const std::string& msg);
 // budgeted device allocation (bytes counted against ctx->budget)
 int bh_dev_alloc(bh_ctx* ctx, void** p, size_t bytes);
 void bh_dev_free(bh_ctx* ctx, void* p, size_t bytes);
+// workspace slot of at least `bytes` (kept by the context until bh_ctx_destroy)
+int bh_ws_get(bh_ctx* ctx, int slot, size_t bytes, void** p);
 
 // RAII device buffer
 template <typename T>
@@ -121,11 +130,25 @@ struct bh_ham {
     double* val64 = nullptr;      // [nnz] -sqrt(code)
     double* diag64 = nullptr;     // [D]
     size_t bytes = 0, bytes64 = 0;
+    // spectral bounds: V in [vmin, vmax]; the largest row sum of |T| (Gershgorin radius of T)
+    double vmin = 0.0, vmax = 0.0, tmax = 0.0;
+    std::vector<double> vlevels;  // distinct values of V ascending (host, filled on first use)
+    // row blocks of the LDS-staged engine, per batch width B: first row of each block, then D
+    std::map<int, std::pair<int32_t*, int64_t>> rowblocks;
+    size_t bytes_rb = 0;
 };
+
+// row blocks of at most (products / B) entries for batch width B (built once, cached in the handle);
+// returns BH_ERANGE when a row does not fit one block
+int bh_ham_rowblocks(bh_ham* h, int B, const int32_t** rb, int64_t* nblocks);
+// engine selection of the operator kernels: 0 = lanes per row (vector), 1 = LDS-staged row blocks,
+// 2 = one thread per (row, b); automatic unless BHX_ENGINE is set (read once).
+int bh_engine_choice(int B, const bh_ham* h);
 
 // ---- primes and packed occupation states ------------------------------------------------------------
 
 constexpr int BH_MAX_SITES = 32;
+constexpr int BH_STAGE_PRODUCTS_HOST = 2048;  // must equal BH_STAGE_PRODUCTS of the row-block engine
 extern const int bh_primes[BH_MAX_SITES];
 
 // A Fock state of up to NW*64/BITS sites packed in NW 64-bit words (BITS bits per site).

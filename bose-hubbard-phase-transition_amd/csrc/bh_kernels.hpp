@@ -99,3 +99,101 @@ __device__ __forceinline__ int64_t bh_xcd_tile(int64_t wg, int64_t nwg) {
     if (per == 0 || wg >= per * 8) return wg;  // tail workgroups keep their natural index
     return (wg % 8) * per + wg / 8;
 }
+
+// ---- row-block engine ------------------------------------------------------------------------------
+// A row block is a run of consecutive rows holding at most BH_STAGE_PRODUCTS / B entries.  A workgroup
+// takes one block at a time: phase 1 streams the block's entries (coalesced, independent loads and
+// gathers, several per lane in flight) and writes the products value * x[col * B + b] to LDS; phase 2
+// gives each (row, b) pair to one thread, which sums its row's products from LDS in column order and
+// hands the sum to the epilogue.  rb[] holds the first row of every block (and D at the end).
+constexpr int BH_STAGE_PRODUCTS = 2048;  // FP64 products per block (16 KB of LDS)
+
+template <typename XT, int B, int FMT, int WG, typename Epi>
+__device__ __forceinline__ void bh_rowblock_run(int64_t t0, int64_t t1, const int32_t* __restrict__ rb,
+                                                const int32_t* __restrict__ rp, const uint32_t* __restrict__ words,
+                                                const int32_t* __restrict__ col64, const double* __restrict__ val64,
+                                                const double* __restrict__ tab, int cb, const XT* __restrict__ x,
+                                                double* prod, Epi& epi) {
+    constexpr int CAP = BH_STAGE_PRODUCTS;  // products per block (entries * B)
+    constexpr int PER = CAP / WG;           // products per thread in phase 1
+    const uint32_t mask = (1u << cb) - 1u;
+    for (int64_t t = t0; t < t1; ++t) {
+        const int32_t r0 = rb[t], r1 = rb[t + 1];
+        const int32_t ks = rp[r0], ke = rp[r1];
+        const int np = (ke - ks) * B;
+        double v[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int i = threadIdx.x + u * WG;
+            v[u] = 0.0;
+            if (i < np) {
+                const int32_t k = ks + i / B;
+                const int bb = i % B;
+                if constexpr (FMT == BH_FMT_PACKED) {
+                    const uint32_t w = words[k];
+                    v[u] = tab[w & mask] * to_d(x[static_cast<int64_t>(w >> cb) * B + bb]);
+                } else {
+                    v[u] = val64[k] * to_d(x[static_cast<int64_t>(col64[k]) * B + bb]);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int i = threadIdx.x + u * WG;
+            if (i < np) prod[i] = v[u];
+        }
+        __syncthreads();
+        const int nq = (r1 - r0) * B;
+        for (int i = threadIdx.x; i < nq; i += WG) {
+            const int32_t r = r0 + i / B;
+            const int bb = i % B;
+            const int32_t a = rp[r] - ks, e = rp[r + 1] - ks;
+            double sum = 0.0;
+            for (int32_t q = a; q < e; ++q) sum += prod[q * B + bb];
+            epi(static_cast<int64_t>(r), bb, sum);
+        }
+        __syncthreads();
+    }
+}
+
+// ---- row-lane engine ---------------------------------------------------------------------------------
+// One thread per (row, b); B consecutive lanes share a row, so each entry's word is one broadcast load
+// and its gather one contiguous B-element segment.  The entry loop is unrolled by U with all loads of
+// an unrolled step issued before the multiply-adds.  Rows [r0, r1) of a workgroup are walked in
+// groups of WG / B.
+template <typename XT, int B, int FMT, int WG, int U, typename Epi>
+__device__ __forceinline__ void bh_rowlane_run(int64_t r0, int64_t r1, const int32_t* __restrict__ rp,
+                                               const uint32_t* __restrict__ words, const int32_t* __restrict__ col64,
+                                               const double* __restrict__ val64, const double* __restrict__ tab, int cb,
+                                               const XT* __restrict__ x, Epi& epi) {
+    const uint32_t mask = (1u << cb) - 1u;
+    const int bb = threadIdx.x % B;
+    for (int64_t base = r0; base < r1; base += WG / B) {
+        const int64_t r = base + threadIdx.x / B;
+        if (r >= r1) continue;
+        const int32_t ks = rp[r], ke = rp[r + 1];
+        double acc = 0.0;
+        for (int32_t k = ks; k < ke; k += U) {
+            double val[U];
+            int64_t c[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const bool ok = k + u < ke;
+                if constexpr (FMT == BH_FMT_PACKED) {
+                    const uint32_t w = ok ? words[k + u] : 0u;
+                    c[u] = static_cast<int64_t>(w >> cb);
+                    val[u] = ok ? tab[w & mask] : 0.0;
+                } else {
+                    c[u] = ok ? static_cast<int64_t>(col64[k + u]) : 0;
+                    val[u] = ok ? val64[k + u] : 0.0;
+                }
+            }
+            XT xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[u] = x[c[u] * B + bb];
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc += val[u] * to_d(xv[u]);
+        }
+        epi(r, bb, acc);
+    }
+}

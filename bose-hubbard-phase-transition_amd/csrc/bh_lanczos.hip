@@ -2,100 +2,45 @@
 //
 // Replaces Op::IRLM_eigen (src/operator.cpp:22-33), which runs Spectra's implicitly restarted
 // Arnoldi (GenEigsBase.h:386-441; nev, ncv = 2 nev + 1, SmallestReal, tol 1e-10) with a serial
-// sparse product per step.  Here:
-//   step j (device, no host round trip):
-//     K1  w = H v_j - beta_j v_{j-1}  (CSR pass fused with the diagonal and the recurrence),
-//         alpha_j = <v_j, w>           (FP64 partials, deterministic two-level grid reduction)
-//     KS  c_l = <y_l, w - alpha_j v_j> for the locked Ritz vectors y_l (only when some are locked)
-//     K2  w'' = w - alpha_j v_j - sum_l c_l y_l, stored as v_{j+1} (unnormalised, storage precision),
-//         beta_{j+1} = ||w''|| (FP64), scale_{j+1} = 1 / beta_{j+1}
-//   every `chunk` steps (host): FP64 analysis of T_j (Sturm bisection + inverse iteration), the
-//     Spectra convergence rule |beta_{j} s_last| < tol max(eps^(2/3), |theta|), and selective
-//     orthogonalisation: a Ritz pair whose bound drops below sqrt(eps_store) ||T|| is locked, its Ritz
-//     vector computed on the device, and later Lanczos vectors are kept orthogonal to it
-//     (Parlett & Scott's selective orthogonalisation).
-//   finish: Ritz vectors in FP64, Rayleigh quotients and residuals with the exact FP64 values of T.
-// Start vector: Spectra's convention, Park-Miller LCG r (SimpleRandom.h:30-96), v0 = H r / ||H r||
-// (Arnoldi.h:147-154).
-// Mixed precision: Krylov vectors and locked vectors stored FP32, all reductions and the W buffer
-// in FP64; the matrix values come exactly from the FP64 table of the packed format.
+// sparse product per step.  Here, per Lanczos step j, four launches on one stream and no host
+// round trip:
+//   K1  w = H v_j - beta_j v_{j-1} (one CSR pass for all B vectors, fused with the diagonal and the
+//       recurrence), written to W in FP64; per-workgroup partials of alpha_j = <v_j, w> and, for the
+//       locked Ritz vectors y_l, of <y_l, w> and <y_l, v_j>
+//   R1  the partials summed in a fixed order (one wave per value)
+//   K2  w'' = w - alpha_j v_j - sum_l c_l y_l with c_l = <y_l, w> - alpha_j <y_l, v_j>, stored as
+//       v_{j+1} (unnormalised, storage precision); partials of ||w''||^2
+//   R2  beta_{j+1} = ||w''||, scale_{j+1} = 1 / beta_{j+1}, step counter advanced
+// Cross-workgroup sums go through these separate launches instead of a last-workgroup reduction:
+// on a multi-XCD device each workgroup's device-scope release would write back its XCD's L2.
+// Every `chunk` steps the host analyses T_j in FP64 (Sturm bisection + inverse iteration), applies
+// Spectra's convergence rule |beta_j s_last| < tol max(eps^(2/3), |theta|), and performs selective
+// orthogonalisation (Parlett & Scott): a Ritz pair of a wanted level whose bound is below
+// sqrt(eps_store) ||T|| is locked, its Ritz vector formed on the device, and later Lanczos vectors
+// are kept orthogonal to it.  The finish forms the Ritz vectors in FP64 and evaluates Rayleigh
+// quotients and residuals with the exact FP64 values of T.
+// Start vector: Spectra's Park-Miller LCG r (SimpleRandom.h:30-96) normalised, v0 = r / ||r||.
+// Spectra projects it first (v0 = A r / ||A r||, Arnoldi.h:147-154); that projection removes the null
+// space of H, and at J = 0 the unit-filling Mott state has E = 0 exactly, so it is not applied.
+// Mixed precision: Krylov and locked vectors stored FP32, W and every reduction FP64; the matrix
+// values come exactly from the FP64 table of the packed format.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <thread>
 
 #include "bh_kernels.hpp"
 #include "bh_spmv.hpp"
 #include "bh_tridiag.hpp"
 
-constexpr int LZ_WG = 512;      // threads per workgroup of the Lanczos kernels (8 waves)
-constexpr int LZ_GRID_MAX = 1024;
-constexpr int LZ_GROUP = 32;    // workgroups per first-level reduction group
-constexpr int LZ_LMAX = 16;     // locked Ritz vectors per solve
-constexpr int LZ_NVMAX = 16;    // Ritz vectors formed per pass
-
-// ---- deterministic grid reduction -------------------------------------------------------------------
-// Each workgroup has written part[wg * nv + v] (v < nv).  The last workgroup of each group of
-// LZ_GROUP sums its group into gpart[group * nv + v]; the last group sums the groups into out[v].
-// Returns true in the single workgroup that wrote out[].  Summation order is fixed.
-__device__ bool lz_grid_reduce(const double* part, double* gpart, int nv, unsigned* counters, double* out) {
-    const int ngroups = (gridDim.x + LZ_GROUP - 1) / LZ_GROUP;
-    const int grp = blockIdx.x / LZ_GROUP;
-    unsigned* gc = counters + grp;
-    unsigned* top = counters + LZ_GRID_MAX / LZ_GROUP;
-    __shared__ int flag;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int members = min(LZ_GROUP, static_cast<int>(gridDim.x) - grp * LZ_GROUP);
-        const unsigned prev = __hip_atomic_fetch_add(gc, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        flag = (prev == static_cast<unsigned>(members - 1)) ? 1 : 0;
-    }
-    __syncthreads();
-    if (!flag) return false;
-    __threadfence();
-    const int w0 = grp * LZ_GROUP;
-    const int w1 = min(w0 + LZ_GROUP, static_cast<int>(gridDim.x));
-    for (int v = threadIdx.x; v < nv; v += blockDim.x) {
-        double s = 0.0;
-        for (int w = w0; w < w1; ++w) s += bh_coherent_load(part + static_cast<size_t>(w) * nv + v);
-        gpart[static_cast<size_t>(grp) * nv + v] = s;
-    }
-    if (threadIdx.x == 0) bh_reset_counter(gc);
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned prev = __hip_atomic_fetch_add(top, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        flag = (prev == static_cast<unsigned>(ngroups - 1)) ? 1 : 0;
-    }
-    __syncthreads();
-    if (!flag) return false;
-    __threadfence();
-    for (int v = threadIdx.x; v < nv; v += blockDim.x) {
-        double s = 0.0;
-        for (int g = 0; g < ngroups; ++g) s += bh_coherent_load(gpart + static_cast<size_t>(g) * nv + v);
-        out[v] = s;
-    }
-    if (threadIdx.x == 0) bh_reset_counter(top);
-    __syncthreads();
-    return true;
-}
-
-// per-workgroup sum of one value per vector b (lane b = lane % B), written to part[wg * nv + q*B + b]
-template <int B>
-__device__ __forceinline__ void lz_block_put(double v, int q, double* lds, double* part, int nv) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int o = B; o < 64; o <<= 1) v += __shfl_xor(v, o, 64);
-    if (lane < B) lds[wid * B + lane] = v;
-    __syncthreads();
-    if (threadIdx.x < B) {
-        double s = 0.0;
-        for (int w = 0; w < LZ_WG / 64; ++w) s += lds[w * B + threadIdx.x];
-        part[static_cast<size_t>(blockIdx.x) * nv + q * B + threadIdx.x] = s;
-    }
-    __syncthreads();
-}
+constexpr int LZ_WG = 512;         // threads per workgroup of the Lanczos kernels (8 waves)
+constexpr int LZ_GRID_MAX = 1024;  // workgroups that write partial sums
+constexpr int LZ_LMAX = 16;        // locked Ritz vectors per solve
+constexpr int LZ_NVMAX = 16;       // Ritz vectors formed per pass
+constexpr int LZ_RED_WG = 1024;    // reduction workgroup: 16 waves, one value per wave
 
 struct LzState {
     double* alpha;   // [maxit+1][B]
@@ -103,14 +48,11 @@ struct LzState {
     double* scale;   // [maxit+2][B]  1 / ||stored vector||
     double* J;       // [B]
     double* U;       // [B]
-    double* cl;      // [LZ_LMAX][B]
-    double* red;     // [(1 + LZ_LMAX) * B] reduction results
+    double* sums;    // [(1 + 2 LZ_LMAX) B] R1 results: alpha, <y_l, w>, <y_l, v_j>
     int* active;     // [B]
     int* nlock;      // [B]
     int* step;       // [1]
-    unsigned* cnt;   // [LZ_GRID_MAX / LZ_GROUP + 1]
-    double* part;    // [LZ_GRID_MAX][(1 + LZ_LMAX) * B]
-    double* gpart;   // [LZ_GRID_MAX / LZ_GROUP][(1 + LZ_LMAX) * B]
+    double* part;    // [LZ_GRID_MAX][nv] per-workgroup partials
 };
 
 template <typename XT>
@@ -128,168 +70,218 @@ struct LzArgs {
     int64_t vs;    // D * B
     double* W;     // [D][B]
     XT* Y;         // [LZ_LMAX][D][B] locked Ritz vectors (unit norm)
-    int64_t ntile;
-    int64_t chunk; // tiles per workgroup
-    int maxit;
     LzState st;
 };
 
-// K1: w = H v_j - beta_j v_{j-1}; alpha partials.  init: w = H r for the start vector r (double [D]).
-template <typename XT, int B, int G, int FMT, bool INIT>
-__global__ void __launch_bounds__(LZ_WG) k_lz_a(LzArgs<XT> a, const double* __restrict__ r0) {
-    constexpr int GB = G * B;
-    constexpr int SLOTS = LZ_WG / GB;
+// Per-workgroup sum of one value per vector b (b = lane % B), stored at part[wg * nv + q * B + b].
+template <int B>
+__device__ __forceinline__ void lz_block_put(double v, int q, double* lds, double* part, int nv) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = B; o < 64; o <<= 1) v += __shfl_xor(v, o, 64);
+    if (lane < B) lds[wid * B + lane] = v;
+    __syncthreads();
+    if (threadIdx.x < B) {
+        double s = 0.0;
+        for (int w = 0; w < LZ_WG / 64; ++w) s += lds[w * B + threadIdx.x];
+        part[static_cast<size_t>(blockIdx.x) * nv + q * B + threadIdx.x] = s;
+    }
+    __syncthreads();
+}
+
+// K1 epilogue state shared by both operator engines
+template <typename XT, int B, int FMT, int LM>
+struct K1Epi {
+    const LzArgs<XT>& a;
+    const XT* vj;
+    const XT* vjm;
+    double sj, bj, Jb, Ub;
+    int j, nl;
+    bool act;
+    double pa = 0.0;
+    double pyw[LM > 0 ? LM : 1] = {};
+    double pyv[LM > 0 ? LM : 1] = {};
+    __device__ __forceinline__ void operator()(int64_t r, int bb, double acc) {
+        if (!act) return;
+        const double d = a.diag32 ? static_cast<double>(a.diag32[r]) : a.diag64[r];
+        const double hx = (FMT == BH_FMT_PACKED) ? -acc : acc;
+        const int64_t idx = r * B + bb;
+        const double xr = to_d(vj[idx]) * sj;
+        double w = Jb * hx * sj + Ub * d * xr;
+        if (j > 0) w -= bj * to_d(vjm[idx]);
+        a.W[idx] = w;
+        pa += xr * w;
+        if constexpr (LM > 0) {
+#pragma unroll
+            for (int l = 0; l < LM; ++l) {
+                if (l < nl) {
+                    const double y = to_d(a.Y[static_cast<int64_t>(l) * a.vs + idx]);
+                    pyw[l] += y * w;
+                    pyv[l] += y * xr;
+                }
+            }
+        }
+    }
+};
+
+template <typename XT, int B, int FMT, int LM>
+__device__ __forceinline__ K1Epi<XT, B, FMT, LM> k1_setup(const LzArgs<XT>& a) {
+    const int b = threadIdx.x % B;  // fixed per thread: LZ_WG is a multiple of B
+    const int j = *a.st.step;
+    K1Epi<XT, B, FMT, LM> e{a,
+                            a.V + static_cast<int64_t>(j) * a.vs,
+                            a.V + static_cast<int64_t>(j > 0 ? j - 1 : 0) * a.vs,
+                            a.st.scale[j * B + b],
+                            j > 0 ? a.st.beta[j * B + b] * a.st.scale[(j - 1) * B + b] : 0.0,
+                            a.st.J[b],
+                            a.st.U[b],
+                            j,
+                            a.st.active[b] ? a.st.nlock[b] : 0,
+                            a.st.active[b] != 0};
+    return e;
+}
+
+template <typename XT, int B, int FMT, int LM>
+__device__ __forceinline__ void k1_finish(K1Epi<XT, B, FMT, LM>& e, double* lds, double* part) {
+    constexpr int NV = (1 + 2 * LM) * B;
+    lz_block_put<B>(e.pa, 0, lds, part, NV);
+    if constexpr (LM > 0) {
+#pragma unroll
+        for (int l = 0; l < LM; ++l) {
+            lz_block_put<B>(e.pyw[l], 1 + l, lds, part, NV);
+            lz_block_put<B>(e.pyv[l], 1 + LM + l, lds, part, NV);
+        }
+    }
+}
+
+// K1 on the LDS-staged row-block engine (B = 1)
+template <typename XT, int B, int FMT, int LM>
+__global__ void __launch_bounds__(LZ_WG) k_lz_a_rb(LzArgs<XT> a, const int32_t* __restrict__ rb, int64_t nblocks) {
+    extern __shared__ double smem[];
+    double* tab = smem;
+    double* prod = smem + (FMT == BH_FMT_PACKED ? (1 << a.cb) : 0);
+    __shared__ double lds[LZ_WG / 64 * B];
+    if constexpr (FMT == BH_FMT_PACKED) {
+        for (int i = threadIdx.x; i < (1 << a.cb); i += LZ_WG) tab[i] = a.table[i];
+        __syncthreads();
+    }
+    auto e = k1_setup<XT, B, FMT, LM>(a);
+    const int64_t per = (nblocks + gridDim.x - 1) / gridDim.x;
+    const int64_t L = bh_xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t t0 = min(L * per, nblocks), t1 = min(t0 + per, nblocks);
+    bh_rowblock_run<XT, B, FMT, LZ_WG>(t0, t1, rb, a.row_ptr, a.words, a.col64, a.val64, tab, a.cb, e.vj, prod, e);
+    k1_finish<XT, B, FMT, LM>(e, lds, a.st.part);
+}
+
+// K1 on the row-lane engine (one thread per (row, b), contiguous row ranges per workgroup)
+template <typename XT, int B, int FMT, int LM>
+__global__ void __launch_bounds__(LZ_WG) k_lz_a_rl(LzArgs<XT> a, int64_t rows_per_wg) {
     extern __shared__ double tab[];
     __shared__ double lds[LZ_WG / 64 * B];
     if constexpr (FMT == BH_FMT_PACKED) {
         for (int i = threadIdx.x; i < (1 << a.cb); i += LZ_WG) tab[i] = a.table[i];
         __syncthreads();
     }
-    const int lane = threadIdx.x;
-    const int slot = lane / GB;
-    const int g = (lane % GB) / B;
-    const int b = lane % B;
-    const int j = INIT ? 0 : *a.st.step;
-    const XT* vj = a.V + static_cast<int64_t>(j) * a.vs;
-    const XT* vjm = a.V + static_cast<int64_t>(j > 0 ? j - 1 : 0) * a.vs;
-    const double sj = INIT ? 1.0 : a.st.scale[j * B + b];
-    const double bj = (!INIT && j > 0) ? a.st.beta[j * B + b] * a.st.scale[(j - 1) * B + b] : 0.0;
-    const bool act = a.st.active[b] != 0;
-    const double Jb = a.st.J[b], Ub = a.st.U[b];
+    auto e = k1_setup<XT, B, FMT, LM>(a);
     const int64_t L = bh_xcd_tile(blockIdx.x, gridDim.x);
-    const int64_t t0 = L * a.chunk, t1 = min(t0 + a.chunk, a.ntile);
-    double pa = 0.0;
-    for (int64_t t = t0; t < t1; ++t) {
-        const int64_t r = t * SLOTS + slot;
-        double acc = 0.0;
-        if (r < a.D && act) {
-            if constexpr (INIT) {
-                // start vector shared by all b: x[c] = r0[c]
-                if constexpr (FMT == BH_FMT_PACKED) {
-                    const uint32_t mask = (1u << a.cb) - 1u;
-                    for (int32_t k = a.row_ptr[r] + g; k < a.row_ptr[r + 1]; k += G) {
-                        const uint32_t w = a.words[k];
-                        acc += tab[w & mask] * r0[w >> a.cb];
-                    }
-                } else {
-                    for (int32_t k = a.row_ptr[r] + g; k < a.row_ptr[r + 1]; k += G) acc += a.val64[k] * r0[a.col64[k]];
-                }
-            } else {
-                acc = bh_row_partial<XT, B, G, FMT>(a.row_ptr[r], a.row_ptr[r + 1], g, b, a.words, a.col64, a.val64, tab,
-                                                     a.cb, vj);
-            }
-        }
-        acc = bh_group_sum<B, G>(acc);
-        if (g == 0 && r < a.D && act) {
-            const double d = a.diag32 ? static_cast<double>(a.diag32[r]) : a.diag64[r];
-            const double hx = (FMT == BH_FMT_PACKED) ? -acc : acc;
-            if constexpr (INIT) {
-                a.W[r * B + b] = Jb * hx + Ub * d * r0[r];
-            } else {
-                const double xr = to_d(vj[r * B + b]) * sj;
-                double w = Jb * hx * sj + Ub * d * xr;
-                if (j > 0) w -= bj * to_d(vjm[r * B + b]);
-                a.W[r * B + b] = w;
-                pa += xr * w;
-            }
-        }
-    }
-    if constexpr (!INIT) {
-        const int nv = B;
-        lz_block_put<B>(pa, 0, lds, a.st.part, nv);
-        if (lz_grid_reduce(a.st.part, a.st.gpart, nv, a.st.cnt, a.st.red)) {
-            if (threadIdx.x < B && a.st.active[threadIdx.x]) a.st.alpha[j * B + threadIdx.x] = a.st.red[threadIdx.x];
-        }
-    }
+    const int64_t r0 = min(L * rows_per_wg, a.D), r1 = min(r0 + rows_per_wg, a.D);
+    if (e.act) bh_rowlane_run<XT, B, FMT, LZ_WG, 4>(r0, r1, a.row_ptr, a.words, a.col64, a.val64, tab, a.cb, e.vj, e);
+    k1_finish<XT, B, FMT, LM>(e, lds, a.st.part);
 }
 
-// KS: c_l = <y_l, w - alpha_j v_j> for l < nlock[b]
+// K2: v_{j+1} = w - alpha v_j - sum_l c_l y_l (stored), partials of ||v_{j+1}||^2.  Block 0 records alpha_j.
 template <typename XT, int B>
-__global__ void __launch_bounds__(LZ_WG) k_lz_so(LzArgs<XT> a, int lmax_used) {
+__global__ void __launch_bounds__(LZ_WG) k_lz_b(LzArgs<XT> a, int lm) {
     __shared__ double lds[LZ_WG / 64 * B];
     const int b = threadIdx.x % B;
     const int j = *a.st.step;
     const XT* vj = a.V + static_cast<int64_t>(j) * a.vs;
-    const double sj = a.st.scale[j * B + b];
-    const double al = a.st.alpha[j * B + b];
-    const int nl = a.st.active[b] ? a.st.nlock[b] : 0;
-    double acc[LZ_LMAX];
-#pragma unroll
-    for (int l = 0; l < LZ_LMAX; ++l) acc[l] = 0.0;
-    constexpr int ROWS = LZ_WG / B;
-    const int64_t L = bh_xcd_tile(blockIdx.x, gridDim.x);
-    const int64_t t0 = L * a.chunk, t1 = min(t0 + a.chunk, a.ntile * (LZ_WG / B) / ROWS);
-    for (int64_t t = t0; t < t1; ++t) {
-        const int64_t r = t * ROWS + threadIdx.x / B;
-        if (r < a.D && nl > 0) {
-            const int64_t idx = r * B + b;
-            const double w = a.W[idx] - al * to_d(vj[idx]) * sj;
-#pragma unroll
-            for (int l = 0; l < LZ_LMAX; ++l)
-                if (l < nl) acc[l] += to_d(a.Y[static_cast<int64_t>(l) * a.vs + idx]) * w;
-        }
-    }
-    const int nv = lmax_used * B;
-    for (int l = 0; l < lmax_used; ++l) lz_block_put<B>(acc[l], l, lds, a.st.part, nv);
-    if (lz_grid_reduce(a.st.part, a.st.gpart, nv, a.st.cnt, a.st.red)) {
-        for (int i = threadIdx.x; i < nv; i += LZ_WG) a.st.cl[i] = a.st.red[i];
-    }
-}
-
-// K2: v_{j+1} = w - alpha v_j - sum_l c_l y_l (stored), beta_{j+1} = ||v_{j+1}||.  INIT stores V[0].
-template <typename XT, int B, bool INIT>
-__global__ void __launch_bounds__(LZ_WG) k_lz_b(LzArgs<XT> a, int use_so) {
-    __shared__ double lds[LZ_WG / 64 * B];
-    const int b = threadIdx.x % B;
-    const int j = INIT ? -1 : *a.st.step;
-    const XT* vj = a.V + static_cast<int64_t>(j < 0 ? 0 : j) * a.vs;
     XT* vn = a.V + static_cast<int64_t>(j + 1) * a.vs;
     const bool act = a.st.active[b] != 0;
-    const double sj = INIT ? 0.0 : a.st.scale[j * B + b];
-    const double al = INIT ? 0.0 : a.st.alpha[j * B + b];
-    const int nl = (use_so && act) ? a.st.nlock[b] : 0;
-    double c[LZ_LMAX];
-#pragma unroll
-    for (int l = 0; l < LZ_LMAX; ++l) c[l] = (l < nl) ? a.st.cl[l * B + b] : 0.0;
+    const double sj = a.st.scale[j * B + b];
+    const double al = a.st.sums[b];
+    const int nl = act ? min(a.st.nlock[b], lm) : 0;
+    if (blockIdx.x == 0 && threadIdx.x < B && act) a.st.alpha[j * B + b] = al;
     constexpr int ROWS = LZ_WG / B;
-    const int64_t ntile_b = (a.D + ROWS - 1) / ROWS;
-    const int64_t per = (ntile_b + gridDim.x - 1) / gridDim.x;
+    constexpr int UN = 4;  // rows per thread in flight
+    const int64_t ntile = (a.D + ROWS - 1) / ROWS;
+    const int64_t per = (ntile + gridDim.x - 1) / gridDim.x;
     const int64_t L = bh_xcd_tile(blockIdx.x, gridDim.x);
-    const int64_t t0 = L * per, t1 = min(t0 + per, ntile_b);
+    const int64_t t0 = min(L * per, ntile), t1 = min(t0 + per, ntile);
     double pn = 0.0;
-    for (int64_t t = t0; t < t1; ++t) {
-        const int64_t r = t * ROWS + threadIdx.x / B;
-        if (r < a.D && act) {
-            const int64_t idx = r * B + b;
-            double w = a.W[idx];
-            if (!INIT) {
-                w -= al * to_d(vj[idx]) * sj;
+    if (act) {
+        for (int64_t t = t0; t < t1; t += UN) {
+            double w[UN], v[UN];
+            int64_t idx[UN];
+            bool ok[UN];
 #pragma unroll
-                for (int l = 0; l < LZ_LMAX; ++l)
-                    if (l < nl) w -= c[l] * to_d(a.Y[static_cast<int64_t>(l) * a.vs + idx]);
+            for (int u = 0; u < UN; ++u) {
+                const int64_t r = (t + u) * ROWS + threadIdx.x / B;
+                ok[u] = (t + u < t1) && r < a.D;
+                idx[u] = ok[u] ? r * B + b : 0;
+                w[u] = ok[u] ? a.W[idx[u]] : 0.0;
+                v[u] = ok[u] ? to_d(vj[idx[u]]) : 0.0;
             }
-            const XT wr = static_cast<XT>(w);
-            vn[idx] = wr;
-            const double wd = to_d(wr);
-            pn += wd * wd;
+#pragma unroll
+            for (int u = 0; u < UN; ++u) w[u] -= al * v[u] * sj;
+            for (int l = 0; l < nl; ++l) {
+                const XT* yl = a.Y + static_cast<int64_t>(l) * a.vs;
+                const double cl = a.st.sums[(1 + l) * B + b] - al * a.st.sums[(1 + lm + l) * B + b];
+#pragma unroll
+                for (int u = 0; u < UN; ++u)
+                    if (ok[u]) w[u] -= cl * to_d(yl[idx[u]]);
+            }
+#pragma unroll
+            for (int u = 0; u < UN; ++u) {
+                if (!ok[u]) continue;
+                const XT wr = static_cast<XT>(w[u]);
+                vn[idx[u]] = wr;
+                const double wd = to_d(wr);
+                pn += wd * wd;
+            }
         }
     }
     lz_block_put<B>(pn, 0, lds, a.st.part, B);
-    if (lz_grid_reduce(a.st.part, a.st.gpart, B, a.st.cnt, a.st.red)) {
-        if (threadIdx.x < B) {
-            const int bb = threadIdx.x;
-            if (a.st.active[bb]) {
-                const double nrm = std::sqrt(a.st.red[bb]);
-                if (INIT) {
-                    a.st.beta[bb] = 0.0;
-                    a.st.scale[bb] = nrm > 0.0 ? 1.0 / nrm : 0.0;
-                } else {
-                    a.st.beta[(j + 1) * B + bb] = nrm;
-                    a.st.scale[(j + 1) * B + bb] = nrm > 0.0 ? 1.0 / nrm : 0.0;
-                }
+}
+
+// V[0][r][b] = r0[r] in storage precision (start vector shared by all b)
+template <typename XT, int B>
+__global__ void k_lz_init(XT* __restrict__ V0, const double* __restrict__ r0, int64_t D) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx < D * B) V0[idx] = static_cast<XT>(r0[idx / B]);
+}
+
+// Reductions of per-workgroup partials, one wave per value in a fixed order (deterministic):
+//   mode 0: out[v] = sum_wg part[wg * nv + v]; values spread over the launch's workgroups
+//   mode 1: v = b: beta_{j+1} = sqrt(sum), scale_{j+1} = 1/beta (active runs), then step = j + 1;
+//           launched as a single workgroup so that the step write follows every read of it
+constexpr int LZ_RED_PLAIN = 0;
+constexpr int LZ_RED_BETA = 1;
+
+template <int B>
+__global__ void __launch_bounds__(LZ_RED_WG) k_lz_reduce(LzState st, const double* __restrict__ part, int nwg, int nv,
+                                                         int mode, double* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int waves = LZ_RED_WG / 64;
+    const int j = mode == LZ_RED_BETA ? *st.step : 0;
+    for (int v = blockIdx.x * waves + (threadIdx.x >> 6); v < nv; v += gridDim.x * waves) {
+        double s = 0.0;
+        for (int w = lane; w < nwg; w += 64) s += part[static_cast<size_t>(w) * nv + v];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (lane == 0) {
+            if (mode == LZ_RED_PLAIN) {
+                out[v] = s;
+            } else if (st.active[v]) {
+                const double nrm = std::sqrt(s);
+                st.beta[(j + 1) * B + v] = nrm;
+                st.scale[(j + 1) * B + v] = nrm > 0.0 ? 1.0 / nrm : 0.0;
             }
         }
-        if (threadIdx.x == 0) *a.st.step = j + 1;
+    }
+    if (mode == LZ_RED_BETA) {
+        __syncthreads();  // every thread of the (single) workgroup has read step
+        if (threadIdx.x == 0) *st.step = j + 1;
     }
 }
 
@@ -305,32 +297,37 @@ __global__ void __launch_bounds__(256) k_lz_ritz(const XT* __restrict__ V, int64
     double acc[LZ_NVMAX];
 #pragma unroll
     for (int i = 0; i < LZ_NVMAX; ++i) acc[i] = 0.0;
-    for (int t = 0; t < nj; ++t) {
-        const double v = to_d(V[static_cast<int64_t>(t) * vs + idx]);
-        const double* ct = C + (static_cast<int64_t>(t) * nv) * B + b;
+    constexpr int UT = 8;  // Krylov vectors read per unrolled step (independent loads in flight)
+    for (int t0 = 0; t0 < nj; t0 += UT) {
+        double v[UT];
 #pragma unroll
-        for (int i = 0; i < LZ_NVMAX; ++i)
-            if (i < nv) acc[i] += v * ct[i * B];
+        for (int u = 0; u < UT; ++u) v[u] = (t0 + u < nj) ? to_d(V[static_cast<int64_t>(t0 + u) * vs + idx]) : 0.0;
+#pragma unroll
+        for (int u = 0; u < UT; ++u) {
+            if (t0 + u >= nj) break;
+            const double* ct = C + (static_cast<int64_t>(t0 + u) * nv) * B + b;
+#pragma unroll
+            for (int i = 0; i < LZ_NVMAX; ++i)
+                if (i < nv) acc[i] += v[u] * ct[i * B];
+        }
     }
 #pragma unroll
     for (int i = 0; i < LZ_NVMAX; ++i)
         if (i < nv) out[static_cast<int64_t>(i) * os + idx] = static_cast<OT>(acc[i]);
 }
 
-// per-(i, b) reductions over rows of [nv][D][B] arrays:
-//   mode 0: out[(2i)B+b] = <y,y>, out[(2i+1)B+b] = <y,Hy>;  mode 1: out[iB+b] = ||Hy - theta y||^2;
-//   mode 2: out[iB+b] = <y,y> (y in storage precision)
+// per-(i, b) partial sums over rows of [nv][D][B] arrays (reduced by k_lz_reduce mode 0):
+//   mode 0: value 2i -> <y,y>, 2i+1 -> <y,Hy>;  mode 1: value i -> ||Hy - theta y||^2;  mode 2: i -> <y,y>
 template <typename YT, int B>
 __global__ void __launch_bounds__(LZ_WG) k_lz_dots(const YT* __restrict__ Y, const double* __restrict__ HY, int64_t D, int nv,
-                                                   int mode, const double* __restrict__ theta, double* part, double* gpart,
-                                                   unsigned* cnt, double* out) {
+                                                   int mode, const double* __restrict__ theta, double* part) {
     __shared__ double lds[LZ_WG / 64 * B];
     const int b = threadIdx.x % B;
     constexpr int ROWS = LZ_WG / B;
     const int64_t vs = D * B;
     const int64_t ntile = (D + ROWS - 1) / ROWS;
     const int64_t per = (ntile + gridDim.x - 1) / gridDim.x;
-    const int64_t t0 = blockIdx.x * per, t1 = min(t0 + per, ntile);
+    const int64_t t0 = min(static_cast<int64_t>(blockIdx.x) * per, ntile), t1 = min(t0 + per, ntile);
     const int nq = mode == 0 ? 2 * nv : nv;
     for (int i = 0; i < nv; ++i) {
         double s0 = 0.0, s1 = 0.0;
@@ -358,7 +355,6 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_dots(const YT* __restrict__ Y, con
             lz_block_put<B>(s0, i, lds, part, nq * B);
         }
     }
-    (void)lz_grid_reduce(part, gpart, nq * B, cnt, out);
 }
 
 // y *= f[i*B + b] over [nv][D][B]
@@ -375,6 +371,27 @@ __global__ void k_lz_scale(YT* __restrict__ Y, int64_t vs, int nv, const double*
 
 namespace {
 
+int lz_debug() {
+    static int d = [] {
+        const char* e = std::getenv("BHX_DEBUG");
+        return e ? std::atoi(e) : 0;
+    }();
+    return d;
+}
+
+// start vector of the solver (Park-Miller sequence x <- 16807 x mod (2^31 - 1), seed 0 -> 1, values
+// x / (2^31 - 1) - 0.5), shared by every run
+std::vector<double> lz_start_vector(int64_t D, uint64_t seed) {
+    std::vector<double> r(static_cast<size_t>(D));
+    const uint64_t p = 2147483647ull;
+    uint64_t x = seed ? (seed & p) : 1;
+    for (int64_t i = 0; i < D; ++i) {
+        x = (16807ull * x) % p;
+        r[i] = static_cast<double>(x) / static_cast<double>(p) - 0.5;
+    }
+    return r;
+}
+
 struct RunHost {
     std::vector<double> alpha, beta;     // alpha[t], beta[t] (beta[0] = 0)
     std::vector<double> lock_theta;      // locked Ritz values
@@ -384,6 +401,7 @@ struct RunHost {
     std::vector<double> theta;           // wanted levels (Ritz values of T)
     std::vector<std::vector<double>> s;  // their eigenvectors of T
     double tnorm = 0.0;
+    double tnorm0 = 0.0;                 // ||T|| at the first analysis (divergence guard)
     std::vector<double> all_theta;       // lowest Ritz values of the last analysis (for gaps)
 };
 
@@ -435,23 +453,31 @@ static int lanczos_impl(bh_ctx* ctx, bh_ham* h, const double* Jh, const double* 
     const double eps23 = std::pow(2.220446049250313e-16, 2.0 / 3.0);
     const int64_t vs = D * B;
 
-    // ---- device buffers ----
-    DevArray<XT> V, Y;
-    DevArray<double> W, st_d, part, gpart, r0;
+    // ---- device buffers: the Krylov basis, W and the locked vectors live in context workspaces ----
+    struct WsView {
+        XT* p;
+    } V{nullptr}, Y{nullptr};
+    struct WsViewD {
+        double* p;
+    } W{nullptr};
+    {
+        void* q = nullptr;
+        BH_TRY(bh_ws_get(ctx, 0, static_cast<size_t>(maxit + 1) * vs * sizeof(XT), &q));
+        V.p = static_cast<XT*>(q);
+        BH_TRY(bh_ws_get(ctx, 1, static_cast<size_t>(vs) * sizeof(double), &q));
+        W.p = static_cast<double*>(q);
+        BH_TRY(bh_ws_get(ctx, 2, static_cast<size_t>(LZ_LMAX) * vs * sizeof(XT), &q));
+        Y.p = static_cast<XT*>(q);
+    }
+    DevArray<double> st_d, part;
     DevArray<int> st_i;
-    DevArray<unsigned> cnt;
-    BH_TRY(dalloc(ctx, V, static_cast<size_t>(maxit + 1) * vs));
-    BH_TRY(dalloc(ctx, W, static_cast<size_t>(vs)));
-    BH_TRY(dalloc(ctx, Y, static_cast<size_t>(LZ_LMAX) * vs));
     const size_t n_alpha = static_cast<size_t>(maxit + 1) * B, n_beta = static_cast<size_t>(maxit + 2) * B;
-    const size_t nd = n_alpha + 2 * n_beta + 2 * B + LZ_LMAX * B + (1 + LZ_LMAX) * B * 2;
+    const size_t n_sums = static_cast<size_t>(1 + 2 * LZ_LMAX) * B;
+    const size_t nd = n_alpha + 2 * n_beta + 2 * B + n_sums;
     BH_TRY(dalloc(ctx, st_d, nd));
     BH_TRY(dalloc(ctx, st_i, static_cast<size_t>(2 * B + 1)));
-    const int nvmax = std::max((1 + LZ_LMAX) * B, 2 * LZ_NVMAX * B);
+    const int nvmax = std::max(static_cast<int>(n_sums), 2 * LZ_NVMAX * B);
     BH_TRY(dalloc(ctx, part, static_cast<size_t>(LZ_GRID_MAX) * nvmax));
-    BH_TRY(dalloc(ctx, gpart, static_cast<size_t>(LZ_GRID_MAX / LZ_GROUP) * nvmax));
-    BH_TRY(dalloc(ctx, cnt, LZ_GRID_MAX / LZ_GROUP + 1));
-    BH_TRY(dalloc(ctx, r0, static_cast<size_t>(D)));
 
     LzState st;
     st.alpha = st_d.p;
@@ -459,14 +485,11 @@ static int lanczos_impl(bh_ctx* ctx, bh_ham* h, const double* Jh, const double* 
     st.scale = st.beta + n_beta;
     st.J = st.scale + n_beta;
     st.U = st.J + B;
-    st.cl = st.U + B;
-    st.red = st.cl + LZ_LMAX * B;
+    st.sums = st.U + B;
     st.active = st_i.p;
     st.nlock = st_i.p + B;
     st.step = st_i.p + 2 * B;
-    st.cnt = cnt.p;
     st.part = part.p;
-    st.gpart = gpart.p;
 
     std::vector<double> Jb(B, 0.0), Ub(B, 0.0);
     std::vector<int> act(B, 0), nlock(B, 0);
@@ -476,28 +499,34 @@ static int lanczos_impl(bh_ctx* ctx, bh_ham* h, const double* Jh, const double* 
         act[b] = 1;
     }
     BH_HIP(ctx, hipMemsetAsync(st_d.p, 0, nd * sizeof(double), s));
-    BH_HIP(ctx, hipMemsetAsync(cnt.p, 0, cnt.bytes(), s));
+    BH_HIP(ctx, hipMemsetAsync(st_i.p, 0, st_i.bytes(), s));
     BH_HIP(ctx, hipMemcpyAsync(st.J, Jb.data(), B * 8, hipMemcpyHostToDevice, s));
     BH_HIP(ctx, hipMemcpyAsync(st.U, Ub.data(), B * 8, hipMemcpyHostToDevice, s));
     BH_HIP(ctx, hipMemcpyAsync(st.active, act.data(), B * 4, hipMemcpyHostToDevice, s));
-    BH_HIP(ctx, hipMemcpyAsync(st.nlock, nlock.data(), B * 4, hipMemcpyHostToDevice, s));
+    // start vector: cached on the device per (D, seed); V[0] = r in storage precision, scale_0 = 1/||V[0]||
     {
-        std::vector<double> r(static_cast<size_t>(D));
-        const uint64_t p = 2147483647ull;
-        uint64_t x = o.seed ? (o.seed & p) : 1;
-        for (int64_t i = 0; i < D; ++i) {
-            x = (16807ull * x) % p;
-            r[i] = static_cast<double>(x) / static_cast<double>(p) - 0.5;
+        void* q = nullptr;
+        BH_TRY(bh_ws_get(ctx, 3, static_cast<size_t>(D) * sizeof(double), &q));
+        double* dr = static_cast<double*>(q);
+        std::vector<double> r = lz_start_vector(D, o.seed);
+        if (ctx->r0_D != D || ctx->r0_seed != o.seed) {
+            BH_HIP(ctx, hipMemcpyAsync(dr, r.data(), D * 8, hipMemcpyHostToDevice, s));
+            BH_HIP(ctx, hipStreamSynchronize(s));
+            ctx->r0_D = D;
+            ctx->r0_seed = o.seed;
         }
-        BH_HIP(ctx, hipMemcpyAsync(r0.p, r.data(), D * 8, hipMemcpyHostToDevice, s));
-        BH_HIP(ctx, hipStreamSynchronize(s));  // r is a local
+        double nrm = 0.0;
+        for (int64_t i = 0; i < D; ++i) {
+            const double v = static_cast<double>(static_cast<XT>(r[i]));
+            nrm += v * v;
+        }
+        std::vector<double> sc(B, nrm > 0 ? 1.0 / std::sqrt(nrm) : 0.0);
+        BH_HIP(ctx, hipMemcpyAsync(st.scale, sc.data(), B * 8, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL((k_lz_init<XT, B>), dim3(static_cast<unsigned>((vs + 255) / 256)), dim3(256), 0, s, V.p, dr, D);
+        BH_HIP(ctx, hipGetLastError());
+        BH_HIP(ctx, hipStreamSynchronize(s));  // sc and r are locals
     }
 
-    const int G = std::max(1, std::min(bh_pick_lanes(h, B), 64 / B));
-    const int64_t slots = LZ_WG / (G * B);
-    const int64_t ntile = (D + slots - 1) / slots;
-    const int grid = static_cast<int>(std::min<int64_t>(ntile, LZ_GRID_MAX));
-    const int64_t tchunk = (ntile + grid - 1) / grid;
     LzArgs<XT> a{D,
                  h->row_ptr,
                  h->words,
@@ -511,53 +540,61 @@ static int lanczos_impl(bh_ctx* ctx, bh_ham* h, const double* Jh, const double* 
                  vs,
                  W.p,
                  Y.p,
-                 ntile,
-                 tchunk,
-                 maxit,
                  st};
-    // the SO kernel walks rows in tiles of LZ_WG / B
-    LzArgs<XT> aso = a;
-    {
-        const int64_t rows = LZ_WG / B;
-        const int64_t nt = (D + rows - 1) / rows;
-        aso.ntile = nt;
-        aso.chunk = (nt + grid - 1) / grid;
-    }
     const size_t shm = FMT == BH_FMT_PACKED ? (static_cast<size_t>(1) << h->code_bits) * sizeof(double) : 0;
+    const int engine = bh_engine_choice(B, h) == 1 ? 1 : 2;  // staged row blocks or one thread per (row, b)
+    const int32_t* rblk = nullptr;
+    int64_t nrblk = 0;
+    if (engine == 1) BH_TRY(bh_ham_rowblocks(h, B, &rblk, &nrblk));
+    const int grid_rb = static_cast<int>(std::min<int64_t>(std::max<int64_t>(nrblk, 1), LZ_GRID_MAX));
+    const size_t shm_rb = shm + BH_STAGE_PRODUCTS * sizeof(double);
+    const int64_t rpw = (D + LZ_GRID_MAX - 1) / LZ_GRID_MAX;
+    const int grid_rl = static_cast<int>((D + rpw - 1) / rpw);
+    const int grid_k1 = engine == 1 ? grid_rb : grid_rl;
+    const int64_t ntile_b = (D + LZ_WG / B - 1) / (LZ_WG / B);
+    const int grid_k2 = static_cast<int>(std::min<int64_t>(ntile_b, LZ_GRID_MAX));
 
-    auto launch_a = [&](bool init) {
-        KTimer kt(ctx, init ? "lanczos_init" : "lanczos_spmv", s);
-#define LA(GG)                                                                                                     \
-    if (G == GG) {                                                                                                 \
-        if (init) hipLaunchKernelGGL((k_lz_a<XT, B, GG, FMT, true>), dim3(grid), dim3(LZ_WG), shm, s, a, r0.p);   \
-        else hipLaunchKernelGGL((k_lz_a<XT, B, GG, FMT, false>), dim3(grid), dim3(LZ_WG), shm, s, a, r0.p);       \
-        return;                                                                                                    \
+    auto launch_k1 = [&](int LM) {
+        KTimer kt(ctx, "lanczos_spmv", s);
+#define K1L(LMV)                                                                                                       \
+    if (LM == LMV) {                                                                                                   \
+        if (engine == 1)                                                                                               \
+            hipLaunchKernelGGL((k_lz_a_rb<XT, B, FMT, LMV>), dim3(grid_rb), dim3(LZ_WG), shm_rb, s, a, rblk, nrblk);   \
+        else                                                                                                           \
+            hipLaunchKernelGGL((k_lz_a_rl<XT, B, FMT, LMV>), dim3(grid_rl), dim3(LZ_WG), shm, s, a, rpw);              \
+        return;                                                                                                        \
     }
-        if constexpr (B == 1) { LA(1) LA(2) LA(4) LA(8) LA(16) LA(32) LA(64) }
-        else if constexpr (B == 2) { LA(1) LA(2) LA(4) LA(8) LA(16) LA(32) }
-        else if constexpr (B == 4) { LA(1) LA(2) LA(4) LA(8) LA(16) }
-        else if constexpr (B == 8) { LA(1) LA(2) LA(4) LA(8) }
-        else if constexpr (B == 16) { LA(1) LA(2) LA(4) }
-        else if constexpr (B == 32) { LA(1) LA(2) }
-        else { LA(1) }
-#undef LA
+        K1L(0) K1L(4) K1L(16)
+#undef K1L
     };
-    auto launch_b = [&](bool init, int use_so) {
-        KTimer kt(ctx, "lanczos_update", s);
-        if (init) hipLaunchKernelGGL((k_lz_b<XT, B, true>), dim3(grid), dim3(LZ_WG), 0, s, a, 0);
-        else hipLaunchKernelGGL((k_lz_b<XT, B, false>), dim3(grid), dim3(LZ_WG), 0, s, a, use_so);
+    auto reduce_plain = [&](int nwg, int nv, double* out) {
+        KTimer kt(ctx, "lanczos_reduce", s);
+        const int waves = LZ_RED_WG / 64;
+        const int g = std::max(1, std::min((nv + waves - 1) / waves, 64));
+        hipLaunchKernelGGL((k_lz_reduce<B>), dim3(g), dim3(LZ_RED_WG), 0, s, st, part.p, nwg, nv, LZ_RED_PLAIN, out);
     };
-    auto launch_so = [&](int lmax_used) {
-        KTimer kt(ctx, "lanczos_so", s);
-        hipLaunchKernelGGL((k_lz_so<XT, B>), dim3(grid), dim3(LZ_WG), 0, s, aso, lmax_used);
+    auto step = [&](int LM) {
+        launch_k1(LM);
+        reduce_plain(grid_k1, (1 + 2 * LM) * B, st.sums);
+        {
+            KTimer kt(ctx, "lanczos_update", s);
+            hipLaunchKernelGGL((k_lz_b<XT, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, a, LM);
+        }
+        {
+            KTimer kt(ctx, "lanczos_reduce", s);
+            hipLaunchKernelGGL((k_lz_reduce<B>), dim3(1), dim3(LZ_RED_WG), 0, s, st, part.p, grid_k2, B, LZ_RED_BETA,
+                               nullptr);
+        }
     };
-
-    // start: W = H r, V[0] = W, scale_0 = 1/||W||
-    launch_a(true);
-    launch_b(true, 0);
-    BH_HIP(ctx, hipGetLastError());
+    const int grid = grid_k2;  // workgroups of the dot-product kernels
 
     std::vector<RunHost> rh(B);
+    std::vector<double> hlo(B, 0.0), hhi(B, 0.0);  // Gershgorin interval of H_b
+    for (int b = 0; b < nreal; ++b) {
+        const double va = Ub[b] * h->vmin, vb = Ub[b] * h->vmax;
+        hlo[b] = std::fmin(va, vb) - std::fabs(Jb[b]) * h->tmax;
+        hhi[b] = std::fmax(va, vb) + std::fabs(Jb[b]) * h->tmax;
+    }
     std::vector<double> ha(n_alpha), hb(n_beta);
     int j = 0;          // steps done
     int lmax_used = 0;  // max locks over runs
@@ -586,41 +623,87 @@ static int lanczos_impl(bh_ctx* ctx, bh_ham* h, const double* Jh, const double* 
 
     while (j < maxit && nleft > 0) {
         const int steps = std::min(std::min(next_check, maxit) - j, maxit - j);
-        for (int t = 0; t < steps; ++t) {
-            launch_a(false);
-            if (lmax_used > 0) launch_so(lmax_used);
-            launch_b(false, lmax_used > 0 ? 1 : 0);
-        }
+        const int LM = lmax_used == 0 ? 0 : (lmax_used <= 4 ? 4 : 16);
+        for (int t = 0; t < steps; ++t) step(LM);
         BH_HIP(ctx, hipGetLastError());
         j += steps;
         BH_HIP(ctx, hipMemcpyAsync(ha.data(), st.alpha, static_cast<size_t>(j) * B * 8, hipMemcpyDeviceToHost, s));
         BH_HIP(ctx, hipMemcpyAsync(hb.data(), st.beta, static_cast<size_t>(j + 1) * B * 8, hipMemcpyDeviceToHost, s));
         BH_HIP(ctx, hipStreamSynchronize(s));
-        next_check = j + chunk;
+        next_check = j + std::max(chunk, j / 4);  // analysis cost grows with j: space the checks out
 
-        // analysis per active run
+        // analysis per active run (independent runs analysed on parallel host threads)
         std::vector<std::vector<int>> new_locks(B);
         std::vector<Analysis> ans(B);
-        bool any_lock = false, any_done = false;
-        for (int b = 0; b < nreal; ++b) {
+        std::vector<int> done_now(B, 0);
+        auto analyse_run = [&](int b) {
             RunHost& R = rh[b];
-            if (R.done) continue;
+            if (R.done) return;
             R.alpha.assign(j, 0.0);
             R.beta.assign(j + 1, 0.0);
             for (int t = 0; t < j; ++t) R.alpha[t] = ha[static_cast<size_t>(t) * B + b];
             for (int t = 0; t <= j; ++t) R.beta[t] = hb[static_cast<size_t>(t) * B + b];
-            const double bnext = R.beta[j];
+            // An invariant Krylov space shows as a beta at the rounding level of the stored vectors.  It
+            // may have happened anywhere since the last check; the steps after it restart from rounding
+            // noise and would duplicate every level, so T is cut at the first such beta.
+            int jr = j;
+            {
+                double amax = 0.0;
+                for (int t = 0; t < j; ++t) amax = std::fmax(amax, std::fabs(R.alpha[t]) + R.beta[t]);
+                const double thr = std::fmax(1e-12, 100.0 * eps_store) * std::fmax(amax, 1e-300);
+                for (int t = 1; t <= j; ++t)
+                    if (R.beta[t] <= thr) {
+                        jr = t;
+                        break;
+                    }
+            }
+            const double bnext = R.beta[jr];
             Analysis& an = ans[b];
-            const int p = std::min(j, 2 * k + 4);
-            analyze(R, j, bnext, p, an);
-            const bool exhausted = bnext <= 1e-12 * std::fmax(an.tnorm, 1e-300) || j >= D;
+            const int p = std::min(jr, 2 * k + 4);
+            analyze(R, jr, bnext, p, an);
+            // Breakdown guard: Ritz values are Rayleigh quotients of H and must lie in its Gershgorin
+            // interval; values outside it (or non-finite coefficients) mean the Lanczos vectors lost
+            // orthogonality beyond repair.  The run stops unconverged (mixed runs are rerun in FP64).
+            bool broken = !std::isfinite(an.tnorm) || !std::isfinite(bnext) || (R.tnorm0 > 0 && an.tnorm > 1e3 * R.tnorm0);
+            if (!broken && !an.theta.empty()) {
+                const double slack = 1e-6 * std::fmax(std::fmax(std::fabs(hlo[b]), std::fabs(hhi[b])), 1.0);
+                if (an.theta.front() < hlo[b] - slack || an.theta.back() > hhi[b] + slack) broken = true;
+            }
+            if (broken) {
+                R.done = true;
+                R.iters = j;
+                R.status = BH_ENOCONV;
+                R.theta.clear();
+                R.s.clear();
+                done_now[b] = 1;
+                return;
+            }
+            if (R.tnorm0 == 0.0) R.tnorm0 = an.tnorm;
+            const bool exhausted = jr < j || bnext <= std::fmax(1e-12, 100.0 * eps_store) * std::fmax(an.tnorm, 1e-300) ||
+                                   j >= D;
             // FP32-stored vectors: Ritz residuals cannot fall below the storage rounding, ~eps32 ||T||
             const double floor = MIXED ? 4.0 * eps_store * an.tnorm : 0.0;
-            // walk distinct levels from the bottom
+            // walk distinct levels from the bottom.  With FP32-stored vectors, copies of a degenerate level
+            // split at the storage rounding (~eps32 ||T||), so they are merged at that scale here and
+            // separated again, if they are distinct, by the FP64 finish.
+            const double tdedup = MIXED ? std::fmax(dedup, 50.0 * eps_store) : dedup;
             std::vector<int> lev;
             for (int i = 0; i < static_cast<int>(an.theta.size()) && static_cast<int>(lev.size()) < k; ++i) {
-                if (!lev.empty() && std::fabs(an.theta[i] - an.theta[lev.back()]) <= dedup * an.tnorm) continue;
+                if (!lev.empty() && std::fabs(an.theta[i] - an.theta[lev.back()]) <= tdedup * an.tnorm) continue;
                 lev.push_back(i);
+            }
+            // extra candidates finished in FP64 beside the k wanted levels
+            std::vector<int> cand = lev;
+            for (int i = lev.empty() ? 0 : lev.back() + 1;
+                 i < static_cast<int>(an.theta.size()) && static_cast<int>(cand.size()) < std::min(k + 3, LZ_NVMAX); ++i) {
+                if (std::fabs(an.theta[i] - an.theta[cand.back()]) <= tdedup * an.tnorm) continue;
+                cand.push_back(i);
+            }
+            if (lz_debug()) {
+                std::fprintf(stderr, "[lz] j=%d run=%d tnorm=%.3e bnext=%.3e locks=%d:", j, b, an.tnorm, bnext, nlock[b]);
+                for (size_t i = 0; i < an.theta.size() && i < 8; ++i)
+                    std::fprintf(stderr, " (%.10f %.1e)", an.theta[i], an.bound[i]);
+                std::fprintf(stderr, "\n");
             }
             bool conv = static_cast<int>(lev.size()) == k || exhausted;
             for (int i : lev) {
@@ -629,33 +712,61 @@ static int lanczos_impl(bh_ctx* ctx, bh_ham* h, const double* Jh, const double* 
             }
             if (conv || j >= maxit) {
                 R.done = true;
-                R.iters = j;
+                R.iters = jr;
                 R.status = conv && static_cast<int>(lev.size()) == k ? BH_OK : BH_ENOCONV;
                 R.theta.clear();
                 R.s.clear();
-                for (int i : lev) {
+                for (int i : cand) {
                     R.theta.push_back(an.theta[i]);
                     R.s.push_back(an.s[i]);
                 }
                 R.tnorm = an.tnorm;
                 R.all_theta = an.theta;
-                act[b] = 0;
-                any_done = true;
-                --nleft;
-                continue;
+                done_now[b] = 1;
+                return;
             }
             // selective orthogonalisation: lock converged wanted-region Ritz vectors
             const double lock_thr = std::sqrt(eps_store) * an.tnorm;
             for (int i : lev) {
                 if (an.bound[i] > lock_thr) continue;
                 if (nlock[b] + static_cast<int>(new_locks[b].size()) >= LZ_LMAX) break;
+                // a level already locked keeps its Ritz value only up to its bound: match within the
+                // lock threshold, so that one level is never locked twice (near-parallel locked vectors
+                // would break the Gram-Schmidt step of the orthogonalisation)
                 bool have = false;
                 for (double th : R.lock_theta)
-                    if (std::fabs(th - an.theta[i]) <= dedup * an.tnorm) have = true;
+                    if (std::fabs(th - an.theta[i]) <= lock_thr) have = true;
+                for (int q : new_locks[b])
+                    if (std::fabs(an.theta[q] - an.theta[i]) <= lock_thr) have = true;
                 if (have) continue;
                 new_locks[b].push_back(i);
-                any_lock = true;
             }
+        };
+        {
+            std::vector<int> todo;
+            for (int b = 0; b < nreal; ++b)
+                if (!rh[b].done) todo.push_back(b);
+            const int nth = std::max(1, std::min(static_cast<int>(todo.size()),
+                                                 static_cast<int>(std::thread::hardware_concurrency())));
+            if (nth <= 1) {
+                for (int b : todo) analyse_run(b);
+            } else {
+                std::vector<std::thread> pool;
+                for (int t = 0; t < nth; ++t)
+                    pool.emplace_back([&, t]() {
+                        for (size_t q = t; q < todo.size(); q += nth) analyse_run(todo[q]);
+                    });
+                for (auto& th : pool) th.join();
+            }
+        }
+        bool any_lock = false, any_done = false;
+        for (int b = 0; b < nreal; ++b) {
+            if (done_now[b]) {
+                act[b] = 0;
+                any_done = true;
+                --nleft;
+            }
+            if (!new_locks[b].empty()) any_lock = true;
         }
         if (any_lock) {
             int nv = 0;
@@ -689,8 +800,8 @@ static int lanczos_impl(bh_ctx* ctx, bh_ham* h, const double* Jh, const double* 
                 if (!new_locks[b].empty()) njb[b] = j;
             BH_TRY(form_vectors(njb, nv, Cbuf, tmp.p, vs));
             // normalise: ||y||^2 per (i, b)
-            hipLaunchKernelGGL((k_lz_dots<XT, B>), dim3(grid), dim3(LZ_WG), 0, s, tmp.p, nullptr, D, nv, 2, nullptr, part.p,
-                               gpart.p, cnt.p, dnorm.p);
+            hipLaunchKernelGGL((k_lz_dots<XT, B>), dim3(grid), dim3(LZ_WG), 0, s, tmp.p, nullptr, D, nv, 2, nullptr, part.p);
+            reduce_plain(grid, nv * B, dnorm.p);
             std::vector<double> nrm(static_cast<size_t>(nv) * B);
             BH_HIP(ctx, hipMemcpyAsync(nrm.data(), dnorm.p, nrm.size() * 8, hipMemcpyDeviceToHost, s));
             BH_HIP(ctx, hipStreamSynchronize(s));
@@ -766,27 +877,38 @@ static int lanczos_impl(bh_ctx* ctx, bh_ham* h, const double* Jh, const double* 
         }
         BH_HIP(ctx, hipGetLastError());
         std::vector<double> dots(static_cast<size_t>(2 * nvf) * B);
-        hipLaunchKernelGGL((k_lz_dots<double, B>), dim3(grid), dim3(LZ_WG), 0, s, Yf.p, HY.p, D, nvf, 0, nullptr, part.p,
-                           gpart.p, cnt.p, dnorm.p);
+        hipLaunchKernelGGL((k_lz_dots<double, B>), dim3(grid), dim3(LZ_WG), 0, s, Yf.p, HY.p, D, nvf, 0, nullptr, part.p);
+        reduce_plain(grid, 2 * nvf * B, dnorm.p);
         BH_HIP(ctx, hipMemcpyAsync(dots.data(), dnorm.p, dots.size() * 8, hipMemcpyDeviceToHost, s));
         BH_HIP(ctx, hipStreamSynchronize(s));
         for (int i = 0; i < nvf; ++i)
             for (int b = 0; b < B; ++b) {
                 const double yy = dots[(2 * i) * B + b], yhy = dots[(2 * i + 1) * B + b];
-                theta_f[i * B + b] = yy > 0 ? yhy / yy : 0.0;
+                theta_f[i * B + b] = yy > 0 ? yhy / yy : NAN;
             }
         DevArray<double> dth;
         BH_TRY(dth.alloc(ctx, theta_f.size()));
         BH_HIP(ctx, hipMemcpyAsync(dth.p, theta_f.data(), theta_f.size() * 8, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL((k_lz_dots<double, B>), dim3(grid), dim3(LZ_WG), 0, s, Yf.p, HY.p, D, nvf, 1, dth.p, part.p,
-                           gpart.p, cnt.p, dnorm.p);
+        hipLaunchKernelGGL((k_lz_dots<double, B>), dim3(grid), dim3(LZ_WG), 0, s, Yf.p, HY.p, D, nvf, 1, dth.p, part.p);
+        reduce_plain(grid, nvf * B, dnorm.p);
         std::vector<double> rr(static_cast<size_t>(nvf) * B);
         BH_HIP(ctx, hipMemcpyAsync(rr.data(), dnorm.p, rr.size() * 8, hipMemcpyDeviceToHost, s));
         BH_HIP(ctx, hipStreamSynchronize(s));
         for (int i = 0; i < nvf; ++i)
             for (int b = 0; b < B; ++b) {
                 const double yy = dots[(2 * i) * B + b];
-                resid_f[i * B + b] = yy > 0 ? std::sqrt(rr[i * B + b] / yy) : 0.0;
+                resid_f[i * B + b] = yy > 0 ? std::sqrt(rr[i * B + b] / yy) : NAN;
+                if (lz_debug() && b < nreal)
+                    std::fprintf(stderr, "[lz] finish run=%d i=%d yy=%.6e yHy=%.10e theta=%.12f resid=%.3e nj=%d\n", b, i, yy,
+                                 dots[(2 * i + 1) * B + b], theta_f[i * B + b], resid_f[i * B + b], njb[b]);
+            }
+        if (lz_debug())
+            for (int b = 0; b < nreal; ++b) {
+                std::fprintf(stderr, "[lz] scales run=%d:", b);
+                for (int t = 0; t < std::min(j + 1, 6); ++t) std::fprintf(stderr, " %.4e", hsc[static_cast<size_t>(t) * B + b]);
+                std::fprintf(stderr, " | C[0..3]:");
+                for (int t = 0; t < std::min(njmax, 4); ++t) std::fprintf(stderr, " %.4e", Cbuf[(static_cast<size_t>(t) * nvf) * B + b]);
+                std::fprintf(stderr, "\n");
             }
         if (vecs_out) {
             // host copy [B][k][D] of the normalised Ritz vectors
@@ -809,9 +931,16 @@ static int lanczos_impl(bh_ctx* ctx, bh_ham* h, const double* Jh, const double* 
         RunHost& R = rh[b];
         const int nl = std::min(static_cast<int>(R.theta.size()), nvf);
         // levels after the FP64 finish, ascending (Ritz vectors of one degenerate level collapse)
-        std::vector<std::pair<double, double>> lv;
-        for (int i = 0; i < nl; ++i) lv.push_back({theta_f[i * B + b], resid_f[i * B + b]});
-        std::sort(lv.begin(), lv.end());
+        std::vector<std::pair<double, double>> lv0, lv;
+        for (int i = 0; i < nl; ++i) lv0.push_back({theta_f[i * B + b], resid_f[i * B + b]});
+        std::sort(lv0.begin(), lv0.end());
+        for (const auto& q : lv0) {  // FP64 Rayleigh quotients of one level agree to ~||r||^2 / gap
+            if (!lv.empty() && std::fabs(q.first - lv.back().first) <= dedup * std::fmax(R.tnorm, 1e-300)) {
+                if (q.second < lv.back().second) lv.back() = q;
+                continue;
+            }
+            lv.push_back(q);
+        }
         for (int i = 0; i < k; ++i) {
             const double ev = i < static_cast<int>(lv.size()) ? lv[i].first : NAN;
             const double rs = i < static_cast<int>(lv.size()) ? lv[i].second : NAN;
@@ -862,6 +991,98 @@ void bh_lanczos_default_opts(bh_lanczos_opts* o) {
     o->want_vectors = 0;
 }
 
+namespace {
+
+// J = 0: H = U V is diagonal; its levels are the distinct values of U V.  Ritz vectors follow the
+// Krylov convention: the start vector's component in each level's eigenspace, normalised.
+int solve_diagonal(bh_ctx* ctx, bh_ham* h, const std::vector<int>& runs, const double* U, const bh_lanczos_opts& o,
+                   bh_lanczos_result* res, std::vector<double>* vecs) {
+    const int k = o.k;
+    std::vector<float> d(static_cast<size_t>(h->D));
+    BH_HIP(ctx, hipMemcpyAsync(d.data(), h->diag, d.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+    BH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (h->vlevels.empty()) {
+        std::vector<double> v(d.begin(), d.end());
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        h->vlevels = v;
+    }
+    std::vector<double> r;
+    if (vecs) r = lz_start_vector(h->D, o.seed);
+    for (int b : runs) {
+        std::vector<double> lv;
+        if (U[b] == 0.0) lv.push_back(0.0);
+        else
+            for (double v : h->vlevels) lv.push_back(U[b] * v);
+        std::sort(lv.begin(), lv.end());
+        for (int i = 0; i < k; ++i) {
+            if (res->evals) res->evals[b * k + i] = i < static_cast<int>(lv.size()) ? lv[i] : NAN;
+            if (res->resid) res->resid[b * k + i] = i < static_cast<int>(lv.size()) ? 0.0 : NAN;
+            if (res->errest) res->errest[b * k + i] = i < static_cast<int>(lv.size()) ? 0.0 : NAN;
+        }
+        if (res->iters) res->iters[b] = 0;
+        if (res->used_fp64) res->used_fp64[b] = o.prec == BH_PREC_FP64 ? 1 : 0;
+        if (res->status) res->status[b] = static_cast<int>(lv.size()) >= k ? BH_OK : BH_ENOCONV;
+        if (vecs) {
+            for (int i = 0; i < k && i < static_cast<int>(lv.size()); ++i) {
+                double* y = vecs->data() + (static_cast<size_t>(b) * k + i) * h->D;
+                double nrm = 0.0;
+                for (int64_t q = 0; q < h->D; ++q) {
+                    const double e = U[b] * static_cast<double>(d[q]);
+                    y[q] = (U[b] == 0.0 || e == lv[i]) ? r[q] : 0.0;
+                    nrm += y[q] * y[q];
+                }
+                nrm = nrm > 0 ? 1.0 / std::sqrt(nrm) : 0.0;
+                for (int64_t q = 0; q < h->D; ++q) y[q] *= nrm;
+            }
+        }
+    }
+    return BH_OK;
+}
+
+// Lanczos on a subset of the runs; results scattered into the caller's arrays
+int solve_subset(bh_ctx* ctx, bh_ham* h, const std::vector<int>& runs, const double* J, const double* U,
+                 const bh_lanczos_opts& o, int prec, bh_lanczos_result* res, std::vector<double>* vecs, bool add_iters) {
+    const int nb = static_cast<int>(runs.size());
+    if (nb == 0) return BH_OK;
+    const int k = o.k;
+    std::vector<double> Jb(nb), Ub(nb), ev(static_cast<size_t>(nb) * k), rs(ev.size()), ee(ev.size());
+    std::vector<int> it(nb), uf(nb), st(nb);
+    for (int i = 0; i < nb; ++i) {
+        Jb[i] = J[runs[i]];
+        Ub[i] = U[runs[i]];
+    }
+    bh_lanczos_opts o2 = o;
+    o2.prec = prec;
+    bh_lanczos_result r2{ev.data(), rs.data(), ee.data(), it.data(), uf.data(), st.data()};
+    int Bp = 1;
+    while (Bp < nb) Bp <<= 1;
+    std::vector<double> v2;
+    if (prec == BH_PREC_FP64) {
+        BH_TRY(bh_ham_prepare_locked(h, BH_PREC_FP64));
+        BH_TRY(lanczos_dispatch<double>(ctx, h, Bp, Jb.data(), Ub.data(), nb, o2, &r2, vecs ? &v2 : nullptr));
+    } else {
+        BH_TRY(lanczos_dispatch<float>(ctx, h, Bp, Jb.data(), Ub.data(), nb, o2, &r2, vecs ? &v2 : nullptr));
+    }
+    for (int i = 0; i < nb; ++i) {
+        const int b = runs[i];
+        for (int q = 0; q < k; ++q) {
+            if (res->evals) res->evals[b * k + q] = ev[i * k + q];
+            if (res->resid) res->resid[b * k + q] = rs[i * k + q];
+            if (res->errest) res->errest[b * k + q] = ee[i * k + q];
+        }
+        if (res->iters) res->iters[b] = (add_iters ? res->iters[b] : 0) + it[i];
+        if (res->used_fp64) res->used_fp64[b] = prec == BH_PREC_FP64 ? 1 : 0;
+        if (res->status) res->status[b] = st[i];
+        if (vecs && !v2.empty())
+            std::copy(v2.begin() + static_cast<int64_t>(i) * k * h->D, v2.begin() + static_cast<int64_t>(i + 1) * k * h->D,
+                      vecs->begin() + static_cast<int64_t>(b) * k * h->D);
+    }
+    return BH_OK;
+}
+
+}  // namespace
+
 int bh_lanczos(bh_ctx* ctx, bh_ham* h, int B, const double* J, const double* U, const bh_lanczos_opts* opts,
                bh_lanczos_result* res) {
     if (!ctx || !h || !J || !U || !opts || !res) return BH_EINVAL;
@@ -870,73 +1091,50 @@ int bh_lanczos(bh_ctx* ctx, bh_ham* h, int B, const double* J, const double* U, 
     if (opts->prec != BH_PREC_FP64 && opts->prec != BH_PREC_MIXED) return bh_set_error(ctx, BH_EINVAL, "lanczos: prec");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     BH_HIP(ctx, hipSetDevice(ctx->device));
-    int Bp = 1;
-    while (Bp < B) Bp <<= 1;
+    const int k = opts->k;
+    // per-run outputs always exist internally; copied to the caller's arrays at the end
+    std::vector<double> ev(static_cast<size_t>(B) * k, NAN), rs(ev.size(), NAN), ee(ev.size(), NAN);
+    std::vector<int> it(B, 0), uf(B, 0), st(B, BH_OK);
+    bh_lanczos_result R{ev.data(), rs.data(), ee.data(), it.data(), uf.data(), st.data()};
     std::vector<double> vecs;
-    std::vector<double>* vp = opts->want_vectors ? &vecs : nullptr;
-    int rc;
-    if (opts->prec == BH_PREC_FP64) {
-        BH_TRY(bh_ham_prepare_locked(h, BH_PREC_FP64));
-        rc = lanczos_dispatch<double>(ctx, h, Bp, J, U, B, *opts, res, vp);
-    } else {
-        rc = lanczos_dispatch<float>(ctx, h, Bp, J, U, B, *opts, res, vp);
+    std::vector<double>* vp = nullptr;
+    if (opts->want_vectors) {
+        vecs.assign(static_cast<size_t>(B) * k * h->D, 0.0);
+        vp = &vecs;
     }
-    if (rc != BH_OK) return rc;
+    std::vector<int> diag_runs, lz_runs;
+    for (int b = 0; b < B; ++b) (J[b] == 0.0 || h->nnz == 0 ? diag_runs : lz_runs).push_back(b);
+    BH_TRY(solve_diagonal(ctx, h, diag_runs, U, *opts, &R, vp));
+    BH_TRY(solve_subset(ctx, h, lz_runs, J, U, *opts, opts->prec, &R, vp, false));
     // mixed: acceptance of E0 by the FP64 finish; failing solves are rerun in FP64
     if (opts->prec == BH_PREC_MIXED && opts->fallback >= 0) {
         const double acc = opts->accept_rel > 0 ? opts->accept_rel : 1e-9;
         std::vector<int> bad;
-        for (int b = 0; b < B; ++b) {
-            const double e0 = res->evals ? res->evals[b * opts->k] : 0.0;
-            const double er = res->errest ? res->errest[b * opts->k] : 0.0;
-            const int stt = res->status ? res->status[b] : BH_OK;
-            if (stt != BH_OK || !std::isfinite(e0) || !(er <= acc * std::fmax(std::fabs(e0), 1e-300))) bad.push_back(b);
+        for (int b : lz_runs) {
+            const double e0 = ev[b * k], er = ee[b * k];
+            if (st[b] != BH_OK || !std::isfinite(e0) || !(er <= acc * std::fmax(std::fabs(e0), 1e-300))) bad.push_back(b);
         }
-        if (!bad.empty()) {
-            const int nb = static_cast<int>(bad.size());
-            const int k = opts->k;
-            std::vector<double> Jb(nb), Ub(nb), ev(nb * k), rs(nb * k), ee(nb * k);
-            std::vector<int> it(nb), uf(nb), stt(nb);
-            for (int i = 0; i < nb; ++i) {
-                Jb[i] = J[bad[i]];
-                Ub[i] = U[bad[i]];
-            }
-            bh_lanczos_opts o2 = *opts;
-            o2.prec = BH_PREC_FP64;
-            bh_lanczos_result r2{ev.data(), rs.data(), ee.data(), it.data(), uf.data(), stt.data()};
-            BH_TRY(bh_ham_prepare_locked(h, BH_PREC_FP64));
-            int Bp2 = 1;
-            while (Bp2 < nb) Bp2 <<= 1;
-            std::vector<double> v2;
-            BH_TRY(lanczos_dispatch<double>(ctx, h, Bp2, Jb.data(), Ub.data(), nb, o2, &r2, vp ? &v2 : nullptr));
-            for (int i = 0; i < nb; ++i) {
-                const int b = bad[i];
-                for (int q = 0; q < k; ++q) {
-                    if (res->evals) res->evals[b * k + q] = ev[i * k + q];
-                    if (res->resid) res->resid[b * k + q] = rs[i * k + q];
-                    if (res->errest) res->errest[b * k + q] = ee[i * k + q];
-                }
-                if (res->iters) res->iters[b] += it[i];
-                if (res->used_fp64) res->used_fp64[b] = 1;
-                if (res->status) res->status[b] = stt[i];
-                if (vp)
-                    std::copy(v2.begin() + static_cast<int64_t>(i) * k * h->D, v2.begin() + static_cast<int64_t>(i + 1) * k * h->D,
-                              vecs.begin() + static_cast<int64_t>(b) * k * h->D);
-            }
+        BH_TRY(solve_subset(ctx, h, bad, J, U, *opts, BH_PREC_FP64, &R, vp, true));
+    }
+    for (int b = 0; b < B; ++b) {
+        for (int q = 0; q < k; ++q) {
+            if (res->evals) res->evals[b * k + q] = ev[b * k + q];
+            if (res->resid) res->resid[b * k + q] = rs[b * k + q];
+            if (res->errest) res->errest[b * k + q] = ee[b * k + q];
         }
+        if (res->iters) res->iters[b] = it[b];
+        if (res->used_fp64) res->used_fp64[b] = uf[b];
+        if (res->status) res->status[b] = st[b];
     }
     if (vp) {
         ctx->last_vecs.swap(vecs);
         ctx->last_vecs_B = B;
-        ctx->last_vecs_k = opts->k;
+        ctx->last_vecs_k = k;
         ctx->last_vecs_D = h->D;
     }
-    int worst = BH_OK;
-    if (res->status)
-        for (int b = 0; b < B; ++b)
-            if (res->status[b] != BH_OK) worst = BH_ENOCONV;
-    if (worst != BH_OK) bh_set_error(ctx, worst, "lanczos: some solves did not converge within max_iter");
-    return worst;
+    for (int b = 0; b < B; ++b)
+        if (st[b] != BH_OK) return bh_set_error(ctx, BH_ENOCONV, "lanczos: some solves did not converge within max_iter");
+    return BH_OK;
 }
 
 int bh_lanczos_copy_vectors(bh_ctx* ctx, int B, int k, double* vecs) {

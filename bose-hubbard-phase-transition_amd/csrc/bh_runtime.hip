@@ -33,6 +33,23 @@ void bh_dev_free(bh_ctx* ctx, void* p, size_t bytes) {
     ctx->used = ctx->used >= bytes ? ctx->used - bytes : 0;
 }
 
+int bh_ws_get(bh_ctx* ctx, int slot, size_t bytes, void** p) {
+    if (slot < 0 || slot >= bh_ctx::NWS) return bh_set_error(ctx, BH_EINVAL, "workspace slot");
+    if (bytes == 0) bytes = 1;
+    if (ctx->ws_bytes[slot] < bytes) {
+        bh_dev_free(ctx, ctx->ws[slot], ctx->ws_bytes[slot]);
+        ctx->ws[slot] = nullptr;
+        ctx->ws_bytes[slot] = 0;
+        void* q = nullptr;
+        int rc = bh_dev_alloc(ctx, &q, bytes);
+        if (rc != BH_OK) return rc;
+        ctx->ws[slot] = q;
+        ctx->ws_bytes[slot] = bytes;
+    }
+    *p = ctx->ws[slot];
+    return BH_OK;
+}
+
 KTimer::KTimer(bh_ctx* c, const char* n, hipStream_t st) : ctx(c), name(n), s(st) {
     if (ctx && ctx->timing) {
         if (hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) {
@@ -130,6 +147,7 @@ int bh_ctx_destroy(bh_ctx* ctx) {
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
+        for (int i = 0; i < bh_ctx::NWS; ++i) bh_dev_free(ctx, ctx->ws[i], ctx->ws_bytes[i]);
         (void)hipStreamDestroy(ctx->stream);
     }
     delete ctx;

@@ -40,6 +40,103 @@ __global__ void __launch_bounds__(BH_WG) k_spmv(SpmvArgs<XT> a) {
     }
 }
 
+template <typename XT, int B, int FMT>
+__global__ void __launch_bounds__(BH_WG) k_spmv_rb(SpmvArgs<XT> a, const int32_t* __restrict__ rb, int64_t nblocks) {
+    extern __shared__ double smem[];
+    double* tab = smem;                                  // (1 << cb) entries (packed format)
+    double* prod = smem + (FMT == BH_FMT_PACKED ? (1 << a.cb) : 0);
+    if constexpr (FMT == BH_FMT_PACKED) {
+        for (int i = threadIdx.x; i < (1 << a.cb); i += BH_WG) tab[i] = a.table[i];
+        __syncthreads();
+    }
+    const int64_t t = a.xcd ? bh_xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+    auto epi = [&](int64_t r, int bb, double sum) {
+        const double hx = (FMT == BH_FMT_PACKED) ? -sum : sum;
+        const double d = a.diag32 ? static_cast<double>(a.diag32[r]) : a.diag64[r];
+        a.y[r * B + bb] = static_cast<XT>(a.J[bb] * hx + a.U[bb] * d * to_d(a.x[r * B + bb]));
+    };
+    bh_rowblock_run<XT, B, FMT, BH_WG>(t, t + 1, rb, a.row_ptr, a.words, a.col64, a.val64, tab, a.cb, a.x, prod, epi);
+}
+
+template <typename XT, int B, int FMT>
+__global__ void __launch_bounds__(BH_WG) k_spmv_rl(SpmvArgs<XT> a, int64_t rows_per_wg) {
+    extern __shared__ double tab[];
+    if constexpr (FMT == BH_FMT_PACKED) {
+        for (int i = threadIdx.x; i < (1 << a.cb); i += BH_WG) tab[i] = a.table[i];
+        __syncthreads();
+    }
+    const int64_t t = a.xcd ? bh_xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int64_t r0 = min(t * rows_per_wg, a.D), r1 = min(r0 + rows_per_wg, a.D);
+    auto epi = [&](int64_t r, int bb, double sum) {
+        const double hx = (FMT == BH_FMT_PACKED) ? -sum : sum;
+        const double d = a.diag32 ? static_cast<double>(a.diag32[r]) : a.diag64[r];
+        a.y[r * B + bb] = static_cast<XT>(a.J[bb] * hx + a.U[bb] * d * to_d(a.x[r * B + bb]));
+    };
+    bh_rowlane_run<XT, B, FMT, BH_WG, 4>(r0, r1, a.row_ptr, a.words, a.col64, a.val64, tab, a.cb, a.x, epi);
+}
+
+template <typename XT, int FMT>
+static void launch_spmv_rl_t(const SpmvArgs<XT>& a, int B, hipStream_t s) {
+    const size_t shm = FMT == BH_FMT_PACKED ? (static_cast<size_t>(1) << a.cb) * sizeof(double) : 0;
+    const int64_t rows_per_wg = BH_WG / B * 4;
+    const unsigned grid = static_cast<unsigned>((a.D + rows_per_wg - 1) / rows_per_wg);
+#define L(BB)                                                                                              \
+    if (B == BB) {                                                                                        \
+        hipLaunchKernelGGL((k_spmv_rl<XT, BB, FMT>), dim3(grid), dim3(BH_WG), shm, s, a, rows_per_wg);    \
+        return;                                                                                           \
+    }
+    L(1) L(2) L(4) L(8) L(16) L(32) L(64)
+#undef L
+}
+
+template <typename XT, int FMT>
+static void launch_spmv_rb_t(const SpmvArgs<XT>& a, int B, const int32_t* rb, int64_t nb, hipStream_t s) {
+    const size_t shm = ((FMT == BH_FMT_PACKED ? (static_cast<size_t>(1) << a.cb) : 0) + BH_STAGE_PRODUCTS) * sizeof(double);
+    const unsigned grid = static_cast<unsigned>(nb);
+#define L(BB)                                                                                            \
+    if (B == BB) {                                                                                      \
+        hipLaunchKernelGGL((k_spmv_rb<XT, BB, FMT>), dim3(grid), dim3(BH_WG), shm, s, a, rb, nb);       \
+        return;                                                                                         \
+    }
+    L(1) L(2) L(4) L(8) L(16) L(32) L(64)
+#undef L
+}
+
+void bh_launch_spmv_rb(const SpmvArgs<float>& a, int B, const int32_t* rb, int64_t nb, hipStream_t s) {
+    launch_spmv_rb_t<float, BH_FMT_PACKED>(a, B, rb, nb, s);
+}
+void bh_launch_spmv_rb(const SpmvArgs<double>& a, int B, int fmt, const int32_t* rb, int64_t nb, hipStream_t s) {
+    if (fmt == BH_FMT_PACKED) launch_spmv_rb_t<double, BH_FMT_PACKED>(a, B, rb, nb, s);
+    else launch_spmv_rb_t<double, BH_FMT_CSR64>(a, B, rb, nb, s);
+}
+
+// one operator application with the engine chosen for (h, B)
+int bh_apply(bh_ctx* ctx, bh_ham* h, const SpmvArgs<double>* a64, const SpmvArgs<float>* a32, int fmt, int B,
+             hipStream_t s) {
+    const int eng = bh_engine_choice(B, h);
+    if (eng == 2) {
+        if (a64) {
+            if (fmt == BH_FMT_PACKED) launch_spmv_rl_t<double, BH_FMT_PACKED>(*a64, B, s);
+            else launch_spmv_rl_t<double, BH_FMT_CSR64>(*a64, B, s);
+        } else {
+            launch_spmv_rl_t<float, BH_FMT_PACKED>(*a32, B, s);
+        }
+        return BH_OK;
+    }
+    if (eng == 1) {
+        const int32_t* rb = nullptr;
+        int64_t nb = 0;
+        BH_TRY(bh_ham_rowblocks(h, B, &rb, &nb));
+        if (a64) bh_launch_spmv_rb(*a64, B, fmt, rb, nb, s);
+        else bh_launch_spmv_rb(*a32, B, rb, nb, s);
+    } else {
+        const int G = bh_pick_lanes(h, B);
+        if (a64) bh_launch_spmv(*a64, B, G, fmt, s);
+        else bh_launch_spmv(*a32, B, G, s);
+    }
+    return BH_OK;
+}
+
 template <typename XT, int FMT>
 static void launch_spmv_t(const SpmvArgs<XT>& a, int B, int G, hipStream_t s) {
     const int GB = G * B;
@@ -99,17 +196,16 @@ int bh_spmv(bh_ctx* ctx, bh_ham* h, int prec, int B, const double* J, const doub
     BH_TRY(dU.alloc(ctx, B));
     BH_HIP(ctx, hipMemcpyAsync(dJ.p, J, B * sizeof(double), hipMemcpyHostToDevice, s));
     BH_HIP(ctx, hipMemcpyAsync(dU.p, U, B * sizeof(double), hipMemcpyHostToDevice, s));
-    const int G = bh_pick_lanes(h, B);
     {
         KTimer kt(ctx, "spmv", s);
         if (prec == BH_PREC_FP64) {
             SpmvArgs<double> a{h->D, h->row_ptr, h->words, h->col64, h->val64, nullptr, h->diag64, h->table, h->code_bits,
                                dJ.p, dU.p, static_cast<const double*>(x), static_cast<double*>(y), 1};
-            bh_launch_spmv(a, B, G, BH_FMT_CSR64, s);
+            BH_TRY(bh_apply(ctx, h, &a, nullptr, BH_FMT_CSR64, B, s));
         } else {
             SpmvArgs<float> a{h->D, h->row_ptr, h->words, nullptr, nullptr, h->diag, nullptr, h->table, h->code_bits,
                               dJ.p, dU.p, static_cast<const float*>(x), static_cast<float*>(y), 1};
-            bh_launch_spmv(a, B, G, s);
+            BH_TRY(bh_apply(ctx, h, nullptr, &a, BH_FMT_PACKED, B, s));
         }
     }
     BH_HIP(ctx, hipGetLastError());
@@ -167,24 +263,22 @@ int bh_spmv_bench(bh_ctx* ctx, bh_ham* h, int prec, int B, int reps, double* avg
     BH_HIP(ctx, hipMemcpyAsync(dJ.p, ones.data(), B * 8, hipMemcpyHostToDevice, s));
     BH_HIP(ctx, hipMemcpyAsync(dU.p, ones.data(), B * 8, hipMemcpyHostToDevice, s));
     BH_HIP(ctx, hipMemsetAsync(dx.p, 0, n * es, s));
-    const int G = bh_pick_lanes(h, B);
-    auto launch = [&]() {
+    auto launch = [&]() -> int {
         if (prec == BH_PREC_FP64) {
             SpmvArgs<double> a{h->D, h->row_ptr, h->words, h->col64, h->val64, nullptr, h->diag64, h->table, h->code_bits,
                                dJ.p, dU.p, reinterpret_cast<const double*>(dx.p), reinterpret_cast<double*>(dy.p), 1};
-            bh_launch_spmv(a, B, G, BH_FMT_CSR64, s);
-        } else {
-            SpmvArgs<float> a{h->D, h->row_ptr, h->words, nullptr, nullptr, h->diag, nullptr, h->table, h->code_bits,
-                              dJ.p, dU.p, reinterpret_cast<const float*>(dx.p), reinterpret_cast<float*>(dy.p), 1};
-            bh_launch_spmv(a, B, G, s);
+            return bh_apply(ctx, h, &a, nullptr, BH_FMT_CSR64, B, s);
         }
+        SpmvArgs<float> a{h->D, h->row_ptr, h->words, nullptr, nullptr, h->diag, nullptr, h->table, h->code_bits,
+                          dJ.p, dU.p, reinterpret_cast<const float*>(dx.p), reinterpret_cast<float*>(dy.p), 1};
+        return bh_apply(ctx, h, nullptr, &a, BH_FMT_PACKED, B, s);
     };
-    launch();  // warm-up
+    BH_TRY(launch());  // warm-up (also builds the row blocks)
     hipEvent_t e0, e1;
     BH_HIP(ctx, hipEventCreate(&e0));
     BH_HIP(ctx, hipEventCreate(&e1));
     BH_HIP(ctx, hipEventRecord(e0, s));
-    for (int i = 0; i < reps; ++i) launch();
+    for (int i = 0; i < reps; ++i) BH_TRY(launch());
     BH_HIP(ctx, hipEventRecord(e1, s));
     BH_HIP(ctx, hipEventSynchronize(e1));
     float ms = 0.f;

@@ -29,3 +29,7 @@ int bh_pick_lanes(const bh_ham* h, int B);
 double bh_spmv_bytes(const bh_ham* h, int prec, int B);
 int bh_is_pow2_le64(int B);
 int bh_ham_prepare_locked(bh_ham* h, int prec);
+void bh_launch_spmv_rb(const SpmvArgs<float>& a, int B, const int32_t* rb, int64_t nb, hipStream_t s);
+void bh_launch_spmv_rb(const SpmvArgs<double>& a, int B, int fmt, const int32_t* rb, int64_t nb, hipStream_t s);
+int bh_apply(bh_ctx* ctx, bh_ham* h, const SpmvArgs<double>* a64, const SpmvArgs<float>* a32, int fmt, int B,
+             hipStream_t s);

@@ -49,6 +49,9 @@ extern "C" {
  * BH_ORDER_ASCENDING: ascending tag (= ascending prod p_i^n_i), the order the code intended. */
 #define BH_ORDER_REFERENCE 0
 #define BH_ORDER_ASCENDING 1
+/* BH_ORDER_LEX: descending lexicographic enumeration order itself (the states before any sort); no
+ * reference function returns it, it is offered as a solver layout whose hops stay closer together. */
+#define BH_ORDER_LEX 2
 
 /* ---- precisions --------------------------------------------------------------------------------- */
 /* BH_PREC_FP64: FP64 CSR (double value + int32 column) and FP64 vectors -- the plain comparator.
