@@ -45,6 +45,40 @@ class LanczosOpts(C.Structure):
                 ("seed", C.c_uint64), ("fallback", C.c_int), ("want_vectors", C.c_int)]
 
 
+class SweepSpec(C.Structure):
+    _fields_ = [("kind", C.c_int), ("Lx", C.c_int), ("Ly", C.c_int), ("periodic", C.c_int), ("n", C.c_int),
+                ("nJ", C.c_int), ("J", C.c_void_p), ("U", C.c_double), ("nmu", C.c_int), ("mu", C.c_void_p),
+                ("flags", C.c_uint), ("order", C.c_int)]
+
+
+class SweepOut(C.Structure):
+    _fields_ = [("E", C.c_void_p), ("gap_ratio", C.c_void_p), ("density", C.c_void_p),
+                ("compressibility", C.c_void_p), ("mu_plus", C.c_void_p), ("mu_minus", C.c_void_p),
+                ("status", C.c_void_p), ("used_fp64", C.c_void_p)]
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int), ("Lx", C.c_int), ("Ly", C.c_int), ("periodic", C.c_int), ("n", C.c_int),
+                ("J", C.c_double), ("U", C.c_double), ("mu", C.c_double)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("lanes", C.c_int), ("prec", C.c_int), ("hbm_budget", C.c_size_t), ("engine_bytes", C.c_size_t),
+                ("flags", C.c_uint), ("lanczos", LanczosOpts)]
+
+
+class Result(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_int), ("k", C.c_int), ("evals", C.c_double * 16),
+                ("resid", C.c_double * 16), ("iters", C.c_int), ("used_fp64", C.c_int), ("t_submit", C.c_double),
+                ("t_start", C.c_double), ("t_done", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("queued", C.c_int64), ("in_flight", C.c_int64),
+                ("engines", C.c_int), ("hbm_used", C.c_size_t), ("p50_latency", C.c_double),
+                ("p99_latency", C.c_double), ("mean_latency", C.c_double), ("busy_seconds", C.c_double)]
+
+
 class LanczosResult(C.Structure):
     _fields_ = [("evals", C.c_void_p), ("resid", C.c_void_p), ("errest", C.c_void_p), ("iters", C.c_void_p),
                 ("used_fp64", C.c_void_p), ("status", C.c_void_p)]
@@ -104,7 +138,14 @@ def _bind_optional(L):
         "bh_lanczos": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, C.POINTER(LanczosOpts), C.POINTER(LanczosResult)]),
         "bh_lanczos_copy_vectors": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
         "bh_tridiag_eigvals": (C.c_int, [C.c_int, _vp, _vp, _vp]),
-        "bh_server_create": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
+        "bh_sweep": (C.c_int, [_vp, C.POINTER(SweepSpec), C.POINTER(LanczosOpts), C.POINTER(SweepOut)]),
+        "bh_server_create": (C.c_int, [_vp, C.POINTER(ServerOpts), C.POINTER(_vp)]),
+        "bh_server_submit": (C.c_int, [_vp, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+        "bh_server_poll": (C.c_int, [_vp, C.c_uint64, C.POINTER(Result)]),
+        "bh_server_wait": (C.c_int, [_vp, C.c_uint64, C.c_double, C.POINTER(Result)]),
+        "bh_server_drain": (C.c_int, [_vp, C.c_double]),
+        "bh_server_stats_get": (C.c_int, [_vp, C.POINTER(ServerStats)]),
+        "bh_server_destroy": (C.c_int, [_vp]),
     }
     for name, (res, args) in opt.items():
         if hasattr(L, name):
@@ -173,6 +214,28 @@ class Context:
 
     def basis(self, m: int, n: int, order: int = ORDER_REFERENCE) -> "Basis":
         return Basis(self, m, n, order)
+
+    def sweep(self, geometry: str, n: int, J, U: float, mu, m: int = 0, Lx: int = 0, Ly: int = 0,
+              periodic: bool = True, k: int = 4, prec: int = PREC_MIXED, flags: int = 0, order: int = ORDER_LEX, **kw):
+        """Batched phase-diagram sweep over the sectors n-1, n, n+1 (see bh_sweep in bh_capi.h)."""
+        J = np.ascontiguousarray(np.atleast_1d(J), np.float64)
+        mu = np.ascontiguousarray(np.atleast_1d(mu), np.float64)
+        kind, a, b = (0, m, 1) if geometry == "chain" else (1, Lx, Ly)
+        sp = SweepSpec(kind, a, b, int(periodic), n, J.size, _ptr(J).value, U, mu.size, _ptr(mu).value, flags, order)
+        o = LanczosOpts()
+        lib().bh_lanczos_default_opts(C.byref(o))
+        o.k, o.prec = k, prec
+        for key, v in kw.items():
+            setattr(o, key, v)
+        nJ, nmu = J.size, mu.size
+        res = {"E": np.full((3, nJ, k), np.nan), "gap_ratio": np.full((3, nJ), np.nan),
+               "density": np.full((nJ, nmu), np.nan), "compressibility": np.full((nJ, nmu), np.nan),
+               "mu_plus": np.full(nJ, np.nan), "mu_minus": np.full(nJ, np.nan),
+               "status": np.zeros((3, nJ), np.int32), "used_fp64": np.zeros((3, nJ), np.int32)}
+        out = SweepOut(*[_ptr(res[f]).value for f in ("E", "gap_ratio", "density", "compressibility", "mu_plus",
+                                                      "mu_minus", "status", "used_fp64")])
+        self.check(lib().bh_sweep(self.h, C.byref(sp), C.byref(o), C.byref(out)))
+        return res
 
 
 class Basis:
@@ -317,3 +380,58 @@ def gap_ratio(levels) -> float:
     mn, mx = np.minimum(a, b), np.maximum(a, b)
     r = np.where(mx != 0, mn / np.where(mx != 0, mx, 1.0), 0.0)
     return float(r.mean())
+
+
+class Server:
+    """Continuous-batching request server (bh_server_* of the C ABI)."""
+
+    def __init__(self, ctx: Context, lanes: int = 16, prec: int = PREC_MIXED, k: int = 4, hbm_budget: int = 0,
+                 engine_bytes: int = 0, flags: int = 0, **lanczos_kw):
+        self.ctx = ctx
+        o = ServerOpts()
+        lib().bh_lanczos_default_opts(C.byref(o.lanczos))
+        o.lanes, o.prec, o.hbm_budget, o.engine_bytes, o.flags = lanes, prec, hbm_budget, engine_bytes, flags
+        o.lanczos.k = k
+        for key, v in lanczos_kw.items():
+            setattr(o.lanczos, key, v)
+        self.h = _vp()
+        ctx.check(lib().bh_server_create(ctx.h, C.byref(o), C.byref(self.h)))
+
+    def submit(self, geometry: str, n: int, J: float, U: float, mu: float = 0.0, m: int = 0, Lx: int = 0, Ly: int = 0,
+               periodic: bool = True) -> int:
+        kind, a, b = (0, m, 1) if geometry == "chain" else (1, Lx, Ly)
+        rid = C.c_uint64()
+        rc = lib().bh_server_submit(self.h, C.byref(Request(kind, a, b, int(periodic), n, J, U, mu)), C.byref(rid))
+        if rc != OK:
+            raise BHError(rc, "bh_server_submit failed")
+        return rid.value
+
+    def wait(self, rid: int, timeout: float = 300.0) -> Result:
+        r = Result()
+        rc = lib().bh_server_wait(self.h, rid, timeout, C.byref(r))
+        if rc != OK:
+            raise BHError(rc, f"request {rid} not done within {timeout} s")
+        return r
+
+    def poll(self, rid: int):
+        r = Result()
+        return r if lib().bh_server_poll(self.h, rid, C.byref(r)) == OK else None
+
+    def drain(self, timeout: float = 600.0) -> bool:
+        return lib().bh_server_drain(self.h, timeout) == OK
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        lib().bh_server_stats_get(self.h, C.byref(st))
+        return st
+
+    def close(self):
+        if self.h:
+            lib().bh_server_destroy(self.h)
+            self.h = _vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -1,0 +1,37 @@
+// bh_engine.hpp -- the Lanczos engine shared by the batch solver, the sweep driver and the server.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "bh_internal.hpp"
+
+struct LaneResult {
+    std::vector<double> evals, resid, errest;  // k entries (NaN when missing)
+    int iters = 0;
+    int status = BH_ENOCONV;
+    std::vector<double> vecs;  // k * D (when requested)
+};
+
+// B lanes over one Hamiltonian; each lane runs one solve (J_b, U_b) with its own step counter, so lanes
+// are admitted and retired independently.  All work goes to the engine's stream.
+class LzEngineBase {
+   public:
+    virtual ~LzEngineBase() = default;
+    virtual int admit(int slot, double J, double U) = 0;
+    virtual int run(int steps) = 0;                     // enqueue steps for the active lanes
+    virtual int check(std::vector<int>& finished) = 0;  // wait, analyse, retire converged lanes
+    virtual int finish(const std::vector<int>& lanes, std::vector<LaneResult>& out, bool want_vectors) = 0;
+    virtual int lanes() const = 0;
+    virtual int active() const = 0;
+    virtual int steps_of(int slot) const = 0;
+    virtual int suggest_chunk() const = 0;  // steps until the next useful analysis
+    virtual size_t device_bytes() const = 0;
+};
+
+// use_ctx_ws: keep the Krylov basis in the context's workspaces (one engine at a time per context);
+// otherwise the engine owns its buffers (several engines may coexist, e.g. in the server)
+std::unique_ptr<LzEngineBase> bh_make_engine(bh_ctx* ctx, bh_ham* h, int prec, int B, const bh_lanczos_opts& o,
+                                             hipStream_t s, bool use_ctx_ws, int* rc);
+// bytes an engine of B lanes allocates for itself (use_ctx_ws = false)
+size_t bh_engine_bytes(int64_t D, int prec, int B, const bh_lanczos_opts& o);

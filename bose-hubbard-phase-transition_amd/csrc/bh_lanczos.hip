@@ -35,6 +35,7 @@
 #include "bh_kernels.hpp"
 #include "bh_spmv.hpp"
 #include "bh_tridiag.hpp"
+#include "bh_engine.hpp"
 
 constexpr int LZ_WG = 512;         // threads per workgroup of the Lanczos kernels (8 waves)
 constexpr int LZ_GRID_MAX = 1024;  // workgroups that write partial sums
@@ -51,7 +52,7 @@ struct LzState {
     double* sums;    // [(1 + 2 LZ_LMAX) B] R1 results: alpha, <y_l, w>, <y_l, v_j>
     int* active;     // [B]
     int* nlock;      // [B]
-    int* step;       // [1]
+    int* step;       // [B] Lanczos step of each lane (lanes are admitted and retired independently)
     double* part;    // [LZ_GRID_MAX][nv] per-workgroup partials
 };
 
@@ -70,6 +71,7 @@ struct LzArgs {
     int64_t vs;    // D * B
     double* W;     // [D][B]
     XT* Y;         // [LZ_LMAX][D][B] locked Ritz vectors (unit norm)
+    int maxit;     // Krylov slots per lane (steps stop there)
     LzState st;
 };
 
@@ -127,7 +129,7 @@ struct K1Epi {
 template <typename XT, int B, int FMT, int LM>
 __device__ __forceinline__ K1Epi<XT, B, FMT, LM> k1_setup(const LzArgs<XT>& a) {
     const int b = threadIdx.x % B;  // fixed per thread: LZ_WG is a multiple of B
-    const int j = *a.st.step;
+    const int j = a.st.step[b];
     K1Epi<XT, B, FMT, LM> e{a,
                             a.V + static_cast<int64_t>(j) * a.vs,
                             a.V + static_cast<int64_t>(j > 0 ? j - 1 : 0) * a.vs,
@@ -137,7 +139,7 @@ __device__ __forceinline__ K1Epi<XT, B, FMT, LM> k1_setup(const LzArgs<XT>& a) {
                             a.st.U[b],
                             j,
                             a.st.active[b] ? a.st.nlock[b] : 0,
-                            a.st.active[b] != 0};
+                            a.st.active[b] != 0 && j < a.maxit};
     return e;
 }
 
@@ -194,10 +196,10 @@ template <typename XT, int B>
 __global__ void __launch_bounds__(LZ_WG) k_lz_b(LzArgs<XT> a, int lm) {
     __shared__ double lds[LZ_WG / 64 * B];
     const int b = threadIdx.x % B;
-    const int j = *a.st.step;
+    const int j = a.st.step[b];
     const XT* vj = a.V + static_cast<int64_t>(j) * a.vs;
     XT* vn = a.V + static_cast<int64_t>(j + 1) * a.vs;
-    const bool act = a.st.active[b] != 0;
+    const bool act = a.st.active[b] != 0 && j < a.maxit;
     const double sj = a.st.scale[j * B + b];
     const double al = a.st.sums[b];
     const int nl = act ? min(a.st.nlock[b], lm) : 0;
@@ -260,10 +262,9 @@ constexpr int LZ_RED_BETA = 1;
 
 template <int B>
 __global__ void __launch_bounds__(LZ_RED_WG) k_lz_reduce(LzState st, const double* __restrict__ part, int nwg, int nv,
-                                                         int mode, double* __restrict__ out) {
+                                                         int mode, int maxit, double* __restrict__ out) {
     const int lane = threadIdx.x & 63;
     const int waves = LZ_RED_WG / 64;
-    const int j = mode == LZ_RED_BETA ? *st.step : 0;
     for (int v = blockIdx.x * waves + (threadIdx.x >> 6); v < nv; v += gridDim.x * waves) {
         double s = 0.0;
         for (int w = lane; w < nwg; w += 64) s += part[static_cast<size_t>(w) * nv + v];
@@ -272,17 +273,25 @@ __global__ void __launch_bounds__(LZ_RED_WG) k_lz_reduce(LzState st, const doubl
         if (lane == 0) {
             if (mode == LZ_RED_PLAIN) {
                 out[v] = s;
-            } else if (st.active[v]) {
-                const double nrm = std::sqrt(s);
-                st.beta[(j + 1) * B + v] = nrm;
-                st.scale[(j + 1) * B + v] = nrm > 0.0 ? 1.0 / nrm : 0.0;
+            } else {
+                // v = b; only this lane of this wave reads or writes step[b] in the launch
+                const int j = st.step[v];
+                if (st.active[v] && j < maxit) {
+                    const double nrm = std::sqrt(s);
+                    st.beta[(j + 1) * B + v] = nrm;
+                    st.scale[(j + 1) * B + v] = nrm > 0.0 ? 1.0 / nrm : 0.0;
+                    st.step[v] = j + 1;
+                }
             }
         }
     }
-    if (mode == LZ_RED_BETA) {
-        __syncthreads();  // every thread of the (single) workgroup has read step
-        if (threadIdx.x == 0) *st.step = j + 1;
-    }
+}
+
+// admission of lane `slot`: V[0][r][slot] = r0[r] in storage precision
+template <typename XT, int B>
+__global__ void k_lz_admit(XT* __restrict__ V0, const double* __restrict__ r0, int64_t D, int slot) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r < D) V0[r * B + slot] = static_cast<XT>(r0[r]);
 }
 
 // Ritz vectors: out[i][r][b] = sum_{t < nj[b]} V[t][r][b] * C[(t * nv + i) * B + b]   (C includes scale_t).
@@ -367,7 +376,11 @@ __global__ void k_lz_scale(YT* __restrict__ Y, int64_t vs, int nv, const double*
     Y[idx] = static_cast<YT>(to_d(Y[idx]) * f[i * B + b]);
 }
 
-// ---- host driver --------------------------------------------------------------------------------------
+// ---- host driver: the Lanczos engine --------------------------------------------------------------------
+//
+// An engine holds B lanes over one Hamiltonian.  A lane runs one solve (J_b, U_b); lanes are admitted
+// and retired independently (each keeps its own step counter and Krylov column), so a serving loop can
+// refill a lane as soon as its solve finishes while the others keep iterating.
 
 namespace {
 
@@ -380,7 +393,7 @@ int lz_debug() {
 }
 
 // start vector of the solver (Park-Miller sequence x <- 16807 x mod (2^31 - 1), seed 0 -> 1, values
-// x / (2^31 - 1) - 0.5), shared by every run
+// x / (2^31 - 1) - 0.5), shared by every lane
 std::vector<double> lz_start_vector(int64_t D, uint64_t seed) {
     std::vector<double> r(static_cast<size_t>(D));
     const uint64_t p = 2147483647ull;
@@ -398,7 +411,7 @@ struct RunHost {
     bool done = false;
     int iters = 0;
     int status = BH_ENOCONV;
-    std::vector<double> theta;           // wanted levels (Ritz values of T)
+    std::vector<double> theta;           // wanted levels and extra candidates (Ritz values of T)
     std::vector<std::vector<double>> s;  // their eigenvectors of T
     double tnorm = 0.0;
     double tnorm0 = 0.0;                 // ||T|| at the first analysis (divergence guard)
@@ -437,208 +450,201 @@ int dalloc(bh_ctx* ctx, DevArray<T>& a, size_t n) {
 }  // namespace
 
 template <typename XT, int B>
-static int lanczos_impl(bh_ctx* ctx, bh_ham* h, const double* Jh, const double* Uh, int nreal,
-                        const bh_lanczos_opts& o, bh_lanczos_result* res, std::vector<double>* vecs_out) {
-    constexpr bool MIXED = std::is_same<XT, float>::value;
-    constexpr int FMT = MIXED ? BH_FMT_PACKED : BH_FMT_CSR64;
-    hipStream_t s = ctx->stream;
-    const int64_t D = h->D;
-    const int k = o.k;
-    int maxit = o.max_iter > 0 ? o.max_iter : 1000;
-    if (maxit > D) maxit = static_cast<int>(D);
-    const int chunk = o.chunk > 0 ? o.chunk : 16;
-    const double tol = o.tol > 0 ? o.tol : 1e-10;
-    const double dedup = o.dedup_rel > 0 ? o.dedup_rel : 1e-9;
-    const double eps_store = MIXED ? 5.96e-8 : 1.11e-16;
-    const double eps23 = std::pow(2.220446049250313e-16, 2.0 / 3.0);
-    const int64_t vs = D * B;
+class LzEngine final : public LzEngineBase {
+    static constexpr bool MIXED = std::is_same<XT, float>::value;
+    static constexpr int FMT = MIXED ? BH_FMT_PACKED : BH_FMT_CSR64;
 
-    // ---- device buffers: the Krylov basis, W and the locked vectors live in context workspaces ----
-    struct WsView {
-        XT* p;
-    } V{nullptr}, Y{nullptr};
-    struct WsViewD {
-        double* p;
-    } W{nullptr};
-    {
-        void* q = nullptr;
-        BH_TRY(bh_ws_get(ctx, 0, static_cast<size_t>(maxit + 1) * vs * sizeof(XT), &q));
-        V.p = static_cast<XT*>(q);
-        BH_TRY(bh_ws_get(ctx, 1, static_cast<size_t>(vs) * sizeof(double), &q));
-        W.p = static_cast<double*>(q);
-        BH_TRY(bh_ws_get(ctx, 2, static_cast<size_t>(LZ_LMAX) * vs * sizeof(XT), &q));
-        Y.p = static_cast<XT*>(q);
-    }
-    DevArray<double> st_d, part;
-    DevArray<int> st_i;
-    const size_t n_alpha = static_cast<size_t>(maxit + 1) * B, n_beta = static_cast<size_t>(maxit + 2) * B;
-    const size_t n_sums = static_cast<size_t>(1 + 2 * LZ_LMAX) * B;
-    const size_t nd = n_alpha + 2 * n_beta + 2 * B + n_sums;
-    BH_TRY(dalloc(ctx, st_d, nd));
-    BH_TRY(dalloc(ctx, st_i, static_cast<size_t>(2 * B + 1)));
-    const int nvmax = std::max(static_cast<int>(n_sums), 2 * LZ_NVMAX * B);
-    BH_TRY(dalloc(ctx, part, static_cast<size_t>(LZ_GRID_MAX) * nvmax));
+   public:
+    LzEngine(bh_ctx* c, bh_ham* ham, const bh_lanczos_opts& opts, hipStream_t stream, bool use_ctx_ws)
+        : ctx(c), h(ham), s(stream), o(opts), ws(use_ctx_ws) {}
 
-    LzState st;
-    st.alpha = st_d.p;
-    st.beta = st.alpha + n_alpha;
-    st.scale = st.beta + n_beta;
-    st.J = st.scale + n_beta;
-    st.U = st.J + B;
-    st.sums = st.U + B;
-    st.active = st_i.p;
-    st.nlock = st_i.p + B;
-    st.step = st_i.p + 2 * B;
-    st.part = part.p;
-
-    std::vector<double> Jb(B, 0.0), Ub(B, 0.0);
-    std::vector<int> act(B, 0), nlock(B, 0);
-    for (int b = 0; b < nreal; ++b) {
-        Jb[b] = Jh[b];
-        Ub[b] = Uh[b];
-        act[b] = 1;
-    }
-    BH_HIP(ctx, hipMemsetAsync(st_d.p, 0, nd * sizeof(double), s));
-    BH_HIP(ctx, hipMemsetAsync(st_i.p, 0, st_i.bytes(), s));
-    BH_HIP(ctx, hipMemcpyAsync(st.J, Jb.data(), B * 8, hipMemcpyHostToDevice, s));
-    BH_HIP(ctx, hipMemcpyAsync(st.U, Ub.data(), B * 8, hipMemcpyHostToDevice, s));
-    BH_HIP(ctx, hipMemcpyAsync(st.active, act.data(), B * 4, hipMemcpyHostToDevice, s));
-    // start vector: cached on the device per (D, seed); V[0] = r in storage precision, scale_0 = 1/||V[0]||
-    {
-        void* q = nullptr;
-        BH_TRY(bh_ws_get(ctx, 3, static_cast<size_t>(D) * sizeof(double), &q));
-        double* dr = static_cast<double*>(q);
+    int setup() {
+        D = h->D;
+        vs = D * B;
+        k = o.k;
+        maxit = o.max_iter > 0 ? o.max_iter : 1000;
+        if (maxit > D) maxit = static_cast<int>(D);
+        chunk = o.chunk > 0 ? o.chunk : 16;
+        tol = o.tol > 0 ? o.tol : 1e-10;
+        dedup = o.dedup_rel > 0 ? o.dedup_rel : 1e-9;
+        eps_store = MIXED ? 5.96e-8 : 1.11e-16;
+        eps23 = std::pow(2.220446049250313e-16, 2.0 / 3.0);
+        const size_t nV = static_cast<size_t>(maxit + 1) * vs, nY = static_cast<size_t>(LZ_LMAX) * vs;
+        if (ws) {
+            void* q = nullptr;
+            BH_TRY(bh_ws_get(ctx, 0, nV * sizeof(XT), &q));
+            V = static_cast<XT*>(q);
+            BH_TRY(bh_ws_get(ctx, 1, static_cast<size_t>(vs) * sizeof(double), &q));
+            W = static_cast<double*>(q);
+            BH_TRY(bh_ws_get(ctx, 2, nY * sizeof(XT), &q));
+            Y = static_cast<XT*>(q);
+            BH_TRY(bh_ws_get(ctx, 3, static_cast<size_t>(D) * sizeof(double), &q));
+            r0 = static_cast<double*>(q);
+        } else {
+            BH_TRY(dalloc(ctx, V_own, nV));
+            BH_TRY(dalloc(ctx, W_own, static_cast<size_t>(vs)));
+            BH_TRY(dalloc(ctx, Y_own, nY));
+            BH_TRY(dalloc(ctx, r0_own, static_cast<size_t>(D)));
+            V = V_own.p;
+            W = W_own.p;
+            Y = Y_own.p;
+            r0 = r0_own.p;
+        }
+        n_alpha = static_cast<size_t>(maxit + 1) * B;
+        n_beta = static_cast<size_t>(maxit + 2) * B;
+        const size_t n_sums = static_cast<size_t>(1 + 2 * LZ_LMAX) * B;
+        const size_t nd = n_alpha + 2 * n_beta + 2 * B + n_sums;
+        BH_TRY(dalloc(ctx, st_d, nd));
+        BH_TRY(dalloc(ctx, st_i, static_cast<size_t>(3 * B)));
+        const int nvmax = std::max(static_cast<int>(n_sums), 2 * LZ_NVMAX * B);
+        BH_TRY(dalloc(ctx, part, static_cast<size_t>(LZ_GRID_MAX) * nvmax));
+        BH_TRY(dalloc(ctx, dnorm, static_cast<size_t>(2 * LZ_NVMAX * B)));
+        BH_TRY(dalloc(ctx, dnj, B));
+        st.alpha = st_d.p;
+        st.beta = st.alpha + n_alpha;
+        st.scale = st.beta + n_beta;
+        st.J = st.scale + n_beta;
+        st.U = st.J + B;
+        st.sums = st.U + B;
+        st.active = st_i.p;
+        st.nlock = st_i.p + B;
+        st.step = st_i.p + 2 * B;
+        st.part = part.p;
+        BH_HIP(ctx, hipMemsetAsync(st_d.p, 0, nd * sizeof(double), s));
+        BH_HIP(ctx, hipMemsetAsync(st_i.p, 0, st_i.bytes(), s));
+        // start vector (cached in the context workspace per (D, seed) when the workspace is used)
         std::vector<double> r = lz_start_vector(D, o.seed);
-        if (ctx->r0_D != D || ctx->r0_seed != o.seed) {
-            BH_HIP(ctx, hipMemcpyAsync(dr, r.data(), D * 8, hipMemcpyHostToDevice, s));
+        if (!ws || ctx->r0_D != D || ctx->r0_seed != o.seed) {
+            BH_HIP(ctx, hipMemcpyAsync(r0, r.data(), D * 8, hipMemcpyHostToDevice, s));
             BH_HIP(ctx, hipStreamSynchronize(s));
-            ctx->r0_D = D;
-            ctx->r0_seed = o.seed;
+            if (ws) {
+                ctx->r0_D = D;
+                ctx->r0_seed = o.seed;
+            }
         }
         double nrm = 0.0;
         for (int64_t i = 0; i < D; ++i) {
             const double v = static_cast<double>(static_cast<XT>(r[i]));
             nrm += v * v;
         }
-        std::vector<double> sc(B, nrm > 0 ? 1.0 / std::sqrt(nrm) : 0.0);
-        BH_HIP(ctx, hipMemcpyAsync(st.scale, sc.data(), B * 8, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL((k_lz_init<XT, B>), dim3(static_cast<unsigned>((vs + 255) / 256)), dim3(256), 0, s, V.p, dr, D);
-        BH_HIP(ctx, hipGetLastError());
-        BH_HIP(ctx, hipStreamSynchronize(s));  // sc and r are locals
-    }
-
-    LzArgs<XT> a{D,
-                 h->row_ptr,
-                 h->words,
-                 h->col64,
-                 h->val64,
-                 MIXED ? h->diag : nullptr,
-                 MIXED ? nullptr : h->diag64,
-                 h->table,
-                 h->code_bits,
-                 V.p,
-                 vs,
-                 W.p,
-                 Y.p,
-                 st};
-    const size_t shm = FMT == BH_FMT_PACKED ? (static_cast<size_t>(1) << h->code_bits) * sizeof(double) : 0;
-    const int engine = bh_engine_choice(B, h) == 1 ? 1 : 2;  // staged row blocks or one thread per (row, b)
-    const int32_t* rblk = nullptr;
-    int64_t nrblk = 0;
-    if (engine == 1) BH_TRY(bh_ham_rowblocks(h, B, &rblk, &nrblk));
-    const int grid_rb = static_cast<int>(std::min<int64_t>(std::max<int64_t>(nrblk, 1), LZ_GRID_MAX));
-    const size_t shm_rb = shm + BH_STAGE_PRODUCTS * sizeof(double);
-    const int64_t rpw = (D + LZ_GRID_MAX - 1) / LZ_GRID_MAX;
-    const int grid_rl = static_cast<int>((D + rpw - 1) / rpw);
-    const int grid_k1 = engine == 1 ? grid_rb : grid_rl;
-    const int64_t ntile_b = (D + LZ_WG / B - 1) / (LZ_WG / B);
-    const int grid_k2 = static_cast<int>(std::min<int64_t>(ntile_b, LZ_GRID_MAX));
-
-    auto launch_k1 = [&](int LM) {
-        KTimer kt(ctx, "lanczos_spmv", s);
-#define K1L(LMV)                                                                                                       \
-    if (LM == LMV) {                                                                                                   \
-        if (engine == 1)                                                                                               \
-            hipLaunchKernelGGL((k_lz_a_rb<XT, B, FMT, LMV>), dim3(grid_rb), dim3(LZ_WG), shm_rb, s, a, rblk, nrblk);   \
-        else                                                                                                           \
-            hipLaunchKernelGGL((k_lz_a_rl<XT, B, FMT, LMV>), dim3(grid_rl), dim3(LZ_WG), shm, s, a, rpw);              \
-        return;                                                                                                        \
-    }
-        K1L(0) K1L(4) K1L(16)
-#undef K1L
-    };
-    auto reduce_plain = [&](int nwg, int nv, double* out) {
-        KTimer kt(ctx, "lanczos_reduce", s);
-        const int waves = LZ_RED_WG / 64;
-        const int g = std::max(1, std::min((nv + waves - 1) / waves, 64));
-        hipLaunchKernelGGL((k_lz_reduce<B>), dim3(g), dim3(LZ_RED_WG), 0, s, st, part.p, nwg, nv, LZ_RED_PLAIN, out);
-    };
-    auto step = [&](int LM) {
-        launch_k1(LM);
-        reduce_plain(grid_k1, (1 + 2 * LM) * B, st.sums);
-        {
-            KTimer kt(ctx, "lanczos_update", s);
-            hipLaunchKernelGGL((k_lz_b<XT, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, a, LM);
-        }
-        {
-            KTimer kt(ctx, "lanczos_reduce", s);
-            hipLaunchKernelGGL((k_lz_reduce<B>), dim3(1), dim3(LZ_RED_WG), 0, s, st, part.p, grid_k2, B, LZ_RED_BETA,
-                               nullptr);
-        }
-    };
-    const int grid = grid_k2;  // workgroups of the dot-product kernels
-
-    std::vector<RunHost> rh(B);
-    std::vector<double> hlo(B, 0.0), hhi(B, 0.0);  // Gershgorin interval of H_b
-    for (int b = 0; b < nreal; ++b) {
-        const double va = Ub[b] * h->vmin, vb = Ub[b] * h->vmax;
-        hlo[b] = std::fmin(va, vb) - std::fabs(Jb[b]) * h->tmax;
-        hhi[b] = std::fmax(va, vb) + std::fabs(Jb[b]) * h->tmax;
-    }
-    std::vector<double> ha(n_alpha), hb(n_beta);
-    int j = 0;          // steps done
-    int lmax_used = 0;  // max locks over runs
-    int next_check = std::max(2 * k + 8, 20);
-    int nleft = nreal;
-    std::vector<double> Cbuf;
-    DevArray<double> dC, dnorm;
-    BH_TRY(dalloc(ctx, dnorm, static_cast<size_t>(2 * LZ_NVMAX * B)));
-
-    DevArray<int> dnj;
-    BH_TRY(dalloc(ctx, dnj, B));
-    auto form_vectors = [&](const std::vector<int>& njb, int nv, const std::vector<double>& C, auto* out,
-                            int64_t os) -> int {
-        // C: [max nj][nv][B]; njb[b]: Krylov vectors used for run b (0 for runs not formed)
-        BH_TRY(dC.alloc(ctx, C.size()));
-        BH_HIP(ctx, hipMemcpyAsync(dC.p, C.data(), C.size() * 8, hipMemcpyHostToDevice, s));
-        BH_HIP(ctx, hipMemcpyAsync(dnj.p, njb.data(), B * sizeof(int), hipMemcpyHostToDevice, s));
-        using OT = std::remove_pointer_t<decltype(out)>;
-        KTimer kt(ctx, "lanczos_ritz", s);
-        hipLaunchKernelGGL((k_lz_ritz<XT, OT, B>), dim3(static_cast<unsigned>((vs + 255) / 256)), dim3(256), 0, s, V.p, vs,
-                           dnj.p, dC.p, nv, out, os);
-        BH_HIP(ctx, hipGetLastError());
-        BH_HIP(ctx, hipStreamSynchronize(s));
+        scale0 = nrm > 0 ? 1.0 / std::sqrt(nrm) : 0.0;
+        a = LzArgs<XT>{D,
+                       h->row_ptr,
+                       h->words,
+                       h->col64,
+                       h->val64,
+                       MIXED ? h->diag : nullptr,
+                       MIXED ? nullptr : h->diag64,
+                       h->table,
+                       h->code_bits,
+                       V,
+                       vs,
+                       W,
+                       Y,
+                       maxit,
+                       st};
+        shm = FMT == BH_FMT_PACKED ? (static_cast<size_t>(1) << h->code_bits) * sizeof(double) : 0;
+        engine = bh_engine_choice(B, h) == 1 ? 1 : 2;  // staged row blocks, or one thread per (row, b)
+        if (engine == 1) BH_TRY(bh_ham_rowblocks(h, B, &rblk, &nrblk));
+        grid_rb = static_cast<int>(std::min<int64_t>(std::max<int64_t>(nrblk, 1), LZ_GRID_MAX));
+        shm_rb = shm + BH_STAGE_PRODUCTS * sizeof(double);
+        rpw = (D + LZ_GRID_MAX - 1) / LZ_GRID_MAX;
+        grid_rl = static_cast<int>((D + rpw - 1) / rpw);
+        grid_k1 = engine == 1 ? grid_rb : grid_rl;
+        const int64_t ntile_b = (D + LZ_WG / B - 1) / (LZ_WG / B);
+        grid_k2 = static_cast<int>(std::min<int64_t>(ntile_b, LZ_GRID_MAX));
+        rh.assign(B, RunHost());
+        nextc.assign(B, 0);
+        act.assign(B, 0);
+        nlock.assign(B, 0);
+        jl.assign(B, 0);
+        Jb.assign(B, 0.0);
+        Ub.assign(B, 0.0);
+        hlo.assign(B, 0.0);
+        hhi.assign(B, 0.0);
+        for (auto& R : rh) R.done = true;
+        bytes = st_d.bytes() + st_i.bytes() + part.bytes() + dnorm.bytes() +
+                (ws ? 0 : V_own.bytes() + W_own.bytes() + Y_own.bytes() + r0_own.bytes());
         return BH_OK;
-    };
+    }
 
-    while (j < maxit && nleft > 0) {
-        const int steps = std::min(std::min(next_check, maxit) - j, maxit - j);
+    int lanes() const override { return B; }
+    int active() const override {
+        int c = 0;
+        for (int b = 0; b < B; ++b) c += act[b];
+        return c;
+    }
+    int steps_of(int slot) const override { return jl[slot]; }
+    size_t device_bytes() const override { return bytes; }
+    // Steps until the next lane reaches its own analysis point.  Each lane follows its own schedule
+    // (first at max(2k+8, 20) steps, then every max(chunk, j/4): the analysis cost grows with j), so the
+    // steps at which a lane is analysed, locked and retired do not depend on the other lanes.
+    int suggest_chunk() const override {
+        int st = -1;
+        for (int b = 0; b < B; ++b)
+            if (act[b]) {
+                const int d = std::max(1, std::min(nextc[b], maxit) - jl[b]);
+                st = st < 0 ? d : std::min(st, d);
+            }
+        return st < 0 ? chunk : st;
+    }
+
+    int admit(int slot, double J, double U) override {
+        if (slot < 0 || slot >= B || act[slot]) return bh_set_error(ctx, BH_EINVAL, "engine: lane busy or out of range");
+        rh[slot] = RunHost();
+        act[slot] = 1;
+        nlock[slot] = 0;
+        jl[slot] = 0;
+        nextc[slot] = std::max(2 * k + 8, 20);
+        Jb[slot] = J;
+        Ub[slot] = U;
+        const double va = U * h->vmin, vb = U * h->vmax;
+        hlo[slot] = std::fmin(va, vb) - std::fabs(J) * h->tmax;
+        hhi[slot] = std::fmax(va, vb) + std::fabs(J) * h->tmax;
+        const int one = 1, zero = 0;
+        const double dz = 0.0;
+        BH_HIP(ctx, hipMemcpyAsync(st.J + slot, &Jb[slot], 8, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(st.U + slot, &Ub[slot], 8, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(st.scale + slot, &scale0, 8, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(st.beta + slot, &dz, 8, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(st.nlock + slot, &zero, 4, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(st.step + slot, &zero, 4, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(st.active + slot, &one, 4, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL((k_lz_admit<XT, B>), dim3(static_cast<unsigned>((D + 255) / 256)), dim3(256), 0, s, V, r0, D,
+                           slot);
+        BH_HIP(ctx, hipGetLastError());
+        BH_HIP(ctx, hipStreamSynchronize(s));  // the sources of the copies are host locals/members
+        return BH_OK;
+    }
+
+    int run(int steps) override {
         const int LM = lmax_used == 0 ? 0 : (lmax_used <= 4 ? 4 : 16);
         for (int t = 0; t < steps; ++t) step(LM);
         BH_HIP(ctx, hipGetLastError());
-        j += steps;
-        BH_HIP(ctx, hipMemcpyAsync(ha.data(), st.alpha, static_cast<size_t>(j) * B * 8, hipMemcpyDeviceToHost, s));
-        BH_HIP(ctx, hipMemcpyAsync(hb.data(), st.beta, static_cast<size_t>(j + 1) * B * 8, hipMemcpyDeviceToHost, s));
-        BH_HIP(ctx, hipStreamSynchronize(s));
-        next_check = j + std::max(chunk, j / 4);  // analysis cost grows with j: space the checks out
+        for (int b = 0; b < B; ++b)
+            if (act[b]) jl[b] = std::min(jl[b] + steps, maxit);
+        return BH_OK;
+    }
 
-        // analysis per active run (independent runs analysed on parallel host threads)
+    int check(std::vector<int>& finished) override {
+        finished.clear();
+        int jmax = 0;
+        for (int b = 0; b < B; ++b)
+            if (act[b]) jmax = std::max(jmax, jl[b]);
+        if (jmax == 0) return BH_OK;
+        ha.resize(static_cast<size_t>(jmax) * B);
+        hb.resize(static_cast<size_t>(jmax + 1) * B);
+        BH_HIP(ctx, hipMemcpyAsync(ha.data(), st.alpha, ha.size() * 8, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipMemcpyAsync(hb.data(), st.beta, hb.size() * 8, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipStreamSynchronize(s));
         std::vector<std::vector<int>> new_locks(B);
         std::vector<Analysis> ans(B);
         std::vector<int> done_now(B, 0);
         auto analyse_run = [&](int b) {
             RunHost& R = rh[b];
-            if (R.done) return;
+            const int j = jl[b];
+            if (j < nextc[b] && j < maxit) return;
+            nextc[b] = j + std::max(chunk, j / 4);
             R.alpha.assign(j, 0.0);
             R.beta.assign(j + 1, 0.0);
             for (int t = 0; t < j; ++t) R.alpha[t] = ha[static_cast<size_t>(t) * B + b];
@@ -663,7 +669,7 @@ static int lanczos_impl(bh_ctx* ctx, bh_ham* h, const double* Jh, const double* 
             analyze(R, jr, bnext, p, an);
             // Breakdown guard: Ritz values are Rayleigh quotients of H and must lie in its Gershgorin
             // interval; values outside it (or non-finite coefficients) mean the Lanczos vectors lost
-            // orthogonality beyond repair.  The run stops unconverged (mixed runs are rerun in FP64).
+            // orthogonality beyond repair.  The lane stops unconverged (mixed runs are rerun in FP64).
             bool broken = !std::isfinite(an.tnorm) || !std::isfinite(bnext) || (R.tnorm0 > 0 && an.tnorm > 1e3 * R.tnorm0);
             if (!broken && !an.theta.empty()) {
                 const double slack = 1e-6 * std::fmax(std::fmax(std::fabs(hlo[b]), std::fabs(hhi[b])), 1.0);
@@ -681,6 +687,12 @@ static int lanczos_impl(bh_ctx* ctx, bh_ham* h, const double* Jh, const double* 
             if (R.tnorm0 == 0.0) R.tnorm0 = an.tnorm;
             const bool exhausted = jr < j || bnext <= std::fmax(1e-12, 100.0 * eps_store) * std::fmax(an.tnorm, 1e-300) ||
                                    j >= D;
+            if (lz_debug()) {
+                std::fprintf(stderr, "[lz] j=%d lane=%d tnorm=%.3e bnext=%.3e locks=%d:", j, b, an.tnorm, bnext, nlock[b]);
+                for (size_t i = 0; i < an.theta.size() && i < 8; ++i)
+                    std::fprintf(stderr, " (%.10f %.1e)", an.theta[i], an.bound[i]);
+                std::fprintf(stderr, "\n");
+            }
             // FP32-stored vectors: Ritz residuals cannot fall below the storage rounding, ~eps32 ||T||
             const double floor = MIXED ? 4.0 * eps_store * an.tnorm : 0.0;
             // walk distinct levels from the bottom.  With FP32-stored vectors, copies of a degenerate level
@@ -698,12 +710,6 @@ static int lanczos_impl(bh_ctx* ctx, bh_ham* h, const double* Jh, const double* 
                  i < static_cast<int>(an.theta.size()) && static_cast<int>(cand.size()) < std::min(k + 3, LZ_NVMAX); ++i) {
                 if (std::fabs(an.theta[i] - an.theta[cand.back()]) <= tdedup * an.tnorm) continue;
                 cand.push_back(i);
-            }
-            if (lz_debug()) {
-                std::fprintf(stderr, "[lz] j=%d run=%d tnorm=%.3e bnext=%.3e locks=%d:", j, b, an.tnorm, bnext, nlock[b]);
-                for (size_t i = 0; i < an.theta.size() && i < 8; ++i)
-                    std::fprintf(stderr, " (%.10f %.1e)", an.theta[i], an.bound[i]);
-                std::fprintf(stderr, "\n");
             }
             bool conv = static_cast<int>(lev.size()) == k || exhausted;
             for (int i : lev) {
@@ -744,8 +750,8 @@ static int lanczos_impl(bh_ctx* ctx, bh_ham* h, const double* Jh, const double* 
         };
         {
             std::vector<int> todo;
-            for (int b = 0; b < nreal; ++b)
-                if (!rh[b].done) todo.push_back(b);
+            for (int b = 0; b < B; ++b)
+                if (act[b]) todo.push_back(b);
             const int nth = std::max(1, std::min(static_cast<int>(todo.size()),
                                                  static_cast<int>(std::thread::hardware_concurrency())));
             if (nth <= 1) {
@@ -760,218 +766,341 @@ static int lanczos_impl(bh_ctx* ctx, bh_ham* h, const double* Jh, const double* 
             }
         }
         bool any_lock = false, any_done = false;
-        for (int b = 0; b < nreal; ++b) {
+        for (int b = 0; b < B; ++b) {
             if (done_now[b]) {
                 act[b] = 0;
                 any_done = true;
-                --nleft;
+                finished.push_back(b);
             }
             if (!new_locks[b].empty()) any_lock = true;
         }
-        if (any_lock) {
-            int nv = 0;
-            for (int b = 0; b < nreal; ++b) nv = std::max(nv, static_cast<int>(new_locks[b].size()));
-            nv = std::min(nv, LZ_NVMAX);
-            // C[t][i][b] = s_i[t] * scale_t[b]
-            std::vector<double> hsc(static_cast<size_t>(j) * B);
-            BH_HIP(ctx, hipMemcpyAsync(hsc.data(), st.scale, hsc.size() * 8, hipMemcpyDeviceToHost, s));
-            BH_HIP(ctx, hipStreamSynchronize(s));
-            Cbuf.assign(static_cast<size_t>(j) * nv * B, 0.0);
-            int base = 0;  // all runs write new vectors at slot nlock[b] + i; form into a staging area
-            (void)base;
-            // stage new Ritz vectors at Y slots [L0, L0+nv) where L0 = max nlock: runs with fewer locks are
-            // compacted afterwards by writing at their own nlock offset (per-run offsets differ), so form
-            // one vector index at a time per slot offset.
-            for (int i = 0; i < nv; ++i) {
-                for (int b = 0; b < nreal; ++b) {
-                    if (i >= static_cast<int>(new_locks[b].size())) continue;
-                    const std::vector<double>& sv = ans[b].s[new_locks[b][i]];
-                    for (int t = 0; t < j; ++t)
-                        Cbuf[(static_cast<size_t>(t) * nv + i) * B + b] = sv[t] * hsc[static_cast<size_t>(t) * B + b];
-                }
-            }
-            // form into W-sized scratch slots at the top of Y, then move per run: Y slots are per (l, b);
-            // writing vector i of run b to slot nlock[b] + i requires per-b offsets, so form into a
-            // temporary [nv][D][B] and copy the columns on the device.
-            DevArray<XT> tmp;
-            BH_TRY(tmp.alloc(ctx, static_cast<size_t>(nv) * vs));
-            std::vector<int> njb(B, 0);
-            for (int b = 0; b < nreal; ++b)
-                if (!new_locks[b].empty()) njb[b] = j;
-            BH_TRY(form_vectors(njb, nv, Cbuf, tmp.p, vs));
-            // normalise: ||y||^2 per (i, b)
-            hipLaunchKernelGGL((k_lz_dots<XT, B>), dim3(grid), dim3(LZ_WG), 0, s, tmp.p, nullptr, D, nv, 2, nullptr, part.p);
-            reduce_plain(grid, nv * B, dnorm.p);
-            std::vector<double> nrm(static_cast<size_t>(nv) * B);
-            BH_HIP(ctx, hipMemcpyAsync(nrm.data(), dnorm.p, nrm.size() * 8, hipMemcpyDeviceToHost, s));
-            BH_HIP(ctx, hipStreamSynchronize(s));
-            for (auto& v : nrm) v = v > 0 ? 1.0 / std::sqrt(v) : 0.0;
-            BH_HIP(ctx, hipMemcpyAsync(dnorm.p, nrm.data(), nrm.size() * 8, hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL((k_lz_scale<XT, B>), dim3(static_cast<unsigned>((vs * nv + 255) / 256)), dim3(256), 0, s,
-                               tmp.p, vs, nv, dnorm.p);
-            // scatter vector i of run b into Y slot nlock[b] + i (strided copy of one interleaved column)
-            for (int b = 0; b < nreal; ++b) {
-                for (int i = 0; i < static_cast<int>(new_locks[b].size()) && i < nv; ++i) {
-                    const int slot = nlock[b] + i;
-                    BH_HIP(ctx, hipMemcpy2DAsync(Y.p + static_cast<int64_t>(slot) * vs + b, B * sizeof(XT),
-                                                 tmp.p + static_cast<int64_t>(i) * vs + b, B * sizeof(XT), sizeof(XT),
-                                                 static_cast<size_t>(D), hipMemcpyDeviceToDevice, s));
-                    rh[b].lock_theta.push_back(ans[b].theta[new_locks[b][i]]);
-                }
-                nlock[b] += std::min(static_cast<int>(new_locks[b].size()), nv);
-                lmax_used = std::max(lmax_used, nlock[b]);
-            }
-            BH_HIP(ctx, hipMemcpyAsync(st.nlock, nlock.data(), B * 4, hipMemcpyHostToDevice, s));
-            BH_HIP(ctx, hipStreamSynchronize(s));
-        }
+        if (any_lock) BH_TRY(lock(new_locks, ans));
         if (any_done) {
             BH_HIP(ctx, hipMemcpyAsync(st.active, act.data(), B * 4, hipMemcpyHostToDevice, s));
             BH_HIP(ctx, hipStreamSynchronize(s));
         }
-    }
-    for (int b = 0; b < nreal; ++b) {
-        if (!rh[b].done) {  // ran out of steps
-            RunHost& R = rh[b];
-            R.done = true;
-            R.iters = j;
-            R.status = BH_ENOCONV;
-        }
+        return BH_OK;
     }
 
-    // ---- FP64 finish: Ritz vectors, Rayleigh quotients and residuals with the exact values ----
-    std::vector<double> hsc(static_cast<size_t>(j + 1) * B);
-    BH_HIP(ctx, hipMemcpyAsync(hsc.data(), st.scale, hsc.size() * 8, hipMemcpyDeviceToHost, s));
-    BH_HIP(ctx, hipStreamSynchronize(s));
-    int nvf = 0;
-    for (int b = 0; b < nreal; ++b) nvf = std::max(nvf, static_cast<int>(rh[b].theta.size()));
-    nvf = std::min(nvf, LZ_NVMAX);
-    std::vector<double> theta_f(static_cast<size_t>(nvf) * B, 0.0), resid_f(static_cast<size_t>(nvf) * B, 0.0);
-    if (nvf > 0) {
-        int njmax = 0;
-        for (int b = 0; b < nreal; ++b)
-            if (!rh[b].s.empty()) njmax = std::max(njmax, static_cast<int>(rh[b].s[0].size()));
-        Cbuf.assign(static_cast<size_t>(njmax) * nvf * B, 0.0);
-        for (int b = 0; b < nreal; ++b)
-            for (int i = 0; i < static_cast<int>(rh[b].s.size()) && i < nvf; ++i)
-                for (int t = 0; t < static_cast<int>(rh[b].s[i].size()); ++t)
-                    Cbuf[(static_cast<size_t>(t) * nvf + i) * B + b] = rh[b].s[i][t] * hsc[static_cast<size_t>(t) * B + b];
-        DevArray<double> Yf, HY;
-        BH_TRY(Yf.alloc(ctx, static_cast<size_t>(nvf) * vs));
-        BH_TRY(HY.alloc(ctx, static_cast<size_t>(nvf) * vs));
-        std::vector<int> njb(B, 0);
-        for (int b = 0; b < nreal; ++b)
-            if (!rh[b].s.empty()) njb[b] = static_cast<int>(rh[b].s[0].size());
-        BH_TRY(form_vectors(njb, nvf, Cbuf, Yf.p, vs));
-        // exact FP64 operator on each Ritz vector (packed words, FP64 table, FP64 vectors)
-        DevArray<double> dJ, dU;
-        BH_TRY(dJ.alloc(ctx, B));
-        BH_TRY(dU.alloc(ctx, B));
-        BH_HIP(ctx, hipMemcpyAsync(dJ.p, Jb.data(), B * 8, hipMemcpyHostToDevice, s));
-        BH_HIP(ctx, hipMemcpyAsync(dU.p, Ub.data(), B * 8, hipMemcpyHostToDevice, s));
-        const int Gf = bh_pick_lanes(h, B);
-        for (int i = 0; i < nvf; ++i) {
-            KTimer kt(ctx, "lanczos_finish_spmv", s);
-            SpmvArgs<double> sa{D, h->row_ptr, h->words, nullptr, nullptr, h->diag, nullptr, h->table, h->code_bits,
-                                dJ.p, dU.p, Yf.p + static_cast<int64_t>(i) * vs, HY.p + static_cast<int64_t>(i) * vs, 1};
-            bh_launch_spmv(sa, B, Gf, BH_FMT_PACKED, s);
-        }
-        BH_HIP(ctx, hipGetLastError());
-        std::vector<double> dots(static_cast<size_t>(2 * nvf) * B);
-        hipLaunchKernelGGL((k_lz_dots<double, B>), dim3(grid), dim3(LZ_WG), 0, s, Yf.p, HY.p, D, nvf, 0, nullptr, part.p);
-        reduce_plain(grid, 2 * nvf * B, dnorm.p);
-        BH_HIP(ctx, hipMemcpyAsync(dots.data(), dnorm.p, dots.size() * 8, hipMemcpyDeviceToHost, s));
+    int finish(const std::vector<int>& lanes, std::vector<LaneResult>& out, bool want_vectors) override {
+        out.assign(lanes.size(), LaneResult());
+        int jmax = 0;
+        for (int b : lanes) jmax = std::max(jmax, jl[b]);
+        std::vector<double> hsc(static_cast<size_t>(jmax + 1) * B);
+        BH_HIP(ctx, hipMemcpyAsync(hsc.data(), st.scale, hsc.size() * 8, hipMemcpyDeviceToHost, s));
         BH_HIP(ctx, hipStreamSynchronize(s));
-        for (int i = 0; i < nvf; ++i)
-            for (int b = 0; b < B; ++b) {
-                const double yy = dots[(2 * i) * B + b], yhy = dots[(2 * i + 1) * B + b];
-                theta_f[i * B + b] = yy > 0 ? yhy / yy : NAN;
+        int nvf = 0;
+        for (int b : lanes) nvf = std::max(nvf, static_cast<int>(rh[b].theta.size()));
+        nvf = std::min(nvf, LZ_NVMAX);
+        std::vector<double> theta_f(static_cast<size_t>(std::max(nvf, 1)) * B, NAN), resid_f(theta_f.size(), NAN);
+        std::vector<double> dots;
+        std::vector<double> hy;
+        if (nvf > 0) {
+            int njmax = 0;
+            std::vector<int> njb(B, 0);
+            for (int b : lanes)
+                if (!rh[b].s.empty()) {
+                    njb[b] = static_cast<int>(rh[b].s[0].size());
+                    njmax = std::max(njmax, njb[b]);
+                }
+            std::vector<double> C(static_cast<size_t>(std::max(njmax, 1)) * nvf * B, 0.0);
+            for (int b : lanes)
+                for (int i = 0; i < static_cast<int>(rh[b].s.size()) && i < nvf; ++i)
+                    for (int t = 0; t < static_cast<int>(rh[b].s[i].size()); ++t)
+                        C[(static_cast<size_t>(t) * nvf + i) * B + b] = rh[b].s[i][t] * hsc[static_cast<size_t>(t) * B + b];
+            DevArray<double> Yf, HY;
+            BH_TRY(Yf.alloc(ctx, static_cast<size_t>(nvf) * vs));
+            BH_TRY(HY.alloc(ctx, static_cast<size_t>(nvf) * vs));
+            BH_TRY(form_vectors(njb, nvf, C, Yf.p, vs));
+            // exact FP64 operator on each Ritz vector (packed words, FP64 table, FP64 vectors)
+            DevArray<double> dJ, dU;
+            BH_TRY(dJ.alloc(ctx, B));
+            BH_TRY(dU.alloc(ctx, B));
+            BH_HIP(ctx, hipMemcpyAsync(dJ.p, Jb.data(), B * 8, hipMemcpyHostToDevice, s));
+            BH_HIP(ctx, hipMemcpyAsync(dU.p, Ub.data(), B * 8, hipMemcpyHostToDevice, s));
+            for (int i = 0; i < nvf; ++i) {
+                KTimer kt(ctx, "lanczos_finish_spmv", s);
+                SpmvArgs<double> sa{D, h->row_ptr, h->words, nullptr, nullptr, h->diag, nullptr, h->table, h->code_bits,
+                                    dJ.p, dU.p, Yf.p + static_cast<int64_t>(i) * vs, HY.p + static_cast<int64_t>(i) * vs, 1};
+                BH_TRY(bh_apply(ctx, h, &sa, nullptr, BH_FMT_PACKED, B, s));
             }
-        DevArray<double> dth;
-        BH_TRY(dth.alloc(ctx, theta_f.size()));
-        BH_HIP(ctx, hipMemcpyAsync(dth.p, theta_f.data(), theta_f.size() * 8, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL((k_lz_dots<double, B>), dim3(grid), dim3(LZ_WG), 0, s, Yf.p, HY.p, D, nvf, 1, dth.p, part.p);
-        reduce_plain(grid, nvf * B, dnorm.p);
-        std::vector<double> rr(static_cast<size_t>(nvf) * B);
-        BH_HIP(ctx, hipMemcpyAsync(rr.data(), dnorm.p, rr.size() * 8, hipMemcpyDeviceToHost, s));
-        BH_HIP(ctx, hipStreamSynchronize(s));
-        for (int i = 0; i < nvf; ++i)
-            for (int b = 0; b < B; ++b) {
-                const double yy = dots[(2 * i) * B + b];
-                resid_f[i * B + b] = yy > 0 ? std::sqrt(rr[i * B + b] / yy) : NAN;
-                if (lz_debug() && b < nreal)
-                    std::fprintf(stderr, "[lz] finish run=%d i=%d yy=%.6e yHy=%.10e theta=%.12f resid=%.3e nj=%d\n", b, i, yy,
-                                 dots[(2 * i + 1) * B + b], theta_f[i * B + b], resid_f[i * B + b], njb[b]);
-            }
-        if (lz_debug())
-            for (int b = 0; b < nreal; ++b) {
-                std::fprintf(stderr, "[lz] scales run=%d:", b);
-                for (int t = 0; t < std::min(j + 1, 6); ++t) std::fprintf(stderr, " %.4e", hsc[static_cast<size_t>(t) * B + b]);
-                std::fprintf(stderr, " | C[0..3]:");
-                for (int t = 0; t < std::min(njmax, 4); ++t) std::fprintf(stderr, " %.4e", Cbuf[(static_cast<size_t>(t) * nvf) * B + b]);
-                std::fprintf(stderr, "\n");
-            }
-        if (vecs_out) {
-            // host copy [B][k][D] of the normalised Ritz vectors
-            std::vector<double> hy(static_cast<size_t>(nvf) * vs);
-            BH_HIP(ctx, hipMemcpyAsync(hy.data(), Yf.p, hy.size() * 8, hipMemcpyDeviceToHost, s));
+            BH_HIP(ctx, hipGetLastError());
+            dots.assign(static_cast<size_t>(2 * nvf) * B, 0.0);
+            hipLaunchKernelGGL((k_lz_dots<double, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yf.p, HY.p, D, nvf, 0, nullptr,
+                               part.p);
+            reduce_plain(grid_k2, 2 * nvf * B, dnorm.p);
+            BH_HIP(ctx, hipMemcpyAsync(dots.data(), dnorm.p, dots.size() * 8, hipMemcpyDeviceToHost, s));
             BH_HIP(ctx, hipStreamSynchronize(s));
-            vecs_out->assign(static_cast<size_t>(nreal) * k * D, 0.0);
-            for (int b = 0; b < nreal; ++b)
-                for (int i = 0; i < std::min(k, nvf); ++i) {
+            for (int i = 0; i < nvf; ++i)
+                for (int b = 0; b < B; ++b) {
+                    const double yy = dots[(2 * i) * B + b], yhy = dots[(2 * i + 1) * B + b];
+                    theta_f[i * B + b] = yy > 0 ? yhy / yy : NAN;
+                }
+            DevArray<double> dth;
+            BH_TRY(dth.alloc(ctx, theta_f.size()));
+            std::vector<double> th0(theta_f);
+            for (auto& v : th0)
+                if (!std::isfinite(v)) v = 0.0;
+            BH_HIP(ctx, hipMemcpyAsync(dth.p, th0.data(), th0.size() * 8, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL((k_lz_dots<double, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yf.p, HY.p, D, nvf, 1, dth.p, part.p);
+            reduce_plain(grid_k2, nvf * B, dnorm.p);
+            std::vector<double> rr(static_cast<size_t>(nvf) * B);
+            BH_HIP(ctx, hipMemcpyAsync(rr.data(), dnorm.p, rr.size() * 8, hipMemcpyDeviceToHost, s));
+            BH_HIP(ctx, hipStreamSynchronize(s));
+            for (int i = 0; i < nvf; ++i)
+                for (int b = 0; b < B; ++b) {
                     const double yy = dots[(2 * i) * B + b];
+                    resid_f[i * B + b] = yy > 0 ? std::sqrt(rr[i * B + b] / yy) : NAN;
+                }
+            if (want_vectors) {
+                hy.resize(static_cast<size_t>(nvf) * vs);
+                BH_HIP(ctx, hipMemcpyAsync(hy.data(), Yf.p, hy.size() * 8, hipMemcpyDeviceToHost, s));
+                BH_HIP(ctx, hipStreamSynchronize(s));
+            }
+        }
+        for (size_t q = 0; q < lanes.size(); ++q) {
+            const int b = lanes[q];
+            RunHost& R = rh[b];
+            LaneResult& L = out[q];
+            L.iters = R.iters;
+            L.status = R.status;
+            L.evals.assign(k, NAN);
+            L.resid.assign(k, NAN);
+            L.errest.assign(k, NAN);
+            const int nl = std::min(static_cast<int>(R.theta.size()), nvf);
+            // levels after the FP64 finish, ascending; Rayleigh quotients of one level agree to ~||r||^2/gap
+            std::vector<std::pair<double, int>> lv0, lv;
+            for (int i = 0; i < nl; ++i) lv0.push_back({theta_f[i * B + b], i});
+            std::sort(lv0.begin(), lv0.end());
+            for (const auto& qq : lv0) {
+                if (!std::isfinite(qq.first)) continue;
+                if (!lv.empty() && std::fabs(qq.first - lv.back().first) <= dedup * std::fmax(R.tnorm, 1e-300)) {
+                    if (resid_f[qq.second * B + b] < resid_f[lv.back().second * B + b]) lv.back() = qq;
+                    continue;
+                }
+                lv.push_back(qq);
+            }
+            if (want_vectors) L.vecs.assign(static_cast<size_t>(k) * D, 0.0);
+            for (int i = 0; i < k && i < static_cast<int>(lv.size()); ++i) {
+                const double ev = lv[i].first;
+                const double rs = resid_f[lv[i].second * B + b];
+                double gap = INFINITY;  // to the nearest other Ritz value of T
+                for (double th : R.all_theta)
+                    if (std::fabs(th - ev) > dedup * std::fmax(R.tnorm, 1e-300)) gap = std::fmin(gap, std::fabs(th - ev));
+                L.evals[i] = ev;
+                L.resid[i] = rs;
+                L.errest[i] = std::isfinite(gap) ? rs * rs / gap : rs;
+                if (want_vectors && !hy.empty()) {
+                    const int iv = lv[i].second;
+                    const double yy = dots[(2 * iv) * B + b];
                     const double f = yy > 0 ? 1.0 / std::sqrt(yy) : 0.0;
                     for (int64_t r = 0; r < D; ++r)
-                        (*vecs_out)[(static_cast<size_t>(b) * k + i) * D + r] = hy[static_cast<size_t>(i) * vs + r * B + b] * f;
+                        L.vecs[static_cast<size_t>(i) * D + r] = hy[static_cast<size_t>(iv) * vs + r * B + b] * f;
                 }
+            }
         }
+        return BH_OK;
     }
 
-    // ---- results ----
-    for (int b = 0; b < nreal; ++b) {
-        RunHost& R = rh[b];
-        const int nl = std::min(static_cast<int>(R.theta.size()), nvf);
-        // levels after the FP64 finish, ascending (Ritz vectors of one degenerate level collapse)
-        std::vector<std::pair<double, double>> lv0, lv;
-        for (int i = 0; i < nl; ++i) lv0.push_back({theta_f[i * B + b], resid_f[i * B + b]});
-        std::sort(lv0.begin(), lv0.end());
-        for (const auto& q : lv0) {  // FP64 Rayleigh quotients of one level agree to ~||r||^2 / gap
-            if (!lv.empty() && std::fabs(q.first - lv.back().first) <= dedup * std::fmax(R.tnorm, 1e-300)) {
-                if (q.second < lv.back().second) lv.back() = q;
-                continue;
-            }
-            lv.push_back(q);
-        }
-        for (int i = 0; i < k; ++i) {
-            const double ev = i < static_cast<int>(lv.size()) ? lv[i].first : NAN;
-            const double rs = i < static_cast<int>(lv.size()) ? lv[i].second : NAN;
-            // gap to the nearest other Ritz value of T (all computed low Ritz values)
-            double gap = INFINITY;
-            for (double th : R.all_theta)
-                if (std::fabs(th - ev) > dedup * std::fmax(R.tnorm, 1e-300)) gap = std::fmin(gap, std::fabs(th - ev));
-            const double err = std::isfinite(gap) ? rs * rs / gap : rs;
-            if (res->evals) res->evals[b * k + i] = ev;
-            if (res->resid) res->resid[b * k + i] = rs;
-            if (res->errest) res->errest[b * k + i] = err;
-        }
-        if (res->iters) res->iters[b] = R.iters;
-        if (res->status) res->status[b] = R.status;
-        if (res->used_fp64) res->used_fp64[b] = MIXED ? 0 : 1;
+   private:
+    void launch_k1(int LM) {
+        KTimer kt(ctx, "lanczos_spmv", s);
+#define K1L(LMV)                                                                                                       \
+    if (LM == LMV) {                                                                                                   \
+        if (engine == 1)                                                                                               \
+            hipLaunchKernelGGL((k_lz_a_rb<XT, B, FMT, LMV>), dim3(grid_rb), dim3(LZ_WG), shm_rb, s, a, rblk, nrblk);   \
+        else                                                                                                           \
+            hipLaunchKernelGGL((k_lz_a_rl<XT, B, FMT, LMV>), dim3(grid_rl), dim3(LZ_WG), shm, s, a, rpw);              \
+        return;                                                                                                        \
     }
-    return BH_OK;
+        K1L(0) K1L(4) K1L(16)
+#undef K1L
+    }
+    void reduce_plain(int nwg, int nv, double* out) {
+        KTimer kt(ctx, "lanczos_reduce", s);
+        const int waves = LZ_RED_WG / 64;
+        const int g = std::max(1, std::min((nv + waves - 1) / waves, 64));
+        hipLaunchKernelGGL((k_lz_reduce<B>), dim3(g), dim3(LZ_RED_WG), 0, s, st, part.p, nwg, nv, LZ_RED_PLAIN, maxit, out);
+    }
+    void step(int LM) {
+        launch_k1(LM);
+        reduce_plain(grid_k1, (1 + 2 * LM) * B, st.sums);
+        {
+            KTimer kt(ctx, "lanczos_update", s);
+            hipLaunchKernelGGL((k_lz_b<XT, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, a, LM);
+        }
+        {
+            KTimer kt(ctx, "lanczos_reduce", s);
+            hipLaunchKernelGGL((k_lz_reduce<B>), dim3(1), dim3(LZ_RED_WG), 0, s, st, part.p, grid_k2, B, LZ_RED_BETA, maxit,
+                               nullptr);
+        }
+    }
+    template <typename OT>
+    int form_vectors(const std::vector<int>& njb, int nv, const std::vector<double>& C, OT* out, int64_t os) {
+        BH_TRY(dC.alloc(ctx, C.size()));
+        BH_HIP(ctx, hipMemcpyAsync(dC.p, C.data(), C.size() * 8, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(dnj.p, njb.data(), B * sizeof(int), hipMemcpyHostToDevice, s));
+        {
+            KTimer kt(ctx, "lanczos_ritz", s);
+            hipLaunchKernelGGL((k_lz_ritz<XT, OT, B>), dim3(static_cast<unsigned>((vs + 255) / 256)), dim3(256), 0, s, V, vs,
+                               dnj.p, dC.p, nv, out, os);
+        }
+        BH_HIP(ctx, hipGetLastError());
+        BH_HIP(ctx, hipStreamSynchronize(s));
+        return BH_OK;
+    }
+    int lock(const std::vector<std::vector<int>>& new_locks, const std::vector<Analysis>& ans) {
+        int nv = 0, jmax = 0;
+        for (int b = 0; b < B; ++b)
+            if (!new_locks[b].empty()) {
+                nv = std::max(nv, static_cast<int>(new_locks[b].size()));
+                jmax = std::max(jmax, jl[b]);
+            }
+        nv = std::min(nv, LZ_NVMAX);
+        std::vector<double> hsc(static_cast<size_t>(jmax) * B);
+        BH_HIP(ctx, hipMemcpyAsync(hsc.data(), st.scale, hsc.size() * 8, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipStreamSynchronize(s));
+        std::vector<double> C(static_cast<size_t>(jmax) * nv * B, 0.0);
+        std::vector<int> njb(B, 0);
+        for (int b = 0; b < B; ++b) {
+            if (new_locks[b].empty()) continue;
+            njb[b] = jl[b];
+            for (int i = 0; i < nv && i < static_cast<int>(new_locks[b].size()); ++i) {
+                const std::vector<double>& sv = ans[b].s[new_locks[b][i]];
+                for (int t = 0; t < static_cast<int>(sv.size()) && t < jl[b]; ++t)
+                    C[(static_cast<size_t>(t) * nv + i) * B + b] = sv[t] * hsc[static_cast<size_t>(t) * B + b];
+            }
+        }
+        DevArray<XT> tmp;
+        BH_TRY(tmp.alloc(ctx, static_cast<size_t>(nv) * vs));
+        BH_TRY(form_vectors(njb, nv, C, tmp.p, vs));
+        // normalise: ||y||^2 per (i, b)
+        hipLaunchKernelGGL((k_lz_dots<XT, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, tmp.p, nullptr, D, nv, 2, nullptr, part.p);
+        reduce_plain(grid_k2, nv * B, dnorm.p);
+        std::vector<double> nrm(static_cast<size_t>(nv) * B);
+        BH_HIP(ctx, hipMemcpyAsync(nrm.data(), dnorm.p, nrm.size() * 8, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipStreamSynchronize(s));
+        for (auto& v : nrm) v = v > 0 ? 1.0 / std::sqrt(v) : 0.0;
+        BH_HIP(ctx, hipMemcpyAsync(dnorm.p, nrm.data(), nrm.size() * 8, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL((k_lz_scale<XT, B>), dim3(static_cast<unsigned>((vs * nv + 255) / 256)), dim3(256), 0, s, tmp.p, vs,
+                           nv, dnorm.p);
+        // vector i of lane b goes to Y slot nlock[b] + i (strided copy of one interleaved column)
+        for (int b = 0; b < B; ++b) {
+            for (int i = 0; i < static_cast<int>(new_locks[b].size()) && i < nv; ++i) {
+                const int slot = nlock[b] + i;
+                BH_HIP(ctx, hipMemcpy2DAsync(Y + static_cast<int64_t>(slot) * vs + b, B * sizeof(XT),
+                                             tmp.p + static_cast<int64_t>(i) * vs + b, B * sizeof(XT), sizeof(XT),
+                                             static_cast<size_t>(D), hipMemcpyDeviceToDevice, s));
+                rh[b].lock_theta.push_back(ans[b].theta[new_locks[b][i]]);
+            }
+            nlock[b] += std::min(static_cast<int>(new_locks[b].size()), nv);
+            lmax_used = std::max(lmax_used, nlock[b]);
+        }
+        BH_HIP(ctx, hipMemcpyAsync(st.nlock, nlock.data(), B * 4, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipStreamSynchronize(s));
+        return BH_OK;
+    }
+
+    bh_ctx* ctx;
+    bh_ham* h;
+    hipStream_t s;
+    bh_lanczos_opts o;
+    bool ws;
+    int64_t D = 0, vs = 0;
+    int k = 4, maxit = 1000, chunk = 16;
+    double tol = 1e-10, dedup = 1e-9, eps_store = 0, eps23 = 0, scale0 = 0;
+    XT* V = nullptr;
+    XT* Y = nullptr;
+    double* W = nullptr;
+    double* r0 = nullptr;
+    DevArray<XT> V_own, Y_own;
+    DevArray<double> W_own, r0_own, st_d, part, dC, dnorm;
+    DevArray<int> st_i, dnj;
+    size_t n_alpha = 0, n_beta = 0, bytes = 0;
+    LzState st{};
+    LzArgs<XT> a{};
+    int engine = 2, grid_rb = 1, grid_rl = 1, grid_k1 = 1, grid_k2 = 1;
+    int64_t rpw = 1, nrblk = 0;
+    const int32_t* rblk = nullptr;
+    size_t shm = 0, shm_rb = 0;
+    std::vector<RunHost> rh;
+    std::vector<int> act, nlock, jl, nextc;
+    std::vector<double> Jb, Ub, hlo, hhi, ha, hb;
+    int lmax_used = 0;
+};
+
+// an engine of B lanes (B rounded up to a power of two) in the requested precision
+std::unique_ptr<LzEngineBase> bh_make_engine(bh_ctx* ctx, bh_ham* h, int prec, int B, const bh_lanczos_opts& o,
+                                             hipStream_t s, bool use_ctx_ws, int* rc) {
+    std::unique_ptr<LzEngineBase> e;
+    *rc = BH_OK;
+    if (prec == BH_PREC_FP64) {
+        *rc = bh_ham_prepare_locked(h, BH_PREC_FP64);
+        if (*rc != BH_OK) return nullptr;
+    }
+#define MK(XT, BB)                                                                  \
+    {                                                                               \
+        auto* p = new LzEngine<XT, BB>(ctx, h, o, s, use_ctx_ws);                   \
+        e.reset(p);                                                                 \
+        *rc = p->setup();                                                           \
+    }
+#define MKB(XT)                   \
+    if (B <= 1) MK(XT, 1)         \
+    else if (B <= 2) MK(XT, 2)    \
+    else if (B <= 4) MK(XT, 4)    \
+    else if (B <= 8) MK(XT, 8)    \
+    else if (B <= 16) MK(XT, 16)  \
+    else if (B <= 32) MK(XT, 32)  \
+    else MK(XT, 64)
+    if (prec == BH_PREC_FP64) MKB(double) else MKB(float)
+#undef MKB
+#undef MK
+    if (*rc != BH_OK) return nullptr;
+    return e;
 }
 
-template <typename XT>
-static int lanczos_dispatch(bh_ctx* ctx, bh_ham* h, int Bp, const double* J, const double* U, int nreal,
-                            const bh_lanczos_opts& o, bh_lanczos_result* r, std::vector<double>* vecs) {
-    switch (Bp) {
-        case 1: return lanczos_impl<XT, 1>(ctx, h, J, U, nreal, o, r, vecs);
-        case 2: return lanczos_impl<XT, 2>(ctx, h, J, U, nreal, o, r, vecs);
-        case 4: return lanczos_impl<XT, 4>(ctx, h, J, U, nreal, o, r, vecs);
-        case 8: return lanczos_impl<XT, 8>(ctx, h, J, U, nreal, o, r, vecs);
-        case 16: return lanczos_impl<XT, 16>(ctx, h, J, U, nreal, o, r, vecs);
-        case 32: return lanczos_impl<XT, 32>(ctx, h, J, U, nreal, o, r, vecs);
-        default: return lanczos_impl<XT, 64>(ctx, h, J, U, nreal, o, r, vecs);
+size_t bh_engine_bytes(int64_t D, int prec, int B, const bh_lanczos_opts& o) {
+    int Bp = 1;
+    while (Bp < B) Bp <<= 1;
+    int maxit = o.max_iter > 0 ? o.max_iter : 1000;
+    if (maxit > D) maxit = static_cast<int>(D);
+    const size_t es = prec == BH_PREC_FP64 ? 8 : 4;
+    const size_t vs = static_cast<size_t>(D) * Bp;
+    return (static_cast<size_t>(maxit + 1) + LZ_LMAX) * vs * es + vs * 8 + static_cast<size_t>(D) * 8 +
+           static_cast<size_t>(LZ_GRID_MAX) * (2 * LZ_NVMAX + 1 + 2 * LZ_LMAX) * Bp * 8 +
+           (static_cast<size_t>(maxit) + 2) * Bp * 24;
+}
+
+// batch solve of nreal lanes to completion on one engine (the body of bh_lanczos)
+static int lanczos_batch(bh_ctx* ctx, bh_ham* h, int prec, const double* J, const double* U, int nreal,
+                         const bh_lanczos_opts& o, bh_lanczos_result* res, std::vector<double>* vecs) {
+    int rc = BH_OK;
+    auto e = bh_make_engine(ctx, h, prec, nreal, o, ctx->stream, true, &rc);
+    if (!e) return rc;
+    for (int b = 0; b < nreal; ++b) BH_TRY(e->admit(b, J[b], U[b]));
+    std::vector<int> fin, all;
+    while (e->active() > 0) {
+        BH_TRY(e->run(e->suggest_chunk()));
+        BH_TRY(e->check(fin));
     }
+    for (int b = 0; b < nreal; ++b) all.push_back(b);
+    std::vector<LaneResult> out;
+    BH_TRY(e->finish(all, out, vecs != nullptr));
+    const int k = o.k;
+    if (vecs) vecs->assign(static_cast<size_t>(nreal) * k * h->D, 0.0);
+    for (int b = 0; b < nreal; ++b) {
+        for (int i = 0; i < k; ++i) {
+            if (res->evals) res->evals[b * k + i] = out[b].evals[i];
+            if (res->resid) res->resid[b * k + i] = out[b].resid[i];
+            if (res->errest) res->errest[b * k + i] = out[b].errest[i];
+        }
+        if (res->iters) res->iters[b] = out[b].iters;
+        if (res->status) res->status[b] = out[b].status;
+        if (res->used_fp64) res->used_fp64[b] = prec == BH_PREC_FP64 ? 1 : 0;
+        if (vecs && !out[b].vecs.empty())
+            std::copy(out[b].vecs.begin(), out[b].vecs.end(), vecs->begin() + static_cast<int64_t>(b) * k * h->D);
+    }
+    return BH_OK;
 }
 
 extern "C" {
@@ -1055,15 +1184,8 @@ int solve_subset(bh_ctx* ctx, bh_ham* h, const std::vector<int>& runs, const dou
     bh_lanczos_opts o2 = o;
     o2.prec = prec;
     bh_lanczos_result r2{ev.data(), rs.data(), ee.data(), it.data(), uf.data(), st.data()};
-    int Bp = 1;
-    while (Bp < nb) Bp <<= 1;
     std::vector<double> v2;
-    if (prec == BH_PREC_FP64) {
-        BH_TRY(bh_ham_prepare_locked(h, BH_PREC_FP64));
-        BH_TRY(lanczos_dispatch<double>(ctx, h, Bp, Jb.data(), Ub.data(), nb, o2, &r2, vecs ? &v2 : nullptr));
-    } else {
-        BH_TRY(lanczos_dispatch<float>(ctx, h, Bp, Jb.data(), Ub.data(), nb, o2, &r2, vecs ? &v2 : nullptr));
-    }
+    BH_TRY(lanczos_batch(ctx, h, prec, Jb.data(), Ub.data(), nb, o2, &r2, vecs ? &v2 : nullptr));
     for (int i = 0; i < nb; ++i) {
         const int b = runs[i];
         for (int q = 0; q < k; ++q) {

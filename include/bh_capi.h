@@ -170,6 +170,102 @@ int bh_lanczos(bh_ctx* ctx, bh_ham* h, int B, const double* J, const double* U, 
 /* Ritz vectors of the last bh_lanczos call with want_vectors (host [B][k][D] doubles). */
 int bh_lanczos_copy_vectors(bh_ctx* ctx, int B, int k, double* vecs);
 
+/* ---- batched phase-diagram sweep ----------------------------------------------------------------- */
+/* Replaces the grid loop of Analysis::exact_parameters / calculate_and_save (src/analysis.cpp:207-387),
+ * which assembles H = H_fixed + H1 p1 + H2 p2 per grid point and solves each point on its own.
+ * Here the sectors n-1, n, n+1 of one lattice are built once each (one CSR per sector) and all J
+ * values of a sector are solved together, one CSR pass per Lanczos step for the whole batch.  The
+ * chemical potential enters afterwards as the N-dependent shift -mu N.  Outputs per sector q
+ * (0: n-1, 1: n, 2: n+1) and J point i:
+ *   E[(q*nJ + i)*k + l]   lowest k distinct levels of J T + U V (no mu term), ascending
+ *   gap_ratio[q*nJ + i]   mean of min/max adjacent gaps over those levels (src/analysis.cpp:433-454)
+ * and per J point i and mu point c:
+ *   density[i*nmu + c]          N* / m with N* = argmin_{N in n-1..n+1} E0(N) - mu N (ties: smaller N)
+ *   compressibility[i*nmu + c]  1 / (m (mu_plus - mu_minus)) when N* = n, NaN otherwise
+ *   mu_plus[i] = E0(n+1) - E0(n), mu_minus[i] = E0(n) - E0(n-1)  (charge-gap edges of sector n)
+ * Sector n-1 is skipped (NaN) when n = 0. */
+typedef struct bh_sweep_spec {
+    int kind;            /* lattice kind of bh_lattice_neighbours: 0 chain, 1 square */
+    int Lx, Ly;          /* chain: Lx = m */
+    int periodic;
+    int n;               /* central boson number */
+    int nJ;              /* J points (<= 64), solved as one batch per sector */
+    const double* J;
+    double U;
+    int nmu;
+    const double* mu;
+    unsigned flags;      /* BH_CONV_* */
+    int order;           /* basis order used for the solves (BH_ORDER_LEX is the fastest) */
+} bh_sweep_spec;
+
+typedef struct bh_sweep_out {
+    double* E;               /* [3][nJ][k] */
+    double* gap_ratio;       /* [3][nJ] */
+    double* density;         /* [nJ][nmu] */
+    double* compressibility; /* [nJ][nmu] */
+    double* mu_plus;         /* [nJ] */
+    double* mu_minus;        /* [nJ] */
+    int* status;             /* [3][nJ] */
+    int* used_fp64;          /* [3][nJ] (may be NULL) */
+} bh_sweep_out;
+
+int bh_sweep(bh_ctx* ctx, const bh_sweep_spec* spec, const bh_lanczos_opts* opts, bh_sweep_out* out);
+
+/* ---- serving --------------------------------------------------------------------------------------- */
+/* No counterpart in the reference (its nearest analogues are the dynamic OpenMP schedule of independent
+ * solves, src/analysis.cpp:302, and the memory-budgeted thread cap, src/resource.cpp:77-82).
+ * A server owns a worker thread and its own device context.  Requests of uneven (lattice, m, n) are
+ * grouped per sector; each sector in flight has a Lanczos engine of L lanes on its own HIP stream and a
+ * cached CSR.  Lanes are refilled as soon as their solve finishes (continuous batching), sector
+ * engines run concurrently on the device, and new sectors are admitted while the HBM budget (CSR plus
+ * Krylov bytes) allows; idle sectors are evicted least-recently-used first.  A lane's arithmetic and
+ * its analysis schedule do not depend on the other lanes, so a request returns the same bits alone or
+ * in a mix served with the same lane count.  Mixed-precision results that fail the FP64 acceptance
+ * test of E0 are rerun in FP64. */
+typedef struct bh_request {
+    int kind, Lx, Ly, periodic; /* lattice as in bh_lattice_neighbours (chain: Lx = m) */
+    int n;                      /* bosons */
+    double J, U, mu;            /* the levels returned include the shift -mu n */
+} bh_request;
+
+typedef struct bh_server_opts {
+    int lanes;              /* lanes per sector engine, upper bound (power of two <= 64; 0: 16) */
+    int prec;               /* BH_PREC_MIXED or BH_PREC_FP64 */
+    size_t hbm_budget;      /* device bytes for CSR cache + engines (0: 80% of free memory) */
+    size_t engine_bytes;    /* cap per engine; lanes shrink for large sectors (0: 16 GiB) */
+    unsigned flags;         /* BH_CONV_* */
+    bh_lanczos_opts lanczos;
+} bh_server_opts;
+
+typedef struct bh_result {
+    uint64_t id;
+    int status;             /* BH_OK, BH_ENOCONV, BH_EBUDGET (sector can never fit), BH_EINVAL */
+    int k;
+    double evals[16];       /* lowest k levels of J T + U V - mu n */
+    double resid[16];
+    int iters;
+    int used_fp64;
+    double t_submit, t_start, t_done; /* seconds on the server's clock */
+} bh_result;
+
+typedef struct bh_server_stats {
+    int64_t submitted, completed, queued, in_flight;
+    int engines;            /* sector engines alive */
+    size_t hbm_used;        /* bytes held by the server's context */
+    double p50_latency, p99_latency, mean_latency; /* seconds, completed requests */
+    double busy_seconds;    /* first submit to last completion */
+} bh_server_stats;
+
+int bh_server_create(bh_ctx* ctx, const bh_server_opts* opts, bh_server** out);
+int bh_server_submit(bh_server* srv, const bh_request* req, uint64_t* id);
+/* BH_EAGAIN while the request is pending; the result is kept until taken once */
+int bh_server_poll(bh_server* srv, uint64_t id, bh_result* out);
+int bh_server_wait(bh_server* srv, uint64_t id, double timeout_s, bh_result* out);
+/* waits until every submitted request has completed (BH_EAGAIN on timeout) */
+int bh_server_drain(bh_server* srv, double timeout_s);
+int bh_server_stats_get(bh_server* srv, bh_server_stats* st);
+int bh_server_destroy(bh_server* srv);
+
 /* Dense FP64 eigenvalues of a symmetric tridiagonal matrix (QL), exposed for tests of the host
  * analysis: d[n] diagonal, e[n-1] off-diagonal; evals ascending. */
 int bh_tridiag_eigvals(int n, const double* d, const double* e, double* evals);
