@@ -42,17 +42,22 @@ constexpr int LZ_GRID_MAX = 1024;  // workgroups that write partial sums
 constexpr int LZ_LMAX = 16;        // locked Ritz vectors per solve
 constexpr int LZ_NVMAX = 16;       // Ritz vectors formed per pass
 constexpr int LZ_RED_WG = 1024;    // reduction workgroup: 16 waves, one value per wave
+constexpr double LZ_ACCEPT_LEVEL = 1e-6;  // mixed: bound on ||r||^2/gap of the levels above E0, relative
 
 struct LzState {
-    double* alpha;   // [maxit+1][B]
-    double* beta;    // [maxit+2][B]
-    double* scale;   // [maxit+2][B]  1 / ||stored vector||
+    double* alpha;   // [slots][B]
+    double* beta;    // [slots+1][B]
+    double* scale;   // [slots+1][B]  1 / ||stored vector||
     double* J;       // [B]
     double* U;       // [B]
     double* sums;    // [(1 + 2 LZ_LMAX) B] R1 results: alpha, <y_l, w>, <y_l, v_j>
+    double* tau;     // [2][LZ_LMAX][B] estimates of |<y_l, v_j>| and |<y_l, v_{j-1}>| (locked y_l)
+    double* ltheta;  // [LZ_LMAX][B] Ritz values of the locked vectors
+    double* anorm;   // [B] estimate of ||H_b||
     int* active;     // [B]
     int* nlock;      // [B]
     int* step;       // [B] Lanczos step of each lane (lanes are admitted and retired independently)
+    int* so;         // [2] orthogonalise against the locked vectors in this step / did in the previous one
     double* part;    // [LZ_GRID_MAX][nv] per-workgroup partials
 };
 
@@ -67,11 +72,12 @@ struct LzArgs {
     const double* diag64;
     const double* table;
     int cb;
-    XT* V;         // [(maxit+1)][D][B]
+    XT* V;         // [slots][D][B] Krylov basis; lane b's vector t at V[t][r][b]
+    XT* Xc;        // [D][B] current Lanczos vector of every lane (the operand of the gathers)
     int64_t vs;    // D * B
     double* W;     // [D][B]
     XT* Y;         // [LZ_LMAX][D][B] locked Ritz vectors (unit norm)
-    int maxit;     // Krylov slots per lane (steps stop there)
+    int maxit;     // slots - 1: a lane stops at this step until it is restarted or retired
     LzState st;
 };
 
@@ -100,6 +106,7 @@ struct K1Epi {
     double sj, bj, Jb, Ub;
     int j, nl;
     bool act;
+    bool so;  // uniform: this step orthogonalises against the locked vectors
     double pa = 0.0;
     double pyw[LM > 0 ? LM : 1] = {};
     double pyv[LM > 0 ? LM : 1] = {};
@@ -130,16 +137,18 @@ template <typename XT, int B, int FMT, int LM>
 __device__ __forceinline__ K1Epi<XT, B, FMT, LM> k1_setup(const LzArgs<XT>& a) {
     const int b = threadIdx.x % B;  // fixed per thread: LZ_WG is a multiple of B
     const int j = a.st.step[b];
+    const bool so = LM > 0 && a.st.so[0] != 0;
     K1Epi<XT, B, FMT, LM> e{a,
-                            a.V + static_cast<int64_t>(j) * a.vs,
+                            a.Xc,
                             a.V + static_cast<int64_t>(j > 0 ? j - 1 : 0) * a.vs,
                             a.st.scale[j * B + b],
                             j > 0 ? a.st.beta[j * B + b] * a.st.scale[(j - 1) * B + b] : 0.0,
                             a.st.J[b],
                             a.st.U[b],
                             j,
-                            a.st.active[b] ? a.st.nlock[b] : 0,
-                            a.st.active[b] != 0 && j < a.maxit};
+                            (a.st.active[b] && so) ? a.st.nlock[b] : 0,
+                            a.st.active[b] != 0 && j < a.maxit,
+                            so};
     return e;
 }
 
@@ -148,6 +157,7 @@ __device__ __forceinline__ void k1_finish(K1Epi<XT, B, FMT, LM>& e, double* lds,
     constexpr int NV = (1 + 2 * LM) * B;
     lz_block_put<B>(e.pa, 0, lds, part, NV);
     if constexpr (LM > 0) {
+        if (!e.so) return;  // uniform
 #pragma unroll
         for (int l = 0; l < LM; ++l) {
             lz_block_put<B>(e.pyw[l], 1 + l, lds, part, NV);
@@ -191,18 +201,20 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_a_rl(LzArgs<XT> a, int64_t rows_pe
     k1_finish<XT, B, FMT, LM>(e, lds, a.st.part);
 }
 
-// K2: v_{j+1} = w - alpha v_j - sum_l c_l y_l (stored), partials of ||v_{j+1}||^2.  Block 0 records alpha_j.
+// K2: v_{j+1} = w - alpha v_j - sum_l c_l y_l (stored in slot j+1 and, replacing v_j, in the current-vector
+// buffer; each thread reads its own elements before writing them), partials of ||v_{j+1}||^2.  Block 0
+// records alpha_j.  The locked vectors enter only in the steps flagged for orthogonalisation.
 template <typename XT, int B>
 __global__ void __launch_bounds__(LZ_WG) k_lz_b(LzArgs<XT> a, int lm) {
     __shared__ double lds[LZ_WG / 64 * B];
     const int b = threadIdx.x % B;
     const int j = a.st.step[b];
-    const XT* vj = a.V + static_cast<int64_t>(j) * a.vs;
+    XT* vj = a.Xc;
     XT* vn = a.V + static_cast<int64_t>(j + 1) * a.vs;
     const bool act = a.st.active[b] != 0 && j < a.maxit;
     const double sj = a.st.scale[j * B + b];
     const double al = a.st.sums[b];
-    const int nl = act ? min(a.st.nlock[b], lm) : 0;
+    const int nl = (act && lm > 0 && a.st.so[0] != 0) ? min(a.st.nlock[b], lm) : 0;
     if (blockIdx.x == 0 && threadIdx.x < B && act) a.st.alpha[j * B + b] = al;
     constexpr int ROWS = LZ_WG / B;
     constexpr int UN = 4;  // rows per thread in flight
@@ -238,6 +250,7 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_b(LzArgs<XT> a, int lm) {
                 if (!ok[u]) continue;
                 const XT wr = static_cast<XT>(w[u]);
                 vn[idx[u]] = wr;
+                vj[idx[u]] = wr;
                 const double wd = to_d(wr);
                 pn += wd * wd;
             }
@@ -254,44 +267,121 @@ __global__ void k_lz_init(XT* __restrict__ V0, const double* __restrict__ r0, in
 }
 
 // Reductions of per-workgroup partials, one wave per value in a fixed order (deterministic):
-//   mode 0: out[v] = sum_wg part[wg * nv + v]; values spread over the launch's workgroups
-//   mode 1: v = b: beta_{j+1} = sqrt(sum), scale_{j+1} = 1/beta (active runs), then step = j + 1;
-//           launched as a single workgroup so that the step write follows every read of it
+//   LZ_RED_PLAIN: out[v] = sum_wg part[wg * nv + v]; values spread over the launch's workgroups
+//   LZ_RED_K1:    as PLAIN for the K1 sums; only the nmin alpha sums when the step does not orthogonalise
+//   LZ_RED_BETA:  v = b: beta_{j+1} = sqrt(sum), scale_{j+1} = 1/beta (active lanes), then step = j + 1.
+//                 The same pass advances, for every locked y_l of the lane, the estimate tau of
+//                 |<y_l, v_{j+1}>| (Parlett & Scott):
+//                   tau_{j+1} = (|theta_l - alpha_j| tau_j + beta_j tau_{j-1} + eps ||H||) / beta_{j+1},
+//                 reset to eps after a step that orthogonalised against y_l.  The next step orthogonalises
+//                 (all lanes, all locked vectors) when some tau exceeds sqrt(eps), and always the step after
+//                 one that started a pair, since v_{j+1} and v_{j+2} both have to be cleaned.  Launched as a
+//                 single workgroup, so that the step and flag writes follow every read of them.
 constexpr int LZ_RED_PLAIN = 0;
 constexpr int LZ_RED_BETA = 1;
+constexpr int LZ_RED_K1 = 2;
 
 template <int B>
 __global__ void __launch_bounds__(LZ_RED_WG) k_lz_reduce(LzState st, const double* __restrict__ part, int nwg, int nv,
-                                                         int mode, int maxit, double* __restrict__ out) {
+                                                         int mode, int maxit, double* __restrict__ out, int nmin,
+                                                         double eps, int so_always) {
     const int lane = threadIdx.x & 63;
     const int waves = LZ_RED_WG / 64;
-    for (int v = blockIdx.x * waves + (threadIdx.x >> 6); v < nv; v += gridDim.x * waves) {
+    __shared__ int want[LZ_RED_WG / 64];
+    const int so_cur = (mode == LZ_RED_PLAIN) ? 1 : st.so[0];
+    const int so_prev = (mode == LZ_RED_BETA) ? st.so[1] : 0;
+    const int nvv = (mode == LZ_RED_K1 && !so_cur) ? nmin : nv;
+    int any = 0;
+    for (int v = blockIdx.x * waves + (threadIdx.x >> 6); v < nvv; v += gridDim.x * waves) {
         double s = 0.0;
         for (int w = lane; w < nwg; w += 64) s += part[static_cast<size_t>(w) * nv + v];
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
-        if (lane == 0) {
-            if (mode == LZ_RED_PLAIN) {
-                out[v] = s;
-            } else {
-                // v = b; only this lane of this wave reads or writes step[b] in the launch
-                const int j = st.step[v];
-                if (st.active[v] && j < maxit) {
-                    const double nrm = std::sqrt(s);
-                    st.beta[(j + 1) * B + v] = nrm;
-                    st.scale[(j + 1) * B + v] = nrm > 0.0 ? 1.0 / nrm : 0.0;
-                    st.step[v] = j + 1;
+        if (mode != LZ_RED_BETA) {
+            if (lane == 0) out[v] = s;
+            continue;
+        }
+        // v = b; only lane 0 of this wave writes step[b] in the launch
+        const int j = st.step[v];
+        if (st.active[v] && j < maxit) {
+            const double nrm = std::sqrt(s);
+            const int nl = st.nlock[v];
+            if (lane < nl && lane < LZ_LMAX) {
+                const size_t i0 = static_cast<size_t>(lane) * B + v, i1 = static_cast<size_t>(LZ_LMAX + lane) * B + v;
+                const double tj = st.tau[i0], tjm = st.tau[i1];
+                double tn = eps;
+                if (!so_cur) {
+                    const double al = st.sums[v], bj = st.beta[j * B + v];
+                    const double inj = eps * st.anorm[v];
+                    tn = nrm > 0.0 ? (std::fabs(st.ltheta[i0] - al) * tj + bj * tjm + inj) / nrm : 1.0;
                 }
+                st.tau[i1] = tj;
+                st.tau[i0] = tn;
+                if (tn > std::sqrt(eps)) any = 1;
+            }
+            if (lane == 0) {
+                st.beta[(j + 1) * B + v] = nrm;
+                st.scale[(j + 1) * B + v] = nrm > 0.0 ? 1.0 / nrm : 0.0;
+                st.step[v] = j + 1;
             }
         }
     }
+    if (mode != LZ_RED_BETA) return;
+    // wave-level OR, then across the workgroup
+    any = __any(any) ? 1 : 0;
+    if (lane == 0) want[threadIdx.x >> 6] = any;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int f = 0;
+        for (int w = 0; w < waves; ++w) f |= want[w];
+        st.so[0] = (so_always || (so_cur && !so_prev)) ? 1 : f;
+        st.so[1] = so_cur;
+    }
 }
 
-// admission of lane `slot`: V[0][r][slot] = r0[r] in storage precision
+// admission of lane `slot`: V[0][r][slot] = Xc[r][slot] = r0[r] in storage precision
 template <typename XT, int B>
-__global__ void k_lz_admit(XT* __restrict__ V0, const double* __restrict__ r0, int64_t D, int slot) {
+__global__ void k_lz_admit(XT* __restrict__ V0, XT* __restrict__ Xc, const double* __restrict__ r0, int64_t D, int slot) {
     const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r < D) V0[r * B + slot] = static_cast<XT>(r0[r]);
+    if (r < D) {
+        const XT x = static_cast<XT>(r0[r]);
+        V0[r * B + slot] = x;
+        Xc[r * B + slot] = x;
+    }
+}
+
+// Thick restart of the lanes with nin[b] > 0, in place: V[c][r][b] = sum_{t < nin[b]} V[t][r][b] C[(t nv + c) B + b]
+// for c < nout[b], and the last of them also becomes the lane's current vector.  Every thread reads all of
+// its (r, b) column before writing any of it, and no two threads share a column, so the in-place update
+// is safe.
+constexpr int LZ_KEEP_MAX = 32;  // restart outputs per lane (kept Ritz vectors + the residual vector)
+
+template <typename XT, int B>
+__global__ void __launch_bounds__(256) k_lz_restart(XT* V, XT* __restrict__ Xc, int64_t vs, const int* __restrict__ nin,
+                                                    const int* __restrict__ nout, const double* __restrict__ C, int nv) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= vs) return;
+    const int b = static_cast<int>(idx % B);
+    const int nj = nin[b], no = nout[b];
+    if (nj <= 0 || no <= 0) return;
+    double acc[LZ_KEEP_MAX];
+#pragma unroll
+    for (int i = 0; i < LZ_KEEP_MAX; ++i) acc[i] = 0.0;
+    for (int t = 0; t < nj; ++t) {
+        const double v = to_d(V[static_cast<int64_t>(t) * vs + idx]);
+        const double* ct = C + (static_cast<int64_t>(t) * nv) * B + b;
+#pragma unroll
+        for (int i = 0; i < LZ_KEEP_MAX; ++i)
+            if (i < no) acc[i] += v * ct[i * B];
+    }
+#pragma unroll
+    for (int i = 0; i < LZ_KEEP_MAX; ++i)
+        if (i < no) V[static_cast<int64_t>(i) * vs + idx] = static_cast<XT>(acc[i]);
+    double last = 0.0;
+#pragma unroll
+    for (int i = 0; i < LZ_KEEP_MAX; ++i)
+        if (i == no - 1) last = acc[i];
+    Xc[idx] = static_cast<XT>(last);
 }
 
 // Ritz vectors: out[i][r][b] = sum_{t < nj[b]} V[t][r][b] * C[(t * nv + i) * B + b]   (C includes scale_t).
@@ -366,6 +456,60 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_dots(const YT* __restrict__ Y, con
     }
 }
 
+// Rayleigh-Ritz of the finished candidates, row i of the Gram matrices: per lane b, partials of
+// value q < nv: <Y_i, Y_q> and value nv + q: <Y_i, HY_q> (reduced by k_lz_reduce mode 0)
+template <int B>
+__global__ void __launch_bounds__(LZ_WG) k_lz_gram_row(const double* __restrict__ Y, const double* __restrict__ HY, int64_t D,
+                                                       int nv, int i, double* part) {
+    __shared__ double lds[LZ_WG / 64 * B];
+    const int b = threadIdx.x % B;
+    constexpr int ROWS = LZ_WG / B;
+    const int64_t vs = D * B;
+    const int64_t ntile = (D + ROWS - 1) / ROWS;
+    const int64_t per = (ntile + gridDim.x - 1) / gridDim.x;
+    const int64_t t0 = min(static_cast<int64_t>(blockIdx.x) * per, ntile), t1 = min(t0 + per, ntile);
+    double gy[LZ_NVMAX], gh[LZ_NVMAX];
+#pragma unroll
+    for (int q = 0; q < LZ_NVMAX; ++q) gy[q] = gh[q] = 0.0;
+    for (int64_t t = t0; t < t1; ++t) {
+        const int64_t r = t * ROWS + threadIdx.x / B;
+        if (r >= D) continue;
+        const int64_t idx = r * B + b;
+        const double yi = Y[static_cast<int64_t>(i) * vs + idx];
+#pragma unroll
+        for (int q = 0; q < LZ_NVMAX; ++q)
+            if (q < nv) {
+                gy[q] += yi * Y[static_cast<int64_t>(q) * vs + idx];
+                gh[q] += yi * HY[static_cast<int64_t>(q) * vs + idx];
+            }
+    }
+    for (int q = 0; q < nv; ++q) {
+        lz_block_put<B>(gy[q], q, lds, part, 2 * nv * B);
+        lz_block_put<B>(gh[q], nv + q, lds, part, 2 * nv * B);
+    }
+}
+
+// in place, per lane b: X[c][r][b] = sum_{i < nin} X[i][r][b] Z[(i nout + c) B + b] for c < nout (nout <= nin;
+// each thread reads its (r, b) column of X before writing it)
+template <int B>
+__global__ void __launch_bounds__(256) k_lz_combine(double* X, int64_t vs, const double* __restrict__ Z, int nin, int nout) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= vs) return;
+    const int b = static_cast<int>(idx % B);
+    double x[LZ_NVMAX];
+#pragma unroll
+    for (int i = 0; i < LZ_NVMAX; ++i) x[i] = i < nin ? X[static_cast<int64_t>(i) * vs + idx] : 0.0;
+#pragma unroll
+    for (int c = 0; c < LZ_NVMAX; ++c) {
+        if (c >= nout) break;
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < LZ_NVMAX; ++i)
+            if (i < nin) acc += x[i] * Z[(static_cast<int64_t>(i) * nout + c) * B + b];
+        X[static_cast<int64_t>(c) * vs + idx] = acc;
+    }
+}
+
 // y *= f[i*B + b] over [nv][D][B]
 template <typename YT, int B>
 __global__ void k_lz_scale(YT* __restrict__ Y, int64_t vs, int nv, const double* __restrict__ f) {
@@ -388,6 +532,19 @@ int lz_debug() {
     static int d = [] {
         const char* e = std::getenv("BHX_DEBUG");
         return e ? std::atoi(e) : 0;
+    }();
+    return d;
+}
+
+// Orthogonalisation against the locked vectors: in every step (default), or only in the steps the
+// Parlett-Scott estimate asks for (BHX_SO=auto).  With FP32-stored vectors and the flag shared by all
+// lanes of an engine, the estimate-driven schedule fires in most steps of a batch anyway and leaves the
+// basis only semi-orthogonal (sqrt(eps32)) to converged levels, which costs accuracy in the excited
+// levels; every-step orthogonalisation keeps it at eps32.
+int lz_so_always() {
+    static int d = [] {
+        const char* e = std::getenv("BHX_SO");
+        return (e && std::strcmp(e, "auto") == 0) ? 0 : 1;
     }();
     return d;
 }
@@ -415,7 +572,7 @@ struct RunHost {
     std::vector<std::vector<double>> s;  // their eigenvectors of T
     double tnorm = 0.0;
     double tnorm0 = 0.0;                 // ||T|| at the first analysis (divergence guard)
-    std::vector<double> all_theta;       // lowest Ritz values of the last analysis (for gaps)
+    bool exhausted = false;              // the run ended on an invariant Krylov space
 };
 
 struct Analysis {
@@ -449,6 +606,17 @@ int dalloc(bh_ctx* ctx, DevArray<T>& a, size_t n) {
 
 }  // namespace
 
+// Krylov slots of a lane: the largest Lanczos matrix it builds before a thick restart (ncv, at most D
+// and at most max_iter) plus one slot for the next Lanczos vector
+static int lz_slots(int64_t D, const bh_lanczos_opts& o) {
+    const int maxsteps = o.max_iter > 0 ? o.max_iter : 10000;
+    int ncv = o.ncv > 0 ? o.ncv : 512;
+    ncv = std::max(ncv, 2 * o.k + 8);
+    int64_t cap = std::min<int64_t>(ncv, D);
+    cap = std::min<int64_t>(cap, maxsteps);
+    return static_cast<int>(std::max<int64_t>(cap, 1)) + 1;
+}
+
 template <typename XT, int B>
 class LzEngine final : public LzEngineBase {
     static constexpr bool MIXED = std::is_same<XT, float>::value;
@@ -462,14 +630,15 @@ class LzEngine final : public LzEngineBase {
         D = h->D;
         vs = D * B;
         k = o.k;
-        maxit = o.max_iter > 0 ? o.max_iter : 1000;
-        if (maxit > D) maxit = static_cast<int>(D);
+        maxsteps = o.max_iter > 0 ? o.max_iter : 10000;
+        slots = lz_slots(D, o);
+        cap = slots - 1;
         chunk = o.chunk > 0 ? o.chunk : 16;
         tol = o.tol > 0 ? o.tol : 1e-10;
         dedup = o.dedup_rel > 0 ? o.dedup_rel : 1e-9;
         eps_store = MIXED ? 5.96e-8 : 1.11e-16;
         eps23 = std::pow(2.220446049250313e-16, 2.0 / 3.0);
-        const size_t nV = static_cast<size_t>(maxit + 1) * vs, nY = static_cast<size_t>(LZ_LMAX) * vs;
+        const size_t nV = static_cast<size_t>(slots) * vs, nY = static_cast<size_t>(LZ_LMAX) * vs;
         if (ws) {
             void* q = nullptr;
             BH_TRY(bh_ws_get(ctx, 0, nV * sizeof(XT), &q));
@@ -480,35 +649,45 @@ class LzEngine final : public LzEngineBase {
             Y = static_cast<XT*>(q);
             BH_TRY(bh_ws_get(ctx, 3, static_cast<size_t>(D) * sizeof(double), &q));
             r0 = static_cast<double*>(q);
+            BH_TRY(bh_ws_get(ctx, 4, static_cast<size_t>(vs) * sizeof(XT), &q));
+            Xc = static_cast<XT*>(q);
         } else {
             BH_TRY(dalloc(ctx, V_own, nV));
             BH_TRY(dalloc(ctx, W_own, static_cast<size_t>(vs)));
             BH_TRY(dalloc(ctx, Y_own, nY));
             BH_TRY(dalloc(ctx, r0_own, static_cast<size_t>(D)));
+            BH_TRY(dalloc(ctx, Xc_own, static_cast<size_t>(vs)));
             V = V_own.p;
             W = W_own.p;
             Y = Y_own.p;
             r0 = r0_own.p;
+            Xc = Xc_own.p;
         }
-        n_alpha = static_cast<size_t>(maxit + 1) * B;
-        n_beta = static_cast<size_t>(maxit + 2) * B;
+        n_alpha = static_cast<size_t>(slots) * B;
+        n_beta = static_cast<size_t>(slots + 1) * B;
         const size_t n_sums = static_cast<size_t>(1 + 2 * LZ_LMAX) * B;
-        const size_t nd = n_alpha + 2 * n_beta + 2 * B + n_sums;
+        const size_t n_tau = static_cast<size_t>(2 * LZ_LMAX) * B, n_lth = static_cast<size_t>(LZ_LMAX) * B;
+        const size_t nd = n_alpha + 2 * n_beta + 2 * B + n_sums + n_tau + n_lth + B;
         BH_TRY(dalloc(ctx, st_d, nd));
-        BH_TRY(dalloc(ctx, st_i, static_cast<size_t>(3 * B)));
+        BH_TRY(dalloc(ctx, st_i, static_cast<size_t>(3 * B + 2)));
         const int nvmax = std::max(static_cast<int>(n_sums), 2 * LZ_NVMAX * B);
         BH_TRY(dalloc(ctx, part, static_cast<size_t>(LZ_GRID_MAX) * nvmax));
         BH_TRY(dalloc(ctx, dnorm, static_cast<size_t>(2 * LZ_NVMAX * B)));
         BH_TRY(dalloc(ctx, dnj, B));
+        BH_TRY(dalloc(ctx, dno, B));
         st.alpha = st_d.p;
         st.beta = st.alpha + n_alpha;
         st.scale = st.beta + n_beta;
         st.J = st.scale + n_beta;
         st.U = st.J + B;
         st.sums = st.U + B;
+        st.tau = st.sums + n_sums;
+        st.ltheta = st.tau + n_tau;
+        st.anorm = st.ltheta + n_lth;
         st.active = st_i.p;
         st.nlock = st_i.p + B;
         st.step = st_i.p + 2 * B;
+        st.so = st_i.p + 3 * B;
         st.part = part.p;
         BH_HIP(ctx, hipMemsetAsync(st_d.p, 0, nd * sizeof(double), s));
         BH_HIP(ctx, hipMemsetAsync(st_i.p, 0, st_i.bytes(), s));
@@ -538,10 +717,11 @@ class LzEngine final : public LzEngineBase {
                        h->table,
                        h->code_bits,
                        V,
+                       Xc,
                        vs,
                        W,
                        Y,
-                       maxit,
+                       cap,
                        st};
         shm = FMT == BH_FMT_PACKED ? (static_cast<size_t>(1) << h->code_bits) * sizeof(double) : 0;
         engine = bh_engine_choice(B, h) == 1 ? 1 : 2;  // staged row blocks, or one thread per (row, b)
@@ -558,13 +738,15 @@ class LzEngine final : public LzEngineBase {
         act.assign(B, 0);
         nlock.assign(B, 0);
         jl.assign(B, 0);
+        tot.assign(B, 0);
+        kept.assign(B, 0);
         Jb.assign(B, 0.0);
         Ub.assign(B, 0.0);
         hlo.assign(B, 0.0);
         hhi.assign(B, 0.0);
         for (auto& R : rh) R.done = true;
         bytes = st_d.bytes() + st_i.bytes() + part.bytes() + dnorm.bytes() +
-                (ws ? 0 : V_own.bytes() + W_own.bytes() + Y_own.bytes() + r0_own.bytes());
+                (ws ? 0 : V_own.bytes() + W_own.bytes() + Y_own.bytes() + r0_own.bytes() + Xc_own.bytes());
         return BH_OK;
     }
 
@@ -574,16 +756,17 @@ class LzEngine final : public LzEngineBase {
         for (int b = 0; b < B; ++b) c += act[b];
         return c;
     }
-    int steps_of(int slot) const override { return jl[slot]; }
+    int steps_of(int slot) const override { return tot[slot]; }
     size_t device_bytes() const override { return bytes; }
     // Steps until the next lane reaches its own analysis point.  Each lane follows its own schedule
-    // (first at max(2k+8, 20) steps, then every max(chunk, j/4): the analysis cost grows with j), so the
-    // steps at which a lane is analysed, locked and retired do not depend on the other lanes.
+    // (first at max(2k+8, 20) steps, then every max(chunk, j/4); always when its Krylov slots are full
+    // and when its step budget is spent), so the steps at which a lane is analysed, locked, restarted
+    // and retired do not depend on the other lanes.
     int suggest_chunk() const override {
         int st = -1;
         for (int b = 0; b < B; ++b)
             if (act[b]) {
-                const int d = std::max(1, std::min(nextc[b], maxit) - jl[b]);
+                const int d = std::max(1, next_event(b) - jl[b]);
                 st = st < 0 ? d : std::min(st, d);
             }
         return st < 0 ? chunk : st;
@@ -595,22 +778,26 @@ class LzEngine final : public LzEngineBase {
         act[slot] = 1;
         nlock[slot] = 0;
         jl[slot] = 0;
+        tot[slot] = 0;
+        kept[slot] = 0;
         nextc[slot] = std::max(2 * k + 8, 20);
         Jb[slot] = J;
         Ub[slot] = U;
         const double va = U * h->vmin, vb = U * h->vmax;
         hlo[slot] = std::fmin(va, vb) - std::fabs(J) * h->tmax;
         hhi[slot] = std::fmax(va, vb) + std::fabs(J) * h->tmax;
+        an0[slot] = std::fmax(std::fabs(hlo[slot]), std::fabs(hhi[slot]));
         const int one = 1, zero = 0;
         const double dz = 0.0;
         BH_HIP(ctx, hipMemcpyAsync(st.J + slot, &Jb[slot], 8, hipMemcpyHostToDevice, s));
         BH_HIP(ctx, hipMemcpyAsync(st.U + slot, &Ub[slot], 8, hipMemcpyHostToDevice, s));
         BH_HIP(ctx, hipMemcpyAsync(st.scale + slot, &scale0, 8, hipMemcpyHostToDevice, s));
         BH_HIP(ctx, hipMemcpyAsync(st.beta + slot, &dz, 8, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(st.anorm + slot, &an0[slot], 8, hipMemcpyHostToDevice, s));
         BH_HIP(ctx, hipMemcpyAsync(st.nlock + slot, &zero, 4, hipMemcpyHostToDevice, s));
         BH_HIP(ctx, hipMemcpyAsync(st.step + slot, &zero, 4, hipMemcpyHostToDevice, s));
         BH_HIP(ctx, hipMemcpyAsync(st.active + slot, &one, 4, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL((k_lz_admit<XT, B>), dim3(static_cast<unsigned>((D + 255) / 256)), dim3(256), 0, s, V, r0, D,
+        hipLaunchKernelGGL((k_lz_admit<XT, B>), dim3(static_cast<unsigned>((D + 255) / 256)), dim3(256), 0, s, V, Xc, r0, D,
                            slot);
         BH_HIP(ctx, hipGetLastError());
         BH_HIP(ctx, hipStreamSynchronize(s));  // the sources of the copies are host locals/members
@@ -622,7 +809,11 @@ class LzEngine final : public LzEngineBase {
         for (int t = 0; t < steps; ++t) step(LM);
         BH_HIP(ctx, hipGetLastError());
         for (int b = 0; b < B; ++b)
-            if (act[b]) jl[b] = std::min(jl[b] + steps, maxit);
+            if (act[b]) {
+                const int d = std::min(steps, cap - jl[b]);
+                jl[b] += d;
+                tot[b] += d;
+            }
         return BH_OK;
     }
 
@@ -639,11 +830,11 @@ class LzEngine final : public LzEngineBase {
         BH_HIP(ctx, hipStreamSynchronize(s));
         std::vector<std::vector<int>> new_locks(B);
         std::vector<Analysis> ans(B);
-        std::vector<int> done_now(B, 0);
+        std::vector<int> done_now(B, 0), restart_now(B, 0);
         auto analyse_run = [&](int b) {
             RunHost& R = rh[b];
             const int j = jl[b];
-            if (j < nextc[b] && j < maxit) return;
+            if (j < next_event(b)) return;
             nextc[b] = j + std::max(chunk, j / 4);
             R.alpha.assign(j, 0.0);
             R.beta.assign(j + 1, 0.0);
@@ -651,21 +842,23 @@ class LzEngine final : public LzEngineBase {
             for (int t = 0; t <= j; ++t) R.beta[t] = hb[static_cast<size_t>(t) * B + b];
             // An invariant Krylov space shows as a beta at the rounding level of the stored vectors.  It
             // may have happened anywhere since the last check; the steps after it restart from rounding
-            // noise and would duplicate every level, so T is cut at the first such beta.
+            // noise and would duplicate every level, so T is cut at the first such beta.  The couplings
+            // inside the block kept by a thick restart may be exactly zero and are not searched.
             int jr = j;
             {
                 double amax = 0.0;
                 for (int t = 0; t < j; ++t) amax = std::fmax(amax, std::fabs(R.alpha[t]) + R.beta[t]);
                 const double thr = std::fmax(1e-12, 100.0 * eps_store) * std::fmax(amax, 1e-300);
-                for (int t = 1; t <= j; ++t)
+                for (int t = kept[b] + 1; t <= j; ++t)
                     if (R.beta[t] <= thr) {
                         jr = t;
                         break;
                     }
             }
             const double bnext = R.beta[jr];
+            const bool full = j >= cap && jr == j;
             Analysis& an = ans[b];
-            const int p = std::min(jr, 2 * k + 4);
+            const int p = std::min(jr, full ? std::max(2 * k + 4, LZ_KEEP_MAX + 8) : 2 * k + 4);
             analyze(R, jr, bnext, p, an);
             // Breakdown guard: Ritz values are Rayleigh quotients of H and must lie in its Gershgorin
             // interval; values outside it (or non-finite coefficients) mean the Lanczos vectors lost
@@ -677,7 +870,7 @@ class LzEngine final : public LzEngineBase {
             }
             if (broken) {
                 R.done = true;
-                R.iters = j;
+                R.iters = tot[b];
                 R.status = BH_ENOCONV;
                 R.theta.clear();
                 R.s.clear();
@@ -688,7 +881,8 @@ class LzEngine final : public LzEngineBase {
             const bool exhausted = jr < j || bnext <= std::fmax(1e-12, 100.0 * eps_store) * std::fmax(an.tnorm, 1e-300) ||
                                    j >= D;
             if (lz_debug()) {
-                std::fprintf(stderr, "[lz] j=%d lane=%d tnorm=%.3e bnext=%.3e locks=%d:", j, b, an.tnorm, bnext, nlock[b]);
+                std::fprintf(stderr, "[lz] j=%d tot=%d lane=%d tnorm=%.3e bnext=%.3e locks=%d:", j, tot[b], b, an.tnorm, bnext,
+                             nlock[b]);
                 for (size_t i = 0; i < an.theta.size() && i < 8; ++i)
                     std::fprintf(stderr, " (%.10f %.1e)", an.theta[i], an.bound[i]);
                 std::fprintf(stderr, "\n");
@@ -716,9 +910,10 @@ class LzEngine final : public LzEngineBase {
                 const double thr = std::fmax(tol * std::fmax(eps23, std::fabs(an.theta[i])), floor);
                 if (an.bound[i] >= thr && !exhausted) conv = false;
             }
-            if (conv || j >= maxit) {
+            if (conv || tot[b] >= maxsteps) {
                 R.done = true;
-                R.iters = jr;
+                R.exhausted = exhausted;
+                R.iters = tot[b];
                 R.status = conv && static_cast<int>(lev.size()) == k ? BH_OK : BH_ENOCONV;
                 R.theta.clear();
                 R.s.clear();
@@ -727,8 +922,27 @@ class LzEngine final : public LzEngineBase {
                     R.s.push_back(an.s[i]);
                 }
                 R.tnorm = an.tnorm;
-                R.all_theta = an.theta;
                 done_now[b] = 1;
+                return;
+            }
+            if (full) {
+                // thick restart: keep the lowest distinct Ritz pairs (one per cluster of copies)
+                std::vector<int> keep;
+                const int pk = std::min(LZ_KEEP_MAX - 1, std::max(1, j / 2));
+                for (int i = 0; i < static_cast<int>(an.theta.size()) && static_cast<int>(keep.size()) < pk; ++i) {
+                    if (!keep.empty() && std::fabs(an.theta[i] - an.theta[keep.back()]) <= tdedup * an.tnorm) continue;
+                    keep.push_back(i);
+                }
+                Restart& P = plan[b];
+                P.j = j;
+                P.bnext = bnext;
+                P.theta.clear();
+                P.s.clear();
+                for (int i : keep) {
+                    P.theta.push_back(an.theta[i]);
+                    P.s.push_back(an.s[i]);
+                }
+                restart_now[b] = 1;
                 return;
             }
             // selective orthogonalisation: lock converged wanted-region Ritz vectors
@@ -765,7 +979,7 @@ class LzEngine final : public LzEngineBase {
                 for (auto& th : pool) th.join();
             }
         }
-        bool any_lock = false, any_done = false;
+        bool any_lock = false, any_done = false, any_restart = false;
         for (int b = 0; b < B; ++b) {
             if (done_now[b]) {
                 act[b] = 0;
@@ -773,8 +987,10 @@ class LzEngine final : public LzEngineBase {
                 finished.push_back(b);
             }
             if (!new_locks[b].empty()) any_lock = true;
+            if (restart_now[b]) any_restart = true;
         }
         if (any_lock) BH_TRY(lock(new_locks, ans));
+        if (any_restart) BH_TRY(restart(restart_now));
         if (any_done) {
             BH_HIP(ctx, hipMemcpyAsync(st.active, act.data(), B * 4, hipMemcpyHostToDevice, s));
             BH_HIP(ctx, hipStreamSynchronize(s));
@@ -825,6 +1041,7 @@ class LzEngine final : public LzEngineBase {
                 BH_TRY(bh_apply(ctx, h, &sa, nullptr, BH_FMT_PACKED, B, s));
             }
             BH_HIP(ctx, hipGetLastError());
+            BH_TRY(rayleigh_ritz(lanes, nvf, Yf.p, HY.p));
             dots.assign(static_cast<size_t>(2 * nvf) * B, 0.0);
             hipLaunchKernelGGL((k_lz_dots<double, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yf.p, HY.p, D, nvf, 0, nullptr,
                                part.p);
@@ -868,25 +1085,49 @@ class LzEngine final : public LzEngineBase {
             L.resid.assign(k, NAN);
             L.errest.assign(k, NAN);
             const int nl = std::min(static_cast<int>(R.theta.size()), nvf);
-            // levels after the FP64 finish, ascending; Rayleigh quotients of one level agree to ~||r||^2/gap
+            // Levels after the FP64 finish, ascending.  Rayleigh quotients of one level agree to
+            // ~||r||^2/gap and are merged.  Each finished pair certifies an eigenvalue within its residual
+            // of its Rayleigh quotient (Weinstein); a pair whose interval holds the value of a pair with a
+            // smaller residual certifies nothing new (it can be an unconverged copy of that level, which
+            // finite-precision Lanczos produces) and is dropped.  A mixed-precision solve that had to drop
+            // one among its wanted levels is reported unconverged, so that it is rerun in FP64.
             std::vector<std::pair<double, int>> lv0, lv;
             for (int i = 0; i < nl; ++i) lv0.push_back({theta_f[i * B + b], i});
             std::sort(lv0.begin(), lv0.end());
+            const double mtol = dedup * std::fmax(R.tnorm, 1e-300);
+            auto rs_of = [&](const std::pair<double, int>& p) { return resid_f[p.second * B + b]; };
+            bool dropped = false;
             for (const auto& qq : lv0) {
                 if (!std::isfinite(qq.first)) continue;
-                if (!lv.empty() && std::fabs(qq.first - lv.back().first) <= dedup * std::fmax(R.tnorm, 1e-300)) {
-                    if (resid_f[qq.second * B + b] < resid_f[lv.back().second * B + b]) lv.back() = qq;
+                if (!lv.empty() && std::fabs(qq.first - lv.back().first) <= mtol) {
+                    if (rs_of(qq) < rs_of(lv.back())) lv.back() = qq;
                     continue;
+                }
+                if (!lv.empty()) {
+                    const double sep = std::fabs(qq.first - lv.back().first);
+                    const double rq = rs_of(qq), rb = rs_of(lv.back());
+                    if (rq >= sep && rq >= rb) {
+                        if (static_cast<int>(lv.size()) < k) dropped = true;
+                        continue;
+                    }
+                    if (rb >= sep && rb > rq) {
+                        if (static_cast<int>(lv.size()) <= k) dropped = true;
+                        lv.back() = qq;
+                        continue;
+                    }
                 }
                 lv.push_back(qq);
             }
+            if (MIXED && dropped && L.status == BH_OK) L.status = BH_ENOCONV;
+            // fewer than k distinct levels is an answer only when the Krylov space was exhausted
+            if (L.status == BH_OK && static_cast<int>(lv.size()) < k && !R.exhausted) L.status = BH_ENOCONV;
             if (want_vectors) L.vecs.assign(static_cast<size_t>(k) * D, 0.0);
             for (int i = 0; i < k && i < static_cast<int>(lv.size()); ++i) {
                 const double ev = lv[i].first;
-                const double rs = resid_f[lv[i].second * B + b];
-                double gap = INFINITY;  // to the nearest other Ritz value of T
-                for (double th : R.all_theta)
-                    if (std::fabs(th - ev) > dedup * std::fmax(R.tnorm, 1e-300)) gap = std::fmin(gap, std::fabs(th - ev));
+                const double rs = rs_of(lv[i]);
+                double gap = INFINITY;  // to the nearest other finished value of the lane
+                for (const auto& qq : lv0)
+                    if (std::isfinite(qq.first) && std::fabs(qq.first - ev) > mtol) gap = std::fmin(gap, std::fabs(qq.first - ev));
                 L.evals[i] = ev;
                 L.resid[i] = rs;
                 L.errest[i] = std::isfinite(gap) ? rs * rs / gap : rs;
@@ -903,6 +1144,78 @@ class LzEngine final : public LzEngineBase {
     }
 
    private:
+    // Rayleigh-Ritz over each lane's finished candidates (FP64 vectors Y, HY = H Y): with G = Y^T Y and
+    // A = Y^T H Y, the pencil (A, G) is solved on the host after dropping the numerically dependent
+    // directions of G (copies of one level that finite-precision Lanczos produces), and Y, HY are replaced
+    // in place by the Ritz vectors of the span and their images.  A candidate that mixes two eigenvectors
+    // is separated when the other one is in the span.
+    int rayleigh_ritz(const std::vector<int>& lanes, int nvf, double* Yp, double* HYp) {
+        DevArray<double> dg, dz;
+        BH_TRY(dg.alloc(ctx, static_cast<size_t>(nvf) * 2 * nvf * B));
+        for (int i = 0; i < nvf; ++i) {
+            hipLaunchKernelGGL((k_lz_gram_row<B>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yp, HYp, D, nvf, i, part.p);
+            reduce_plain(grid_k2, 2 * nvf * B, dg.p + static_cast<size_t>(i) * 2 * nvf * B);
+        }
+        std::vector<double> g(static_cast<size_t>(nvf) * 2 * nvf * B);
+        BH_HIP(ctx, hipMemcpyAsync(g.data(), dg.p, g.size() * 8, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipStreamSynchronize(s));
+        std::vector<double> Z(static_cast<size_t>(nvf) * nvf * B, 0.0);
+        for (int b : lanes) {
+            const int n = std::min(static_cast<int>(rh[b].s.size()), nvf);
+            if (n == 0) continue;
+            auto G = [&](int i, int q) { return g[(static_cast<size_t>(i) * 2 * nvf + q) * B + b]; };
+            auto A = [&](int i, int q) { return g[(static_cast<size_t>(i) * 2 * nvf + nvf + q) * B + b]; };
+            std::vector<double> Gm(n * n), gw(n), Ug(n * n);
+            for (int i = 0; i < n; ++i)
+                for (int q = 0; q < n; ++q) Gm[i * n + q] = 0.5 * (G(i, q) + G(q, i));
+            bhtri::jacobi_eig(n, Gm.data(), gw.data(), Ug.data());
+            const double gmax = gw[n - 1];
+            std::vector<int> keep;
+            for (int c = 0; c < n; ++c)
+                if (gw[c] > 1e-10 * gmax && gw[c] > 0.0) keep.push_back(c);
+            const int nk = static_cast<int>(keep.size());
+            if (nk == 0) continue;
+            // S = U_keep diag(g^-1/2) (n x nk); Ahat = S^T A S
+            std::vector<double> S(n * nk), Ah(nk * nk), th(nk), Wv(nk * nk);
+            for (int i = 0; i < n; ++i)
+                for (int c = 0; c < nk; ++c) S[i * nk + c] = Ug[i * n + keep[c]] / std::sqrt(gw[keep[c]]);
+            for (int c = 0; c < nk; ++c)
+                for (int d = 0; d < nk; ++d) {
+                    double v = 0.0;
+                    for (int i = 0; i < n; ++i)
+                        for (int q = 0; q < n; ++q) v += S[i * nk + c] * 0.5 * (A(i, q) + A(q, i)) * S[q * nk + d];
+                    Ah[c * nk + d] = v;
+                }
+            bhtri::jacobi_eig(nk, Ah.data(), th.data(), Wv.data());
+            for (int i = 0; i < n; ++i)
+                for (int c = 0; c < nk; ++c) {
+                    double v = 0.0;
+                    for (int d = 0; d < nk; ++d) v += S[i * nk + d] * Wv[d * nk + c];
+                    Z[(static_cast<size_t>(i) * nvf + c) * B + b] = v;
+                }
+        }
+        BH_TRY(dz.alloc(ctx, Z.size()));
+        BH_HIP(ctx, hipMemcpyAsync(dz.p, Z.data(), Z.size() * 8, hipMemcpyHostToDevice, s));
+        const unsigned grid = static_cast<unsigned>((vs + 255) / 256);
+        hipLaunchKernelGGL((k_lz_combine<B>), dim3(grid), dim3(256), 0, s, Yp, vs, dz.p, nvf, nvf);
+        hipLaunchKernelGGL((k_lz_combine<B>), dim3(grid), dim3(256), 0, s, HYp, vs, dz.p, nvf, nvf);
+        BH_HIP(ctx, hipGetLastError());
+        BH_HIP(ctx, hipStreamSynchronize(s));  // Z is freed on return
+        return BH_OK;
+    }
+    struct Restart {
+        int j = 0;
+        double bnext = 0.0;
+        std::vector<double> theta;
+        std::vector<std::vector<double>> s;
+    };
+
+    // next step count at which lane b must be analysed
+    int next_event(int b) const {
+        int e = std::min(nextc[b], cap);
+        e = std::min<int64_t>(e, static_cast<int64_t>(jl[b]) + std::max(0, maxsteps - tot[b]));
+        return e;
+    }
     void launch_k1(int LM) {
         KTimer kt(ctx, "lanczos_spmv", s);
 #define K1L(LMV)                                                                                                       \
@@ -920,19 +1233,27 @@ class LzEngine final : public LzEngineBase {
         KTimer kt(ctx, "lanczos_reduce", s);
         const int waves = LZ_RED_WG / 64;
         const int g = std::max(1, std::min((nv + waves - 1) / waves, 64));
-        hipLaunchKernelGGL((k_lz_reduce<B>), dim3(g), dim3(LZ_RED_WG), 0, s, st, part.p, nwg, nv, LZ_RED_PLAIN, maxit, out);
+        hipLaunchKernelGGL((k_lz_reduce<B>), dim3(g), dim3(LZ_RED_WG), 0, s, st, part.p, nwg, nv, LZ_RED_PLAIN, cap, out, nv,
+                           0.0, 0);
     }
     void step(int LM) {
         launch_k1(LM);
-        reduce_plain(grid_k1, (1 + 2 * LM) * B, st.sums);
+        {
+            KTimer kt(ctx, "lanczos_reduce", s);
+            const int nv = (1 + 2 * LM) * B;
+            const int waves = LZ_RED_WG / 64;
+            const int g = std::max(1, std::min((nv + waves - 1) / waves, 64));
+            hipLaunchKernelGGL((k_lz_reduce<B>), dim3(g), dim3(LZ_RED_WG), 0, s, st, part.p, grid_k1, nv, LZ_RED_K1, cap,
+                               st.sums, B, 0.0, 0);
+        }
         {
             KTimer kt(ctx, "lanczos_update", s);
             hipLaunchKernelGGL((k_lz_b<XT, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, a, LM);
         }
         {
             KTimer kt(ctx, "lanczos_reduce", s);
-            hipLaunchKernelGGL((k_lz_reduce<B>), dim3(1), dim3(LZ_RED_WG), 0, s, st, part.p, grid_k2, B, LZ_RED_BETA, maxit,
-                               nullptr);
+            hipLaunchKernelGGL((k_lz_reduce<B>), dim3(1), dim3(LZ_RED_WG), 0, s, st, part.p, grid_k2, B, LZ_RED_BETA, cap,
+                               nullptr, B, eps_store, lz_so_always());
         }
     }
     template <typename OT>
@@ -946,6 +1267,79 @@ class LzEngine final : public LzEngineBase {
                                dnj.p, dC.p, nv, out, os);
         }
         BH_HIP(ctx, hipGetLastError());
+        BH_HIP(ctx, hipStreamSynchronize(s));
+        return BH_OK;
+    }
+    // restarts the flagged lanes (plans made by the analysis): kept Ritz vectors in the tridiagonal basis of
+    // arrow_to_tridiag in slots 0..p-1, the normalised next Lanczos vector in slot p, T rewritten to match
+    int restart(const std::vector<int>& flag) {
+        int jmax = 0, nv = 0;
+        for (int b = 0; b < B; ++b)
+            if (flag[b]) {
+                jmax = std::max(jmax, plan[b].j);
+                nv = std::max(nv, static_cast<int>(plan[b].theta.size()) + 1);
+            }
+        std::vector<double> hsc(static_cast<size_t>(jmax + 1) * B);
+        std::vector<int> hstep(B);
+        BH_HIP(ctx, hipMemcpyAsync(hsc.data(), st.scale, hsc.size() * 8, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipMemcpyAsync(hstep.data(), st.step, B * 4, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipStreamSynchronize(s));
+        std::vector<double> C(static_cast<size_t>(jmax + 1) * nv * B, 0.0);
+        std::vector<int> nin(B, 0), nout(B, 0);
+        int pmax = 0;
+        for (int b = 0; b < B; ++b) {
+            if (!flag[b]) continue;
+            const Restart& P = plan[b];
+            const int p = static_cast<int>(P.theta.size()), j = P.j;
+            std::vector<double> sig(p), Q(static_cast<size_t>(p) * p), ta(p), tb(p);
+            double ns = 0.0;
+            for (int i = 0; i < p; ++i) {
+                sig[i] = P.bnext * P.s[i][j - 1];
+                ns += sig[i] * sig[i];
+            }
+            ns = std::sqrt(ns);
+            bhtri::arrow_to_tridiag(p, P.theta.data(), sig.data(), Q.data(), ta.data(), tb.data());
+            for (int t = 0; t < j; ++t) {
+                const double sc = hsc[static_cast<size_t>(t) * B + b];
+                for (int c = 0; c < p; ++c) {
+                    double v = 0.0;
+                    for (int i = 0; i < p; ++i) v += P.s[i][t] * Q[i + static_cast<size_t>(p) * c];
+                    C[(static_cast<size_t>(t) * nv + c) * B + b] = v * sc;
+                }
+            }
+            C[(static_cast<size_t>(j) * nv + p) * B + b] = hsc[static_cast<size_t>(j) * B + b];
+            nin[b] = j + 1;
+            nout[b] = p + 1;
+            // T of the restarted lane; rows 0..p of the host copies are written back for every lane
+            for (int c = 0; c < p; ++c) ha[static_cast<size_t>(c) * B + b] = ta[c];
+            hb[b] = 0.0;
+            for (int c = 1; c < p; ++c) hb[static_cast<size_t>(c) * B + b] = tb[c];
+            hb[static_cast<size_t>(p) * B + b] = ns;
+            for (int c = 0; c <= p; ++c) hsc[static_cast<size_t>(c) * B + b] = 1.0;
+            hstep[b] = p;
+            jl[b] = p;
+            kept[b] = p;
+            nextc[b] = p + chunk;
+            pmax = std::max(pmax, p);
+            if (lz_debug()) std::fprintf(stderr, "[lz] restart lane=%d j=%d tot=%d keep=%d sigma=%.3e\n", b, j, tot[b], p, ns);
+        }
+        BH_TRY(dC.alloc(ctx, C.size()));
+        BH_HIP(ctx, hipMemcpyAsync(dC.p, C.data(), C.size() * 8, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(dnj.p, nin.data(), B * sizeof(int), hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(dno.p, nout.data(), B * sizeof(int), hipMemcpyHostToDevice, s));
+        {
+            KTimer kt(ctx, "lanczos_restart", s);
+            hipLaunchKernelGGL((k_lz_restart<XT, B>), dim3(static_cast<unsigned>((vs + 255) / 256)), dim3(256), 0, s, V, Xc, vs,
+                               dnj.p, dno.p, dC.p, nv);
+        }
+        BH_HIP(ctx, hipGetLastError());
+        BH_HIP(ctx, hipMemcpyAsync(st.alpha, ha.data(), static_cast<size_t>(pmax) * B * 8, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(st.beta, hb.data(), static_cast<size_t>(pmax + 1) * B * 8, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(st.scale, hsc.data(), static_cast<size_t>(pmax + 1) * B * 8, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(st.step, hstep.data(), B * 4, hipMemcpyHostToDevice, s));
+        // the first two steps after a restart orthogonalise against the locked vectors
+        const int so2[2] = {1, 0};
+        BH_HIP(ctx, hipMemcpyAsync(st.so, so2, 2 * 4, hipMemcpyHostToDevice, s));
         BH_HIP(ctx, hipStreamSynchronize(s));
         return BH_OK;
     }
@@ -984,18 +1378,34 @@ class LzEngine final : public LzEngineBase {
         BH_HIP(ctx, hipMemcpyAsync(dnorm.p, nrm.data(), nrm.size() * 8, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL((k_lz_scale<XT, B>), dim3(static_cast<unsigned>((vs * nv + 255) / 256)), dim3(256), 0, s, tmp.p, vs,
                            nv, dnorm.p);
-        // vector i of lane b goes to Y slot nlock[b] + i (strided copy of one interleaved column)
+        // vector i of lane b goes to Y slot nlock[b] + i (strided copy of one interleaved column); its Ritz
+        // value and a fresh orthogonality estimate go to the device state
+        std::vector<double> lth(static_cast<size_t>(LZ_LMAX) * B), tau(static_cast<size_t>(2 * LZ_LMAX) * B), anv(B);
+        BH_HIP(ctx, hipMemcpyAsync(lth.data(), st.ltheta, lth.size() * 8, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipMemcpyAsync(tau.data(), st.tau, tau.size() * 8, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipMemcpyAsync(anv.data(), st.anorm, anv.size() * 8, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipStreamSynchronize(s));
         for (int b = 0; b < B; ++b) {
             for (int i = 0; i < static_cast<int>(new_locks[b].size()) && i < nv; ++i) {
                 const int slot = nlock[b] + i;
                 BH_HIP(ctx, hipMemcpy2DAsync(Y + static_cast<int64_t>(slot) * vs + b, B * sizeof(XT),
                                              tmp.p + static_cast<int64_t>(i) * vs + b, B * sizeof(XT), sizeof(XT),
                                              static_cast<size_t>(D), hipMemcpyDeviceToDevice, s));
-                rh[b].lock_theta.push_back(ans[b].theta[new_locks[b][i]]);
+                const double th = ans[b].theta[new_locks[b][i]];
+                rh[b].lock_theta.push_back(th);
+                lth[static_cast<size_t>(slot) * B + b] = th;
+                tau[static_cast<size_t>(slot) * B + b] = eps_store;
+                tau[static_cast<size_t>(LZ_LMAX + slot) * B + b] = eps_store;
             }
+            if (!new_locks[b].empty()) anv[b] = ans[b].tnorm;
             nlock[b] += std::min(static_cast<int>(new_locks[b].size()), nv);
             lmax_used = std::max(lmax_used, nlock[b]);
         }
+        const int so2[2] = {1, 0};  // orthogonalise the next two Lanczos vectors against the locked set
+        BH_HIP(ctx, hipMemcpyAsync(st.ltheta, lth.data(), lth.size() * 8, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(st.tau, tau.data(), tau.size() * 8, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(st.anorm, anv.data(), anv.size() * 8, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(st.so, so2, 2 * 4, hipMemcpyHostToDevice, s));
         BH_HIP(ctx, hipMemcpyAsync(st.nlock, nlock.data(), B * 4, hipMemcpyHostToDevice, s));
         BH_HIP(ctx, hipStreamSynchronize(s));
         return BH_OK;
@@ -1007,15 +1417,16 @@ class LzEngine final : public LzEngineBase {
     bh_lanczos_opts o;
     bool ws;
     int64_t D = 0, vs = 0;
-    int k = 4, maxit = 1000, chunk = 16;
+    int k = 4, maxsteps = 10000, slots = 2, cap = 1, chunk = 16;
     double tol = 1e-10, dedup = 1e-9, eps_store = 0, eps23 = 0, scale0 = 0;
     XT* V = nullptr;
     XT* Y = nullptr;
+    XT* Xc = nullptr;
     double* W = nullptr;
     double* r0 = nullptr;
-    DevArray<XT> V_own, Y_own;
+    DevArray<XT> V_own, Y_own, Xc_own;
     DevArray<double> W_own, r0_own, st_d, part, dC, dnorm;
-    DevArray<int> st_i, dnj;
+    DevArray<int> st_i, dnj, dno;
     size_t n_alpha = 0, n_beta = 0, bytes = 0;
     LzState st{};
     LzArgs<XT> a{};
@@ -1024,8 +1435,10 @@ class LzEngine final : public LzEngineBase {
     const int32_t* rblk = nullptr;
     size_t shm = 0, shm_rb = 0;
     std::vector<RunHost> rh;
-    std::vector<int> act, nlock, jl, nextc;
+    std::vector<int> act, nlock, jl, tot, kept, nextc;
     std::vector<double> Jb, Ub, hlo, hhi, ha, hb;
+    double an0[B] = {};
+    Restart plan[B];
     int lmax_used = 0;
 };
 
@@ -1062,13 +1475,28 @@ std::unique_ptr<LzEngineBase> bh_make_engine(bh_ctx* ctx, bh_ham* h, int prec, i
 size_t bh_engine_bytes(int64_t D, int prec, int B, const bh_lanczos_opts& o) {
     int Bp = 1;
     while (Bp < B) Bp <<= 1;
-    int maxit = o.max_iter > 0 ? o.max_iter : 1000;
-    if (maxit > D) maxit = static_cast<int>(D);
+    const int slots = lz_slots(D, o);
     const size_t es = prec == BH_PREC_FP64 ? 8 : 4;
     const size_t vs = static_cast<size_t>(D) * Bp;
-    return (static_cast<size_t>(maxit + 1) + LZ_LMAX) * vs * es + vs * 8 + static_cast<size_t>(D) * 8 +
+    return (static_cast<size_t>(slots) + LZ_LMAX + 1) * vs * es + vs * 8 + static_cast<size_t>(D) * 8 +
            static_cast<size_t>(LZ_GRID_MAX) * (2 * LZ_NVMAX + 1 + 2 * LZ_LMAX) * Bp * 8 +
-           (static_cast<size_t>(maxit) + 2) * Bp * 24;
+           (static_cast<size_t>(slots) + 2) * Bp * 24 + static_cast<size_t>(3 * LZ_LMAX + 1) * Bp * 8;
+}
+
+bool bh_lane_accepted(int status, const double* ev, const double* ee, int k, double accept_rel) {
+    const double acc = accept_rel > 0 ? accept_rel : 1e-9;
+    if (status != BH_OK || k < 1 || !std::isfinite(ev[0]) || !(ee[0] <= acc * std::fmax(std::fabs(ev[0]), 1e-300)))
+        return false;
+    // levels above E0: ||r||^2/gap within LZ_ACCEPT_LEVEL of the level's scale (its magnitude, at least
+    // 1e-3 of the largest wanted magnitude, so that levels near zero are not held to an absolute 0)
+    double emax = 0.0;
+    for (int i = 0; i < k; ++i)
+        if (std::isfinite(ev[i])) emax = std::fmax(emax, std::fabs(ev[i]));
+    for (int i = 1; i < k; ++i) {
+        if (!std::isfinite(ev[i])) continue;
+        if (!(ee[i] <= LZ_ACCEPT_LEVEL * std::fmax(std::fabs(ev[i]), 1e-3 * emax))) return false;
+    }
+    return true;
 }
 
 // batch solve of nreal lanes to completion on one engine (the body of bh_lanczos)
@@ -1110,7 +1538,7 @@ void bh_lanczos_default_opts(bh_lanczos_opts* o) {
     std::memset(o, 0, sizeof(*o));
     o->k = 4;
     o->prec = BH_PREC_MIXED;
-    o->max_iter = 1000;
+    o->max_iter = 10000;
     o->chunk = 16;
     o->tol = 1e-10;
     o->dedup_rel = 1e-9;
@@ -1118,6 +1546,7 @@ void bh_lanczos_default_opts(bh_lanczos_opts* o) {
     o->seed = 0;
     o->fallback = 1;
     o->want_vectors = 0;
+    o->ncv = 0;
 }
 
 namespace {
@@ -1232,10 +1661,8 @@ int bh_lanczos(bh_ctx* ctx, bh_ham* h, int B, const double* J, const double* U, 
     if (opts->prec == BH_PREC_MIXED && opts->fallback >= 0) {
         const double acc = opts->accept_rel > 0 ? opts->accept_rel : 1e-9;
         std::vector<int> bad;
-        for (int b : lz_runs) {
-            const double e0 = ev[b * k], er = ee[b * k];
-            if (st[b] != BH_OK || !std::isfinite(e0) || !(er <= acc * std::fmax(std::fabs(e0), 1e-300))) bad.push_back(b);
-        }
+        for (int b : lz_runs)
+            if (!bh_lane_accepted(st[b], &ev[b * k], &ee[b * k], k, acc)) bad.push_back(b);
         BH_TRY(solve_subset(ctx, h, bad, J, U, *opts, BH_PREC_FP64, &R, vp, true));
     }
     for (int b = 0; b < B; ++b) {

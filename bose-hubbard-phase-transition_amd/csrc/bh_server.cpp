@@ -279,9 +279,7 @@ void worker_loop(bh_server* S) {
                 Pending pd = sc.lane_req[p][lane];
                 sc.lane_busy[p][lane] = 0;
                 const LaneResult& L = res[i];
-                const double acc = S->opts.lanczos.accept_rel > 0 ? S->opts.lanczos.accept_rel : 1e-9;
-                const double e0 = L.evals.empty() ? NAN : L.evals[0];
-                const bool ok = L.status == BH_OK && std::isfinite(e0) && L.errest[0] <= acc * std::fmax(std::fabs(e0), 1e-300);
+                const bool ok = bh_lane_accepted(L.status, L.evals.data(), L.errest.data(), k, S->opts.lanczos.accept_rel);
                 if (p == 0 && !ok && S->opts.lanczos.fallback >= 0) {
                     pd.prec = BH_PREC_FP64;
                     sc.queue[1].push_back(pd);  // rerun in FP64

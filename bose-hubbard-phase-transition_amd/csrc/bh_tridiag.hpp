@@ -116,6 +116,132 @@ inline void inverse_iteration(int n, const double* a, const double* b, double th
     for (int t = 0; t < n; ++t) s[t] = sg * x[t];
 }
 
+// Thick restart: the kept Ritz pairs (theta_i, y_i) of a Lanczos run couple to the next Lanczos vector
+// only through sigma_i (H y_i = theta_i y_i + sigma_i v).  This builds an orthonormal p x p matrix Q
+// (column-major, Q[i + p * c]) such that Q^T diag(theta) Q is tridiagonal (diagonal a[0..p-1],
+// off-diagonal b[1..p-1], b[c] coupling c-1 and c) and the last column is sigma / ||sigma||; in the basis
+// u_c = sum_i Q[i, c] y_i only u_{p-1} couples to v, with coefficient ||sigma||, so the restarted run
+// continues with the ordinary three-term recurrence.  Lanczos with full reorthogonalisation on
+// diag(theta) started from sigma, then reversed; after a breakdown (a kept pair that does not couple to
+// v) it continues from the unit vector least represented so far, with coupling 0.
+inline void arrow_to_tridiag(int p, const double* theta, const double* sigma, double* Q, double* a, double* b) {
+    std::vector<double> q(static_cast<size_t>(p) * p, 0.0), al(p, 0.0), be(p + 1, 0.0), w(p);
+    double ns = 0.0, tmax = 0.0;
+    for (int i = 0; i < p; ++i) {
+        ns += sigma[i] * sigma[i];
+        tmax = std::fmax(tmax, std::fabs(theta[i]));
+    }
+    ns = std::sqrt(ns);
+    auto fresh = [&](int col) {
+        // unit vector e_m with the smallest weight in columns [0, col), orthogonalised against them
+        int best = 0;
+        double bw = 1e300;
+        for (int m = 0; m < p; ++m) {
+            double s = 0.0;
+            for (int c = 0; c < col; ++c) s += q[m + static_cast<size_t>(p) * c] * q[m + static_cast<size_t>(p) * c];
+            if (s < bw) {
+                bw = s;
+                best = m;
+            }
+        }
+        for (int i = 0; i < p; ++i) w[i] = (i == best) ? 1.0 : 0.0;
+    };
+    auto orth_into = [&](int col) {
+        for (int rep = 0; rep < 2; ++rep)
+            for (int c = 0; c < col; ++c) {
+                double d = 0.0;
+                for (int i = 0; i < p; ++i) d += q[i + static_cast<size_t>(p) * c] * w[i];
+                for (int i = 0; i < p; ++i) w[i] -= d * q[i + static_cast<size_t>(p) * c];
+            }
+        double n2 = 0.0;
+        for (int i = 0; i < p; ++i) n2 += w[i] * w[i];
+        n2 = std::sqrt(n2);
+        for (int i = 0; i < p; ++i) q[i + static_cast<size_t>(p) * col] = n2 > 0 ? w[i] / n2 : 0.0;
+        return n2;
+    };
+    if (ns > 0.0) {
+        for (int i = 0; i < p; ++i) w[i] = sigma[i] / ns;
+    } else {
+        fresh(0);
+    }
+    orth_into(0);
+    const double brk = 1e-13 * std::fmax(tmax, 1e-300);
+    for (int c = 0; c < p; ++c) {
+        const double* qc = &q[static_cast<size_t>(p) * c];
+        for (int i = 0; i < p; ++i) w[i] = theta[i] * qc[i];
+        double alpha = 0.0;
+        for (int i = 0; i < p; ++i) alpha += qc[i] * w[i];
+        al[c] = alpha;
+        if (c + 1 == p) break;
+        const double nb = orth_into(c + 1);  // w minus its components on columns 0..c, normalised
+        if (nb > brk) {
+            be[c + 1] = nb;
+        } else {
+            be[c + 1] = 0.0;
+            fresh(c + 1);
+            orth_into(c + 1);
+        }
+    }
+    // reversed order: new column c is forward column p-1-c
+    for (int c = 0; c < p; ++c) {
+        a[c] = al[p - 1 - c];
+        for (int i = 0; i < p; ++i) Q[i + static_cast<size_t>(p) * c] = q[i + static_cast<size_t>(p) * (p - 1 - c)];
+    }
+    b[0] = 0.0;
+    for (int c = 0; c + 1 < p; ++c) b[c + 1] = be[p - 1 - c];
+}
+
+// eigen-decomposition of a small dense symmetric matrix (cyclic Jacobi): A (n x n, row-major, destroyed),
+// eigenvalues w ascending, eigenvectors as the columns of V (row-major V[i * n + c])
+inline void jacobi_eig(int n, double* A, double* w, double* V) {
+    for (int i = 0; i < n; ++i)
+        for (int c = 0; c < n; ++c) V[i * n + c] = (i == c) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int c = 0; c < n; ++c) {
+                tot += A[i * n + c] * A[i * n + c];
+                if (i != c) off += A[i * n + c] * A[i * n + c];
+            }
+        if (off <= 1e-30 * tot || off == 0.0) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = A[p * n + q];
+                if (apq == 0.0) continue;
+                const double th = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int r = 0; r < n; ++r) {
+                    const double arp = A[r * n + p], arq = A[r * n + q];
+                    A[r * n + p] = c * arp - s * arq;
+                    A[r * n + q] = s * arp + c * arq;
+                }
+                for (int r = 0; r < n; ++r) {
+                    const double apr = A[p * n + r], aqr = A[q * n + r];
+                    A[p * n + r] = c * apr - s * aqr;
+                    A[q * n + r] = s * apr + c * aqr;
+                }
+                for (int r = 0; r < n; ++r) {
+                    const double vrp = V[r * n + p], vrq = V[r * n + q];
+                    V[r * n + p] = c * vrp - s * vrq;
+                    V[r * n + q] = s * vrp + c * vrq;
+                }
+            }
+    }
+    std::vector<int> ord(n);
+    for (int i = 0; i < n; ++i) ord[i] = i;
+    std::vector<double> d(n);
+    for (int i = 0; i < n; ++i) d[i] = A[i * n + i];
+    for (int i = 1; i < n; ++i)
+        for (int j = i; j > 0 && d[ord[j]] < d[ord[j - 1]]; --j) std::swap(ord[j], ord[j - 1]);
+    std::vector<double> Vs(static_cast<size_t>(n) * n);
+    for (int c = 0; c < n; ++c) {
+        w[c] = d[ord[c]];
+        for (int r = 0; r < n; ++r) Vs[r * n + c] = V[r * n + ord[c]];
+    }
+    for (int i = 0; i < n * n; ++i) V[i] = Vs[i];
+}
+
 // all eigenvalues of a symmetric tridiagonal matrix, ascending (implicit QL, Wilkinson shifts)
 inline int ql_eigvals(int n, const double* d_in, const double* e_in, double* w) {
     std::vector<double> d(d_in, d_in + n), e(static_cast<size_t>(n) + 1, 0.0);

@@ -145,7 +145,7 @@ int bh_spmv_bench(bh_ctx* ctx, bh_ham* h, int prec, int B, int reps, double* avg
 typedef struct bh_lanczos_opts {
     int k;             /* wanted eigenvalues (distinct levels), >= 1 */
     int prec;          /* BH_PREC_FP64 or BH_PREC_MIXED */
-    int max_iter;      /* Krylov dimension cap (0: 1000) */
+    int max_iter;      /* Lanczos steps (operator applications) per solve, over all restarts (0: 10000) */
     int chunk;         /* steps between convergence checks (0: 16) */
     double tol;        /* Spectra rule: |beta s_last| < tol * max(eps^(2/3), |theta|) (0: 1e-10) */
     double dedup_rel;  /* level merge tolerance relative to ||T|| (0: 1e-9) */
@@ -153,6 +153,8 @@ typedef struct bh_lanczos_opts {
     uint64_t seed;     /* Park-Miller start vector seed (0 -> 1, Spectra's convention) */
     int fallback;      /* mixed: rerun failing solves in FP64 (1, default) or report them (-1) */
     int want_vectors;  /* 1: keep the k Ritz vectors (bh_lanczos_copy_vectors) */
+    int ncv;           /* Krylov vectors kept per solve (Spectra's ncv); a solve that fills them is thick-
+                          restarted with its lowest Ritz vectors (0: min(D, 512); at least 2k + 8) */
 } bh_lanczos_opts;
 
 typedef struct bh_lanczos_result {

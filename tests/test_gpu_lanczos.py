@@ -153,3 +153,36 @@ def test_full_size_mixed_vs_fp64(ctx, geom, n, J):
     print("fp64", r64.evals[0], r64.iters, "mixed", r32.evals[0], r32.iters, r32.used_fp64, flush=True)
     h.close()
     b.close()
+
+
+@pytest.mark.parametrize("prec", [bhx.PREC_FP64, bhx.PREC_MIXED])
+def test_thick_restart_small_basis(ctx, prec):
+    """A Krylov basis of 24 vectors forces thick restarts; levels still equal the dense spectrum."""
+    m, off, sites, b, h = build(ctx, ("chain", 8, True), 8)
+    ho = O.hamiltonian(m, 8, off, sites)
+    want = O.distinct_levels(np.linalg.eigvalsh(ho.dense(0.3, 1.0)), 4)
+    r = h.lanczos(0.3, 1.0, k=4, prec=prec, ncv=24)
+    print("restarted", r.iters, r.used_fp64, r.evals[0], flush=True)
+    assert r.status[0] == 0 and r.iters[0] > 24
+    assert rel(r.evals[0, 0], want[0]) <= 1e-9
+    assert np.all(rel(r.evals[0], want) <= (1e-9 if prec == bhx.PREC_FP64 else MIXED_EXCITED))
+    h.close()
+    b.close()
+
+
+@pytest.mark.parametrize("prec", [bhx.PREC_FP64, bhx.PREC_MIXED])
+def test_small_hopping_needs_restarts(ctx, prec):
+    """Deep in the Mott regime the low band (width ~4J) is tiny against ||H|| ~ U n^2 / 2; with a basis of
+    32 vectors the solve runs through several thick restarts (chain m=8, n=7, J/U=0.004)."""
+    m, off, sites, b, h = build(ctx, ("chain", 8, True), 7)
+    ho = O.hamiltonian(m, 7, off, sites)
+    want = O.distinct_levels(np.linalg.eigvalsh(ho.dense(0.004, 1.0)), 4)
+    r = h.lanczos(0.004, 1.0, k=4, prec=prec, ncv=32)
+    print("small J", r.iters, r.used_fp64, r.evals[0], want, flush=True)
+    assert r.status[0] == 0 and r.iters[0] > 32
+    assert rel(r.evals[0, 0], want[0]) <= 1e-9
+    # the band holds a level at ~J^2 / U: errors are measured against the band scale J
+    scale = np.maximum(np.abs(want), 0.004)
+    assert np.all(np.abs(r.evals[0] - want) <= (1e-9 if prec == bhx.PREC_FP64 else MIXED_EXCITED) * scale)
+    h.close()
+    b.close()
