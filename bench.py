@@ -61,11 +61,26 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES",
+                    help="only run one device-to-device copy kernel of BYTES bytes (calibrates HBM counters) and exit")
+    ap.add_argument("--quick", action="store_true",
+                    help="time the mixed batch only (no plain FP64 comparator, SpMV table or CPU leg): for profilers")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported. Solves are independent: a "
               "multi-GPU run shards batches or requests across devices (see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calib:
+        import torch  # the copy kernel of the calibration only
+
+        a = torch.ones(args.copy_calib, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copied {args.copy_calib} bytes")
+        return
 
     m, n, J0, U, mu, k = 12, 12, 0.1, 1.0, 0.5, 4
     B = args.batch
@@ -80,13 +95,15 @@ def main():
     log(f"build D={ham.D} nnz={ham.nnz} {build_s * 1e3:.1f} ms")
 
     # plain FP64 GPU Lanczos: FP64 CSR and vectors, unbatched (accuracy comparator and baseline rate)
-    r64 = ham.lanczos(J0, U, k=k, prec=bhx.PREC_FP64)
-    reps64 = 3
-    t0 = time.perf_counter()
-    for _ in range(reps64):
+    fp64_s, r64 = float("nan"), None
+    if not args.quick:
         r64 = ham.lanczos(J0, U, k=k, prec=bhx.PREC_FP64)
-    fp64_s = (time.perf_counter() - t0) / reps64
-    log(f"plain fp64: {fp64_s * 1e3:.1f} ms/solve, steps={r64.iters[0]}, E={r64.evals[0]}")
+        reps64 = 3
+        t0 = time.perf_counter()
+        for _ in range(reps64):
+            r64 = ham.lanczos(J0, U, k=k, prec=bhx.PREC_FP64)
+        fp64_s = (time.perf_counter() - t0) / reps64
+        log(f"plain fp64: {fp64_s * 1e3:.1f} ms/solve, steps={r64.iters[0]}, E={r64.evals[0]}")
 
     for _ in range(args.warmup):
         ham.lanczos(Js, U, k=k, prec=bhx.PREC_MIXED)
@@ -102,7 +119,7 @@ def main():
     upd_count, upd_total = ctx.kernel_stats("lanczos_update")
     ms_step = dt / args.steps * 1e3
     value = B * args.steps / dt
-    rel0 = abs(res.evals[0, 0] - r64.evals[0, 0]) / abs(r64.evals[0, 0])
+    rel0 = abs(res.evals[0, 0] - r64.evals[0, 0]) / abs(r64.evals[0, 0]) if r64 is not None else float("nan")
     log(f"mixed batched B={B}: {ms_step:.1f} ms/step ({value:.1f} solves/s), steps={res.iters.tolist()}, "
         f"fp64 reruns={int(res.used_fp64.sum())}, |dE0|/|E0| vs fp64 = {rel0:.2e}")
 
@@ -117,14 +134,14 @@ def main():
         f"K2 avg {upd_total / max(upd_count, 1) * 1e3:.1f} us")
 
     spmv = {}
-    for prec, name in ((bhx.PREC_MIXED, "mixed"), (bhx.PREC_FP64, "fp64")):
+    for prec, name in () if args.quick else ((bhx.PREC_MIXED, "mixed"), (bhx.PREC_FP64, "fp64")):
         for Bs in (1, B):
             ms, by = ham.spmv_bench(prec, Bs, 30)
             spmv[f"{name}_B{Bs}"] = {"us": ms * 1e3, "alg_GBs": by / ms / 1e6}
     log("spmv:", json.dumps(spmv))
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not args.quick:
         cpu = cpu_baseline(m, n, J0, U, k, int(r64.iters[0]))
         log("cpu baseline:", json.dumps(cpu))
 

@@ -42,7 +42,8 @@ class BHError(RuntimeError):
 class LanczosOpts(C.Structure):
     _fields_ = [("k", C.c_int), ("prec", C.c_int), ("max_iter", C.c_int), ("chunk", C.c_int),
                 ("tol", C.c_double), ("dedup_rel", C.c_double), ("accept_rel", C.c_double),
-                ("seed", C.c_uint64), ("fallback", C.c_int), ("want_vectors", C.c_int), ("ncv", C.c_int)]
+                ("seed", C.c_uint64), ("fallback", C.c_int), ("want_vectors", C.c_int), ("ncv", C.c_int),
+                ("fullre_d", C.c_int)]
 
 
 class SweepSpec(C.Structure):

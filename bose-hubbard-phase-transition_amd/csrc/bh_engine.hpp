@@ -27,8 +27,11 @@ class LzEngineBase {
     virtual int steps_of(int slot) const = 0;
     virtual int suggest_chunk() const = 0;  // steps until the next useful analysis
     virtual size_t device_bytes() const = 0;
+    virtual bool fp64() const = 0;  // Krylov vectors stored in FP64
 };
 
+// sectors with D at most opts.fullre_d (0: 16384, < 0: none) run FP64 with full reorthogonalisation
+bool bh_lz_fullre(int64_t D, const bh_lanczos_opts& o);
 // use_ctx_ws: keep the Krylov basis in the context's workspaces (one engine at a time per context);
 // otherwise the engine owns its buffers (several engines may coexist, e.g. in the server)
 std::unique_ptr<LzEngineBase> bh_make_engine(bh_ctx* ctx, bh_ham* h, int prec, int B, const bh_lanczos_opts& o,
@@ -36,5 +39,7 @@ std::unique_ptr<LzEngineBase> bh_make_engine(bh_ctx* ctx, bh_ham* h, int prec, i
 // acceptance of a mixed-precision solve after the FP64 finish (else it is rerun in FP64): converged,
 // E0 within accept_rel (0: 1e-9) by its bound ||r||^2/gap, and the levels above it within a looser bound
 bool bh_lane_accepted(int status, const double* evals, const double* errest, int k, double accept_rel);
+// options of the FP64 rerun of a rejected mixed solve
+bh_lanczos_opts bh_fallback_opts(const bh_lanczos_opts& o);
 // bytes an engine of B lanes allocates for itself (use_ctx_ws = false)
 size_t bh_engine_bytes(int64_t D, int prec, int B, const bh_lanczos_opts& o);

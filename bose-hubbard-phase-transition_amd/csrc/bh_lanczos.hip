@@ -29,6 +29,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <thread>
 
@@ -259,6 +260,58 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_b(LzArgs<XT> a, int lm) {
     lz_block_put<B>(pn, 0, lds, a.st.part, B);
 }
 
+// Full reorthogonalisation (small sectors), classical Gram-Schmidt pass over the whole basis: workgroup t
+// computes dd[t][b] = scale_t <V[t][.][b], v_{j+1}[.][b]> for the lanes whose step j has t <= j.
+template <typename XT, int B>
+__global__ void __launch_bounds__(LZ_WG) k_lz_fr_dots(LzArgs<XT> a, double* __restrict__ dd) {
+    __shared__ double lds[LZ_WG / 64 * B];
+    const int t = blockIdx.x;
+    const int b = threadIdx.x % B;
+    const int j = a.st.step[b];
+    const bool act = a.st.active[b] != 0 && j < a.maxit && t <= j;
+    double acc = 0.0;
+    if (act) {
+        const XT* vt = a.V + static_cast<int64_t>(t) * a.vs;
+        const XT* vn = a.V + static_cast<int64_t>(j + 1) * a.vs;
+        for (int64_t r = threadIdx.x / B; r < a.D; r += LZ_WG / B) acc += to_d(vt[r * B + b]) * to_d(vn[r * B + b]);
+        acc *= a.st.scale[t * B + b];
+    }
+    lz_block_put<B>(acc, 0, lds, dd, B);
+}
+
+// v_{j+1} -= sum_{t <= j} dd[t] scale_t V[t]; the last pass also refreshes the current vector and writes the
+// partials of ||v_{j+1}||^2 (as K2 does)
+template <typename XT, int B>
+__global__ void __launch_bounds__(LZ_WG) k_lz_fr_axpy(LzArgs<XT> a, const double* __restrict__ dd, int last) {
+    __shared__ double lds[LZ_WG / 64 * B];
+    const int b = threadIdx.x % B;
+    const int j = a.st.step[b];
+    const bool act = a.st.active[b] != 0 && j < a.maxit;
+    constexpr int ROWS = LZ_WG / B;
+    const int64_t ntile = (a.D + ROWS - 1) / ROWS;
+    const int64_t per = (ntile + gridDim.x - 1) / gridDim.x;
+    const int64_t t0 = min(static_cast<int64_t>(blockIdx.x) * per, ntile), t1 = min(t0 + per, ntile);
+    double pn = 0.0;
+    if (act) {
+        XT* vn = a.V + static_cast<int64_t>(j + 1) * a.vs;
+        for (int64_t tt = t0; tt < t1; ++tt) {
+            const int64_t r = tt * ROWS + threadIdx.x / B;
+            if (r >= a.D) continue;
+            const int64_t idx = r * B + b;
+            double w = to_d(vn[idx]);
+            for (int t = 0; t <= j; ++t) w -= dd[t * B + b] * a.st.scale[t * B + b] * to_d(a.V[static_cast<int64_t>(t) * a.vs + idx]);
+            const XT wr = static_cast<XT>(w);
+            vn[idx] = wr;
+            if (last) {
+                a.Xc[idx] = wr;
+                const double wd = to_d(wr);
+                pn += wd * wd;
+            }
+        }
+    }
+    if (last) lz_block_put<B>(pn, 0, lds, a.st.part, B);
+}
+
 // V[0][r][b] = r0[r] in storage precision (start vector shared by all b)
 template <typename XT, int B>
 __global__ void k_lz_init(XT* __restrict__ V0, const double* __restrict__ r0, int64_t D) {
@@ -385,34 +438,58 @@ __global__ void __launch_bounds__(256) k_lz_restart(XT* V, XT* __restrict__ Xc, 
 }
 
 // Ritz vectors: out[i][r][b] = sum_{t < nj[b]} V[t][r][b] * C[(t * nv + i) * B + b]   (C includes scale_t).
-// Each run stops at its own step count: Krylov slots past it were never written for that run.
+// Each run stops at its own step count: Krylov slots past it were never written for that run.  The
+// coefficients of a run of slots are staged in LDS once per workgroup (they are shared by all rows), and
+// each thread carries LZ_RZ_R elements, so that the Krylov basis is streamed once with several
+// independent loads in flight.
+constexpr int LZ_RZ_R = 4;       // elements per thread (strided by the workgroup size)
+constexpr int LZ_RZ_LDS = 4096;  // coefficients staged per pass (32 KB)
+
 template <typename XT, typename OT, int B>
-__global__ void __launch_bounds__(256) k_lz_ritz(const XT* __restrict__ V, int64_t vs, const int* __restrict__ njb,
+__global__ void __launch_bounds__(256) k_lz_ritz(const XT* __restrict__ V, int64_t vs, const int* __restrict__ njb, int njmax,
                                                  const double* __restrict__ C, int nv, OT* __restrict__ out, int64_t os) {
-    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (idx >= vs) return;
-    const int b = static_cast<int>(idx % B);
+    __shared__ double cs[LZ_RZ_LDS];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * (256 * LZ_RZ_R) + threadIdx.x;
+    const int b = static_cast<int>(base % B);  // the same lane for all of this thread's elements (256 % B == 0)
     const int nj = njb[b];
-    double acc[LZ_NVMAX];
+    double acc[LZ_RZ_R][LZ_NVMAX];
 #pragma unroll
-    for (int i = 0; i < LZ_NVMAX; ++i) acc[i] = 0.0;
-    constexpr int UT = 8;  // Krylov vectors read per unrolled step (independent loads in flight)
-    for (int t0 = 0; t0 < nj; t0 += UT) {
-        double v[UT];
+    for (int u = 0; u < LZ_RZ_R; ++u)
 #pragma unroll
-        for (int u = 0; u < UT; ++u) v[u] = (t0 + u < nj) ? to_d(V[static_cast<int64_t>(t0 + u) * vs + idx]) : 0.0;
+        for (int i = 0; i < LZ_NVMAX; ++i) acc[u][i] = 0.0;
+    const int per_t = nv * B;
+    const int tc = max(1, LZ_RZ_LDS / per_t);
+    for (int t0 = 0; t0 < njmax; t0 += tc) {
+        const int t1 = min(t0 + tc, njmax);
+        __syncthreads();
+        for (int q = threadIdx.x; q < (t1 - t0) * per_t; q += 256) cs[q] = C[static_cast<int64_t>(t0) * per_t + q];
+        __syncthreads();
+        const int te = min(t1, nj);
+        for (int t = t0; t < te; ++t) {
+            double v[LZ_RZ_R];
 #pragma unroll
-        for (int u = 0; u < UT; ++u) {
-            if (t0 + u >= nj) break;
-            const double* ct = C + (static_cast<int64_t>(t0 + u) * nv) * B + b;
+            for (int u = 0; u < LZ_RZ_R; ++u) {
+                const int64_t idx = base + u * 256;
+                v[u] = idx < vs ? to_d(V[static_cast<int64_t>(t) * vs + idx]) : 0.0;
+            }
+            const double* ct = cs + (t - t0) * per_t + b;
 #pragma unroll
-            for (int i = 0; i < LZ_NVMAX; ++i)
-                if (i < nv) acc[i] += v[u] * ct[i * B];
+            for (int i = 0; i < LZ_NVMAX; ++i) {
+                if (i >= nv) break;
+                const double c = ct[i * B];
+#pragma unroll
+                for (int u = 0; u < LZ_RZ_R; ++u) acc[u][i] += v[u] * c;
+            }
         }
     }
 #pragma unroll
-    for (int i = 0; i < LZ_NVMAX; ++i)
-        if (i < nv) out[static_cast<int64_t>(i) * os + idx] = static_cast<OT>(acc[i]);
+    for (int u = 0; u < LZ_RZ_R; ++u) {
+        const int64_t idx = base + u * 256;
+        if (idx >= vs) continue;
+#pragma unroll
+        for (int i = 0; i < LZ_NVMAX; ++i)
+            if (i < nv) out[static_cast<int64_t>(i) * os + idx] = static_cast<OT>(acc[u][i]);
+    }
 }
 
 // per-(i, b) partial sums over rows of [nv][D][B] arrays (reduced by k_lz_reduce mode 0):
@@ -617,6 +694,14 @@ static int lz_slots(int64_t D, const bh_lanczos_opts& o) {
     return static_cast<int>(std::max<int64_t>(cap, 1)) + 1;
 }
 
+// Sectors up to this dimension run FP64 Lanczos with full reorthogonalisation: their solves are bound by
+// launch latency, not bandwidth, so FP32 storage buys nothing there, and reorthogonalising against the whole
+// basis (j D B words per step) is cheap.
+bool bh_lz_fullre(int64_t D, const bh_lanczos_opts& o) {
+    const int64_t lim = o.fullre_d > 0 ? o.fullre_d : (o.fullre_d < 0 ? 0 : 16384);
+    return D <= lim;
+}
+
 template <typename XT, int B>
 class LzEngine final : public LzEngineBase {
     static constexpr bool MIXED = std::is_same<XT, float>::value;
@@ -633,6 +718,7 @@ class LzEngine final : public LzEngineBase {
         maxsteps = o.max_iter > 0 ? o.max_iter : 10000;
         slots = lz_slots(D, o);
         cap = slots - 1;
+        fullre = bh_lz_fullre(D, o);
         chunk = o.chunk > 0 ? o.chunk : 16;
         tol = o.tol > 0 ? o.tol : 1e-10;
         dedup = o.dedup_rel > 0 ? o.dedup_rel : 1e-9;
@@ -675,6 +761,7 @@ class LzEngine final : public LzEngineBase {
         BH_TRY(dalloc(ctx, dnorm, static_cast<size_t>(2 * LZ_NVMAX * B)));
         BH_TRY(dalloc(ctx, dnj, B));
         BH_TRY(dalloc(ctx, dno, B));
+        if (fullre) BH_TRY(dalloc(ctx, dd, static_cast<size_t>(slots) * B));
         st.alpha = st_d.p;
         st.beta = st.alpha + n_alpha;
         st.scale = st.beta + n_beta;
@@ -758,8 +845,9 @@ class LzEngine final : public LzEngineBase {
     }
     int steps_of(int slot) const override { return tot[slot]; }
     size_t device_bytes() const override { return bytes; }
+    bool fp64() const override { return !MIXED; }
     // Steps until the next lane reaches its own analysis point.  Each lane follows its own schedule
-    // (first at max(2k+8, 20) steps, then every max(chunk, j/4); always when its Krylov slots are full
+    // (first at max(2k+8, 20) steps, then every max(chunk, j/8); always when its Krylov slots are full
     // and when its step budget is spent), so the steps at which a lane is analysed, locked, restarted
     // and retired do not depend on the other lanes.
     int suggest_chunk() const override {
@@ -835,7 +923,7 @@ class LzEngine final : public LzEngineBase {
             RunHost& R = rh[b];
             const int j = jl[b];
             if (j < next_event(b)) return;
-            nextc[b] = j + std::max(chunk, j / 4);
+            nextc[b] = j + std::max(chunk, j / 8);
             R.alpha.assign(j, 0.0);
             R.beta.assign(j + 1, 0.0);
             for (int t = 0; t < j; ++t) R.alpha[t] = ha[static_cast<size_t>(t) * B + b];
@@ -893,10 +981,18 @@ class LzEngine final : public LzEngineBase {
             // split at the storage rounding (~eps32 ||T||), so they are merged at that scale here and
             // separated again, if they are distinct, by the FP64 finish.
             const double tdedup = MIXED ? std::fmax(dedup, 50.0 * eps_store) : dedup;
+            // lev[q]: first Ritz value of the q-th distinct level; lbound[q]: the smallest bound among the
+            // level's copies (a copy that keeps converging carries the level)
             std::vector<int> lev;
-            for (int i = 0; i < static_cast<int>(an.theta.size()) && static_cast<int>(lev.size()) < k; ++i) {
-                if (!lev.empty() && std::fabs(an.theta[i] - an.theta[lev.back()]) <= tdedup * an.tnorm) continue;
+            std::vector<double> lbound;
+            for (int i = 0; i < static_cast<int>(an.theta.size()); ++i) {
+                if (!lev.empty() && std::fabs(an.theta[i] - an.theta[lev.back()]) <= tdedup * an.tnorm) {
+                    lbound.back() = std::fmin(lbound.back(), an.bound[i]);
+                    continue;
+                }
+                if (static_cast<int>(lev.size()) == k) break;
                 lev.push_back(i);
+                lbound.push_back(an.bound[i]);
             }
             // extra candidates finished in FP64 beside the k wanted levels
             std::vector<int> cand = lev;
@@ -906,9 +1002,10 @@ class LzEngine final : public LzEngineBase {
                 cand.push_back(i);
             }
             bool conv = static_cast<int>(lev.size()) == k || exhausted;
-            for (int i : lev) {
-                const double thr = std::fmax(tol * std::fmax(eps23, std::fabs(an.theta[i])), floor);
-                if (an.bound[i] >= thr && !exhausted) conv = false;
+            std::vector<double> cthr(lev.size());
+            for (size_t q = 0; q < lev.size(); ++q) {
+                cthr[q] = std::fmax(tol * std::fmax(eps23, std::fabs(an.theta[lev[q]])), floor);
+                if (lbound[q] >= cthr[q] && !exhausted) conv = false;
             }
             if (conv || tot[b] >= maxsteps) {
                 R.done = true;
@@ -945,9 +1042,12 @@ class LzEngine final : public LzEngineBase {
                 restart_now[b] = 1;
                 return;
             }
-            // selective orthogonalisation: lock converged wanted-region Ritz vectors
+            // selective orthogonalisation: lock converged wanted-region Ritz vectors (Parlett and Scott:
+            // bound below sqrt(eps) ||T||); not needed when every step reorthogonalises fully
+            if (fullre) return;
             const double lock_thr = std::sqrt(eps_store) * an.tnorm;
-            for (int i : lev) {
+            for (size_t q = 0; q < lev.size(); ++q) {
+                const int i = lev[q];
                 if (an.bound[i] > lock_thr) continue;
                 if (nlock[b] + static_cast<int>(new_locks[b].size()) >= LZ_LMAX) break;
                 // a level already locked keeps its Ritz value only up to its bound: match within the
@@ -1250,21 +1350,34 @@ class LzEngine final : public LzEngineBase {
             KTimer kt(ctx, "lanczos_update", s);
             hipLaunchKernelGGL((k_lz_b<XT, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, a, LM);
         }
+        if (fullre) {
+            // two classical Gram-Schmidt passes against every stored Lanczos vector of the lane
+            KTimer kt(ctx, "lanczos_reorth", s);
+            for (int pass = 0; pass < 2; ++pass) {
+                hipLaunchKernelGGL((k_lz_fr_dots<XT, B>), dim3(static_cast<unsigned>(cap)), dim3(LZ_WG), 0, s, a, dd.p);
+                hipLaunchKernelGGL((k_lz_fr_axpy<XT, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, a, dd.p, pass);
+            }
+        }
         {
             KTimer kt(ctx, "lanczos_reduce", s);
             hipLaunchKernelGGL((k_lz_reduce<B>), dim3(1), dim3(LZ_RED_WG), 0, s, st, part.p, grid_k2, B, LZ_RED_BETA, cap,
                                nullptr, B, eps_store, lz_so_always());
         }
     }
+    // C holds rows t < max(njb) of [(t * nv + i) * B + b]
     template <typename OT>
     int form_vectors(const std::vector<int>& njb, int nv, const std::vector<double>& C, OT* out, int64_t os) {
+        int njmax = 0;
+        for (int b = 0; b < B; ++b) njmax = std::max(njmax, njb[b]);
+        if (C.size() < static_cast<size_t>(njmax) * nv * B) return bh_set_error(ctx, BH_EINVAL, "ritz: coefficient rows");
         BH_TRY(dC.alloc(ctx, C.size()));
         BH_HIP(ctx, hipMemcpyAsync(dC.p, C.data(), C.size() * 8, hipMemcpyHostToDevice, s));
         BH_HIP(ctx, hipMemcpyAsync(dnj.p, njb.data(), B * sizeof(int), hipMemcpyHostToDevice, s));
         {
             KTimer kt(ctx, "lanczos_ritz", s);
-            hipLaunchKernelGGL((k_lz_ritz<XT, OT, B>), dim3(static_cast<unsigned>((vs + 255) / 256)), dim3(256), 0, s, V, vs,
-                               dnj.p, dC.p, nv, out, os);
+            const int64_t per_wg = 256 * LZ_RZ_R;
+            hipLaunchKernelGGL((k_lz_ritz<XT, OT, B>), dim3(static_cast<unsigned>((vs + per_wg - 1) / per_wg)), dim3(256), 0, s,
+                               V, vs, dnj.p, njmax, dC.p, nv, out, os);
         }
         BH_HIP(ctx, hipGetLastError());
         BH_HIP(ctx, hipStreamSynchronize(s));
@@ -1418,6 +1531,7 @@ class LzEngine final : public LzEngineBase {
     bool ws;
     int64_t D = 0, vs = 0;
     int k = 4, maxsteps = 10000, slots = 2, cap = 1, chunk = 16;
+    bool fullre = false;  // every step reorthogonalises against the whole basis (small sectors)
     double tol = 1e-10, dedup = 1e-9, eps_store = 0, eps23 = 0, scale0 = 0;
     XT* V = nullptr;
     XT* Y = nullptr;
@@ -1425,7 +1539,7 @@ class LzEngine final : public LzEngineBase {
     double* W = nullptr;
     double* r0 = nullptr;
     DevArray<XT> V_own, Y_own, Xc_own;
-    DevArray<double> W_own, r0_own, st_d, part, dC, dnorm;
+    DevArray<double> W_own, r0_own, st_d, part, dC, dnorm, dd;
     DevArray<int> st_i, dnj, dno;
     size_t n_alpha = 0, n_beta = 0, bytes = 0;
     LzState st{};
@@ -1447,6 +1561,7 @@ std::unique_ptr<LzEngineBase> bh_make_engine(bh_ctx* ctx, bh_ham* h, int prec, i
                                              hipStream_t s, bool use_ctx_ws, int* rc) {
     std::unique_ptr<LzEngineBase> e;
     *rc = BH_OK;
+    if (bh_lz_fullre(h->D, o)) prec = BH_PREC_FP64;
     if (prec == BH_PREC_FP64) {
         *rc = bh_ham_prepare_locked(h, BH_PREC_FP64);
         if (*rc != BH_OK) return nullptr;
@@ -1476,25 +1591,39 @@ size_t bh_engine_bytes(int64_t D, int prec, int B, const bh_lanczos_opts& o) {
     int Bp = 1;
     while (Bp < B) Bp <<= 1;
     const int slots = lz_slots(D, o);
-    const size_t es = prec == BH_PREC_FP64 ? 8 : 4;
+    const bool fr = bh_lz_fullre(D, o);
+    const size_t es = (prec == BH_PREC_FP64 || fr) ? 8 : 4;
     const size_t vs = static_cast<size_t>(D) * Bp;
     return (static_cast<size_t>(slots) + LZ_LMAX + 1) * vs * es + vs * 8 + static_cast<size_t>(D) * 8 +
            static_cast<size_t>(LZ_GRID_MAX) * (2 * LZ_NVMAX + 1 + 2 * LZ_LMAX) * Bp * 8 +
-           (static_cast<size_t>(slots) + 2) * Bp * 24 + static_cast<size_t>(3 * LZ_LMAX + 1) * Bp * 8;
+           (static_cast<size_t>(slots) + 2) * Bp * 24 + static_cast<size_t>(3 * LZ_LMAX + 1) * Bp * 8 +
+           (fr ? static_cast<size_t>(slots) * Bp * 8 : 0);
+}
+
+// Options of the FP64 rerun of a rejected mixed solve: full reorthogonalisation whatever the sector size,
+// over a thick-restarted basis of at most 160 vectors (the rerun is the safety net: it has to be dependable
+// on clusters of close levels, where selective orthogonalisation can let copies of levels through)
+bh_lanczos_opts bh_fallback_opts(const bh_lanczos_opts& o) {
+    bh_lanczos_opts f = o;
+    f.prec = BH_PREC_FP64;
+    f.fullre_d = std::numeric_limits<int>::max();
+    f.ncv = o.ncv > 0 ? std::min(o.ncv, 160) : 160;
+    return f;
 }
 
 bool bh_lane_accepted(int status, const double* ev, const double* ee, int k, double accept_rel) {
     const double acc = accept_rel > 0 ? accept_rel : 1e-9;
     if (status != BH_OK || k < 1 || !std::isfinite(ev[0]) || !(ee[0] <= acc * std::fmax(std::fabs(ev[0]), 1e-300)))
         return false;
-    // levels above E0: ||r||^2/gap within LZ_ACCEPT_LEVEL of the level's scale (its magnitude, at least
-    // 1e-3 of the largest wanted magnitude, so that levels near zero are not held to an absolute 0)
-    double emax = 0.0;
-    for (int i = 0; i < k; ++i)
-        if (std::isfinite(ev[i])) emax = std::fmax(emax, std::fabs(ev[i]));
+    // levels above E0: ||r||^2/gap within LZ_ACCEPT_LEVEL of the level's scale, its magnitude or the
+    // width of the wanted window E_top - E0 if larger (levels near zero are not held to an absolute 0)
+    double top = ev[0];
+    for (int i = 1; i < k; ++i)
+        if (std::isfinite(ev[i])) top = std::fmax(top, ev[i]);
+    const double width = top - ev[0];
     for (int i = 1; i < k; ++i) {
         if (!std::isfinite(ev[i])) continue;
-        if (!(ee[i] <= LZ_ACCEPT_LEVEL * std::fmax(std::fabs(ev[i]), 1e-3 * emax))) return false;
+        if (!(ee[i] <= LZ_ACCEPT_LEVEL * std::fmax(std::fabs(ev[i]), width))) return false;
     }
     return true;
 }
@@ -1524,7 +1653,7 @@ static int lanczos_batch(bh_ctx* ctx, bh_ham* h, int prec, const double* J, cons
         }
         if (res->iters) res->iters[b] = out[b].iters;
         if (res->status) res->status[b] = out[b].status;
-        if (res->used_fp64) res->used_fp64[b] = prec == BH_PREC_FP64 ? 1 : 0;
+        if (res->used_fp64) res->used_fp64[b] = e->fp64() ? 1 : 0;
         if (vecs && !out[b].vecs.empty())
             std::copy(out[b].vecs.begin(), out[b].vecs.end(), vecs->begin() + static_cast<int64_t>(b) * k * h->D);
     }
@@ -1547,6 +1676,7 @@ void bh_lanczos_default_opts(bh_lanczos_opts* o) {
     o->fallback = 1;
     o->want_vectors = 0;
     o->ncv = 0;
+    o->fullre_d = 0;
 }
 
 namespace {
@@ -1623,7 +1753,7 @@ int solve_subset(bh_ctx* ctx, bh_ham* h, const std::vector<int>& runs, const dou
             if (res->errest) res->errest[b * k + q] = ee[i * k + q];
         }
         if (res->iters) res->iters[b] = (add_iters ? res->iters[b] : 0) + it[i];
-        if (res->used_fp64) res->used_fp64[b] = prec == BH_PREC_FP64 ? 1 : 0;
+        if (res->used_fp64) res->used_fp64[b] = uf[i];
         if (res->status) res->status[b] = st[i];
         if (vecs && !v2.empty())
             std::copy(v2.begin() + static_cast<int64_t>(i) * k * h->D, v2.begin() + static_cast<int64_t>(i + 1) * k * h->D,
@@ -1662,8 +1792,9 @@ int bh_lanczos(bh_ctx* ctx, bh_ham* h, int B, const double* J, const double* U, 
         const double acc = opts->accept_rel > 0 ? opts->accept_rel : 1e-9;
         std::vector<int> bad;
         for (int b : lz_runs)
-            if (!bh_lane_accepted(st[b], &ev[b * k], &ee[b * k], k, acc)) bad.push_back(b);
-        BH_TRY(solve_subset(ctx, h, bad, J, U, *opts, BH_PREC_FP64, &R, vp, true));
+            if (!uf[b] && !bh_lane_accepted(st[b], &ev[b * k], &ee[b * k], k, acc)) bad.push_back(b);
+        const bh_lanczos_opts ofb = bh_fallback_opts(*opts);
+        BH_TRY(solve_subset(ctx, h, bad, J, U, ofb, BH_PREC_FP64, &R, vp, true));
     }
     for (int b = 0; b < B; ++b) {
         for (int q = 0; q < k; ++q) {

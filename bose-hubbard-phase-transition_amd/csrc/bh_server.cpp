@@ -163,8 +163,10 @@ int ensure_engine(bh_server* S, Sector& sc, int p) {
     if (sc.eng[p]) return BH_OK;
     BH_TRY(ensure_ham(S, sc));
     const int prec = p == 0 ? BH_PREC_MIXED : BH_PREC_FP64;
+    // the FP64 engine serves the reruns of rejected mixed solves: full reorthogonalisation
+    const bh_lanczos_opts lo = p == 0 ? S->opts.lanczos : bh_fallback_opts(S->opts.lanczos);
     const int L = p == 0 ? lanes_for(S, sc.ham->D, prec) : std::min(4, lanes_for(S, sc.ham->D, prec));
-    const size_t need = bh_engine_bytes(sc.ham->D, prec, L, S->opts.lanczos) +
+    const size_t need = bh_engine_bytes(sc.ham->D, prec, L, lo) +
                         (prec == BH_PREC_FP64 ? static_cast<size_t>(sc.ham->nnz) * 12 : 0);
     evict_for(S, need, &sc);
     if (server_used(S) + need > S->opts.hbm_budget) {
@@ -173,7 +175,7 @@ int ensure_engine(bh_server* S, Sector& sc, int p) {
     }
     if (!sc.stream[p]) BH_HIP(S->ctx, hipStreamCreateWithFlags(&sc.stream[p], hipStreamNonBlocking));
     int rc = BH_OK;
-    sc.eng[p] = bh_make_engine(S->ctx, sc.ham, prec, L, S->opts.lanczos, sc.stream[p], false, &rc);
+    sc.eng[p] = bh_make_engine(S->ctx, sc.ham, prec, L, lo, sc.stream[p], false, &rc);
     if (!sc.eng[p]) return rc;
     sc.eng_bytes[p] = sc.eng[p]->device_bytes();
     sc.lane_req[p].assign(sc.eng[p]->lanes(), Pending{});
@@ -279,7 +281,8 @@ void worker_loop(bh_server* S) {
                 Pending pd = sc.lane_req[p][lane];
                 sc.lane_busy[p][lane] = 0;
                 const LaneResult& L = res[i];
-                const bool ok = bh_lane_accepted(L.status, L.evals.data(), L.errest.data(), k, S->opts.lanczos.accept_rel);
+                const bool ok = sc.eng[p]->fp64() ||
+                                bh_lane_accepted(L.status, L.evals.data(), L.errest.data(), k, S->opts.lanczos.accept_rel);
                 if (p == 0 && !ok && S->opts.lanczos.fallback >= 0) {
                     pd.prec = BH_PREC_FP64;
                     sc.queue[1].push_back(pd);  // rerun in FP64
@@ -290,7 +293,7 @@ void worker_loop(bh_server* S) {
                     ev[q] = L.evals[q] - pd.req.mu * pd.req.n;
                     rs[q] = L.resid[q];
                 }
-                complete(S, pd, L.status, ev.data(), rs.data(), k, L.iters, p, sc.lane_start[p][lane]);
+                complete(S, pd, L.status, ev.data(), rs.data(), k, L.iters, sc.eng[p]->fp64() ? 1 : 0, sc.lane_start[p][lane]);
             }
             sc.last_use = S->now();
         }

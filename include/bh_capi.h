@@ -155,6 +155,8 @@ typedef struct bh_lanczos_opts {
     int want_vectors;  /* 1: keep the k Ritz vectors (bh_lanczos_copy_vectors) */
     int ncv;           /* Krylov vectors kept per solve (Spectra's ncv); a solve that fills them is thick-
                           restarted with its lowest Ritz vectors (0: min(D, 512); at least 2k + 8) */
+    int fullre_d;      /* sectors with D <= fullre_d run FP64 Lanczos with full reorthogonalisation whatever
+                          prec asks (0: 16384; -1: never); used_fp64 reports it */
 } bh_lanczos_opts;
 
 typedef struct bh_lanczos_result {
@@ -162,7 +164,8 @@ typedef struct bh_lanczos_result {
     double* resid;   /* [B][k] FP64 residual norms ||H y - theta y|| (may be NULL) */
     double* errest;  /* [B][k] a posteriori eigenvalue error bound ||r||^2 / gap (may be NULL) */
     int* iters;      /* [B] Lanczos steps taken (may be NULL) */
-    int* used_fp64;  /* [B] 1 when the FP64 fallback produced the result (may be NULL) */
+    int* used_fp64;  /* [B] 1 when an FP64 engine produced the result: asked for, a small sector
+                        (fullre_d), or the fallback of a mixed solve (may be NULL) */
     int* status;     /* [B] per-solve BH_OK / BH_ENOCONV (may be NULL) */
 } bh_lanczos_result;
 
