@@ -26,9 +26,12 @@ struct bh_ctx {
     static constexpr int NWS = 8;
     void* ws[NWS] = {};
     size_t ws_bytes[NWS] = {};
-    // start vector cached in workspace slot 3 for (r0_D, r0_seed)
+    // start vector cached in workspace slot 3 for (r0_D, r0_seed), with the inverse norm of its copy in the
+    // storage precision r0_mixed (1: FP32)
     int64_t r0_D = -1;
     uint64_t r0_seed = 0;
+    int r0_mixed = -1;
+    double r0_scale = 0.0;
     // Ritz vectors kept by the last bh_lanczos(want_vectors=1): [B][k][D] doubles on the host
     std::vector<double> last_vecs;
     int last_vecs_B = 0, last_vecs_k = 0;

@@ -437,32 +437,38 @@ __global__ void __launch_bounds__(256) k_lz_restart(XT* V, XT* __restrict__ Xc, 
     Xc[idx] = static_cast<XT>(last);
 }
 
-// Ritz vectors: out[i][r][b] = sum_{t < nj[b]} V[t][r][b] * C[(t * nv + i) * B + b]   (C includes scale_t).
-// Each run stops at its own step count: Krylov slots past it were never written for that run.  The
-// coefficients of a run of slots are staged in LDS once per workgroup (they are shared by all rows), and
-// each thread carries LZ_RZ_R elements, so that the Krylov basis is streamed once with several
-// independent loads in flight.
+// Ritz vectors: out[i][r][b] = sum_{t < nj[b]} V[t][r][b] * C[(t * nv + i) * B + b]   (C includes scale_t),
+// for the group of outputs i0 <= i < i0 + ng (ng <= LZ_RZ_G; the host loops over groups).  Each run stops
+// at its own step count: Krylov slots past it were never written for that run.  The coefficients of a run
+// of slots are staged in LDS once per workgroup (they are shared by all rows), and each thread carries
+// LZ_RZ_R elements, so that the Krylov basis is streamed with several independent loads in flight; the
+// accumulators have compile-time indices only, so they stay in registers.
 constexpr int LZ_RZ_R = 4;       // elements per thread (strided by the workgroup size)
+constexpr int LZ_RZ_G = 8;       // outputs per launch
 constexpr int LZ_RZ_LDS = 4096;  // coefficients staged per pass (32 KB)
 
 template <typename XT, typename OT, int B>
 __global__ void __launch_bounds__(256) k_lz_ritz(const XT* __restrict__ V, int64_t vs, const int* __restrict__ njb, int njmax,
-                                                 const double* __restrict__ C, int nv, OT* __restrict__ out, int64_t os) {
+                                                 const double* __restrict__ C, int nv, int i0, int ng, OT* __restrict__ out,
+                                                 int64_t os) {
     __shared__ double cs[LZ_RZ_LDS];
     const int64_t base = static_cast<int64_t>(blockIdx.x) * (256 * LZ_RZ_R) + threadIdx.x;
     const int b = static_cast<int>(base % B);  // the same lane for all of this thread's elements (256 % B == 0)
     const int nj = njb[b];
-    double acc[LZ_RZ_R][LZ_NVMAX];
+    double acc[LZ_RZ_R][LZ_RZ_G];
 #pragma unroll
     for (int u = 0; u < LZ_RZ_R; ++u)
 #pragma unroll
-        for (int i = 0; i < LZ_NVMAX; ++i) acc[u][i] = 0.0;
-    const int per_t = nv * B;
+        for (int i = 0; i < LZ_RZ_G; ++i) acc[u][i] = 0.0;
+    const int per_t = LZ_RZ_G * B;  // staged coefficients per slot: the group's outputs for every lane
     const int tc = max(1, LZ_RZ_LDS / per_t);
     for (int t0 = 0; t0 < njmax; t0 += tc) {
         const int t1 = min(t0 + tc, njmax);
         __syncthreads();
-        for (int q = threadIdx.x; q < (t1 - t0) * per_t; q += 256) cs[q] = C[static_cast<int64_t>(t0) * per_t + q];
+        for (int q = threadIdx.x; q < (t1 - t0) * per_t; q += 256) {
+            const int tt = q / per_t, rem = q % per_t, i = rem / B, bb = rem % B;
+            cs[q] = i < ng ? C[(static_cast<int64_t>(t0 + tt) * nv + i0 + i) * B + bb] : 0.0;
+        }
         __syncthreads();
         const int te = min(t1, nj);
         for (int t = t0; t < te; ++t) {
@@ -474,8 +480,7 @@ __global__ void __launch_bounds__(256) k_lz_ritz(const XT* __restrict__ V, int64
             }
             const double* ct = cs + (t - t0) * per_t + b;
 #pragma unroll
-            for (int i = 0; i < LZ_NVMAX; ++i) {
-                if (i >= nv) break;
+            for (int i = 0; i < LZ_RZ_G; ++i) {
                 const double c = ct[i * B];
 #pragma unroll
                 for (int u = 0; u < LZ_RZ_R; ++u) acc[u][i] += v[u] * c;
@@ -485,10 +490,9 @@ __global__ void __launch_bounds__(256) k_lz_ritz(const XT* __restrict__ V, int64
 #pragma unroll
     for (int u = 0; u < LZ_RZ_R; ++u) {
         const int64_t idx = base + u * 256;
-        if (idx >= vs) continue;
 #pragma unroll
-        for (int i = 0; i < LZ_NVMAX; ++i)
-            if (i < nv) out[static_cast<int64_t>(i) * os + idx] = static_cast<OT>(acc[u][i]);
+        for (int i = 0; i < LZ_RZ_G; ++i)
+            if (i < ng && idx < vs) out[static_cast<int64_t>(i0 + i) * os + idx] = static_cast<OT>(acc[u][i]);
     }
 }
 
@@ -642,6 +646,7 @@ std::vector<double> lz_start_vector(int64_t D, uint64_t seed) {
 struct RunHost {
     std::vector<double> alpha, beta;     // alpha[t], beta[t] (beta[0] = 0)
     std::vector<double> lock_theta;      // locked Ritz values
+    std::vector<double> lock_bound;      // their bounds when they were (last) formed
     bool done = false;
     int iters = 0;
     int status = BH_ENOCONV;
@@ -778,22 +783,27 @@ class LzEngine final : public LzEngineBase {
         st.part = part.p;
         BH_HIP(ctx, hipMemsetAsync(st_d.p, 0, nd * sizeof(double), s));
         BH_HIP(ctx, hipMemsetAsync(st_i.p, 0, st_i.bytes(), s));
-        // start vector (cached in the context workspace per (D, seed) when the workspace is used)
-        std::vector<double> r = lz_start_vector(D, o.seed);
-        if (!ws || ctx->r0_D != D || ctx->r0_seed != o.seed) {
+        // start vector and the inverse norm of its stored copy (cached in the context workspace per
+        // (D, seed, storage precision) when the workspace is used)
+        if (!ws || ctx->r0_D != D || ctx->r0_seed != o.seed || ctx->r0_mixed != static_cast<int>(MIXED)) {
+            std::vector<double> r = lz_start_vector(D, o.seed);
             BH_HIP(ctx, hipMemcpyAsync(r0, r.data(), D * 8, hipMemcpyHostToDevice, s));
+            double nrm = 0.0;
+            for (int64_t i = 0; i < D; ++i) {
+                const double v = static_cast<double>(static_cast<XT>(r[i]));
+                nrm += v * v;
+            }
+            scale0 = nrm > 0 ? 1.0 / std::sqrt(nrm) : 0.0;
             BH_HIP(ctx, hipStreamSynchronize(s));
             if (ws) {
                 ctx->r0_D = D;
                 ctx->r0_seed = o.seed;
+                ctx->r0_mixed = static_cast<int>(MIXED);
+                ctx->r0_scale = scale0;
             }
+        } else {
+            scale0 = ctx->r0_scale;
         }
-        double nrm = 0.0;
-        for (int64_t i = 0; i < D; ++i) {
-            const double v = static_cast<double>(static_cast<XT>(r[i]));
-            nrm += v * v;
-        }
-        scale0 = nrm > 0 ? 1.0 / std::sqrt(nrm) : 0.0;
         a = LzArgs<XT>{D,
                        h->row_ptr,
                        h->words,
@@ -916,7 +926,7 @@ class LzEngine final : public LzEngineBase {
         BH_HIP(ctx, hipMemcpyAsync(ha.data(), st.alpha, ha.size() * 8, hipMemcpyDeviceToHost, s));
         BH_HIP(ctx, hipMemcpyAsync(hb.data(), st.beta, hb.size() * 8, hipMemcpyDeviceToHost, s));
         BH_HIP(ctx, hipStreamSynchronize(s));
-        std::vector<std::vector<int>> new_locks(B);
+        std::vector<std::vector<int>> new_locks(B), lock_slot(B);  // Ritz index, Y slot (-1: append)
         std::vector<Analysis> ans(B);
         std::vector<int> done_now(B, 0), restart_now(B, 0);
         auto analyse_run = [&](int b) {
@@ -1060,6 +1070,23 @@ class LzEngine final : public LzEngineBase {
                     if (std::fabs(an.theta[q] - an.theta[i]) <= lock_thr) have = true;
                 if (have) continue;
                 new_locks[b].push_back(i);
+                lock_slot[b].push_back(-1);
+            }
+            // Refresh: a locked vector is off by about bound/gap from its eigenvector.  When a close level
+            // surfaces later, the Krylov space kept orthogonal to the old vector rebuilds a second copy of
+            // the locked level from that remainder; a locked level whose Ritz bound has since dropped a
+            // thousandfold gets its vector re-formed from the current Ritz vector.
+            for (int l = 0; l < static_cast<int>(R.lock_theta.size()); ++l) {
+                int best = -1;
+                for (int i = 0; i < static_cast<int>(an.theta.size()); ++i)
+                    if (std::fabs(an.theta[i] - R.lock_theta[l]) <= lock_thr && (best < 0 || an.bound[i] < an.bound[best]))
+                        best = i;
+                if (best < 0 || !(an.bound[best] <= 1e-3 * R.lock_bound[l])) continue;
+                bool taken = false;
+                for (int q : new_locks[b]) taken = taken || q == best;
+                if (taken) continue;
+                new_locks[b].push_back(best);
+                lock_slot[b].push_back(l);
             }
         };
         {
@@ -1089,7 +1116,7 @@ class LzEngine final : public LzEngineBase {
             if (!new_locks[b].empty()) any_lock = true;
             if (restart_now[b]) any_restart = true;
         }
-        if (any_lock) BH_TRY(lock(new_locks, ans));
+        if (any_lock) BH_TRY(lock(new_locks, lock_slot, ans));
         if (any_restart) BH_TRY(restart(restart_now));
         if (any_done) {
             BH_HIP(ctx, hipMemcpyAsync(st.active, act.data(), B * 4, hipMemcpyHostToDevice, s));
@@ -1376,8 +1403,9 @@ class LzEngine final : public LzEngineBase {
         {
             KTimer kt(ctx, "lanczos_ritz", s);
             const int64_t per_wg = 256 * LZ_RZ_R;
-            hipLaunchKernelGGL((k_lz_ritz<XT, OT, B>), dim3(static_cast<unsigned>((vs + per_wg - 1) / per_wg)), dim3(256), 0, s,
-                               V, vs, dnj.p, njmax, dC.p, nv, out, os);
+            for (int i0 = 0; i0 < nv; i0 += LZ_RZ_G)
+                hipLaunchKernelGGL((k_lz_ritz<XT, OT, B>), dim3(static_cast<unsigned>((vs + per_wg - 1) / per_wg)), dim3(256), 0,
+                                   s, V, vs, dnj.p, njmax, dC.p, nv, i0, std::min(LZ_RZ_G, nv - i0), out, os);
         }
         BH_HIP(ctx, hipGetLastError());
         BH_HIP(ctx, hipStreamSynchronize(s));
@@ -1456,7 +1484,9 @@ class LzEngine final : public LzEngineBase {
         BH_HIP(ctx, hipStreamSynchronize(s));
         return BH_OK;
     }
-    int lock(const std::vector<std::vector<int>>& new_locks, const std::vector<Analysis>& ans) {
+    // forms the Ritz vectors new_locks[b] and stores them in Y: appended (lock_slot -1) or replacing slot
+    int lock(const std::vector<std::vector<int>>& new_locks, const std::vector<std::vector<int>>& lock_slot,
+             const std::vector<Analysis>& ans) {
         int nv = 0, jmax = 0;
         for (int b = 0; b < B; ++b)
             if (!new_locks[b].empty()) {
@@ -1499,19 +1529,29 @@ class LzEngine final : public LzEngineBase {
         BH_HIP(ctx, hipMemcpyAsync(anv.data(), st.anorm, anv.size() * 8, hipMemcpyDeviceToHost, s));
         BH_HIP(ctx, hipStreamSynchronize(s));
         for (int b = 0; b < B; ++b) {
+            int appended = 0;
             for (int i = 0; i < static_cast<int>(new_locks[b].size()) && i < nv; ++i) {
-                const int slot = nlock[b] + i;
+                const int target = lock_slot[b][i];
+                if (target < 0 && nlock[b] + appended >= LZ_LMAX) continue;
+                const int slot = target >= 0 ? target : nlock[b] + appended++;
                 BH_HIP(ctx, hipMemcpy2DAsync(Y + static_cast<int64_t>(slot) * vs + b, B * sizeof(XT),
                                              tmp.p + static_cast<int64_t>(i) * vs + b, B * sizeof(XT), sizeof(XT),
                                              static_cast<size_t>(D), hipMemcpyDeviceToDevice, s));
                 const double th = ans[b].theta[new_locks[b][i]];
-                rh[b].lock_theta.push_back(th);
+                const double bd = ans[b].bound[new_locks[b][i]];
+                if (target < 0) {
+                    rh[b].lock_theta.push_back(th);
+                    rh[b].lock_bound.push_back(bd);
+                } else {
+                    rh[b].lock_theta[slot] = th;
+                    rh[b].lock_bound[slot] = bd;
+                }
                 lth[static_cast<size_t>(slot) * B + b] = th;
                 tau[static_cast<size_t>(slot) * B + b] = eps_store;
                 tau[static_cast<size_t>(LZ_LMAX + slot) * B + b] = eps_store;
             }
             if (!new_locks[b].empty()) anv[b] = ans[b].tnorm;
-            nlock[b] += std::min(static_cast<int>(new_locks[b].size()), nv);
+            nlock[b] += appended;
             lmax_used = std::max(lmax_used, nlock[b]);
         }
         const int so2[2] = {1, 0};  // orthogonalise the next two Lanczos vectors against the locked set

@@ -1,0 +1,366 @@
+// bh_host.cpp -- the reference's host classes over the C ABI (see bh_host.hpp).
+#include "bh_host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <limits>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <tuple>
+
+#include "../csrc/bh_tridiag.hpp"
+
+namespace bhx {
+
+Settings& settings() {
+    static Settings s = [] {
+        Settings t;
+        if (const char* e = std::getenv("BHX_DEVICE")) t.device = std::atoi(e);
+        return t;
+    }();
+    return s;
+}
+
+namespace {
+struct CtxHolder {
+    bh_ctx* ctx = nullptr;
+    ~CtxHolder() {
+        if (ctx) bh_ctx_destroy(ctx);
+    }
+};
+CtxHolder& holder() {
+    static CtxHolder h;
+    return h;
+}
+std::mutex& mu() {
+    static std::mutex m;
+    return m;
+}
+}  // namespace
+
+bh_ctx* context() {
+    std::lock_guard<std::mutex> lk(mu());
+    CtxHolder& h = holder();
+    if (!h.ctx) {
+        const int rc = bh_ctx_create(settings().device, 0, &h.ctx);
+        if (rc != BH_OK) throw std::runtime_error("bhx: cannot create the device context (status " + std::to_string(rc) + ")");
+    }
+    return h.ctx;
+}
+
+void check(int rc, const char* what) {
+    if (rc == BH_OK) return;
+    const char* msg = holder().ctx ? bh_last_error(holder().ctx) : "";
+    throw std::runtime_error(std::string(what) + ": " + (msg ? msg : "") + " (status " + std::to_string(rc) + ")");
+}
+
+Sector::~Sector() {
+    if (ham) bh_ham_destroy(ham);
+    if (basis) bh_basis_destroy(basis);
+}
+
+std::shared_ptr<Sector> sector(const std::vector<std::vector<int>>& neighbours, int m, int n) {
+    using Key = std::tuple<std::vector<int>, std::vector<int>, int, int, unsigned>;
+    static std::map<Key, std::weak_ptr<Sector>> cache;
+    static std::mutex cm;
+    if (static_cast<int>(neighbours.size()) != m) throw std::invalid_argument("neighbour lists must have one entry per site");
+    std::vector<int> off(1, 0), sites;
+    for (const auto& l : neighbours) {
+        for (int s : l) sites.push_back(s);
+        off.push_back(static_cast<int>(sites.size()));
+    }
+    const Key key{off, sites, m, n, settings().flags};
+    std::lock_guard<std::mutex> lk(cm);
+    auto it = cache.find(key);
+    if (it != cache.end())
+        if (auto sp = it->second.lock()) return sp;
+    auto s = std::make_shared<Sector>();
+    s->m = m;
+    s->n = n;
+    s->off = off;
+    s->sites = sites;
+    bh_ctx* ctx = context();
+    check(bh_basis_build(ctx, m, n, BH_ORDER_REFERENCE, &s->basis), "basis");
+    check(bh_hamiltonian_build(ctx, s->basis, s->off.data(), s->sites.empty() ? nullptr : s->sites.data(),
+                               settings().flags, &s->ham),
+          "hamiltonian");
+    cache[key] = s;
+    return s;
+}
+
+}  // namespace bhx
+
+// ---- Neighbours --------------------------------------------------------------------------------------
+
+Neighbours::Neighbours(int m_) : m(m_) {}
+Neighbours::~Neighbours() = default;
+
+static std::vector<std::vector<int>> lists_of(int kind, int Lx, int Ly, bool closed) {
+    const int m = Lx * Ly;
+    std::vector<int> off(static_cast<size_t>(m) + 1), sites(static_cast<size_t>(4 * m + 4));
+    const int ne = bh_lattice_neighbours(kind, Lx, Ly, closed ? 1 : 0, off.data(), sites.data());
+    if (ne < 0) throw std::invalid_argument("invalid lattice");
+    std::vector<std::vector<int>> out(static_cast<size_t>(m));
+    for (int i = 0; i < m; ++i) out[i].assign(sites.begin() + off[i], sites.begin() + off[i + 1]);
+    return out;
+}
+
+void Neighbours::chain_neighbours(bool closed) {
+    neighbours = m >= 1 ? lists_of(0, m, 1, closed) : std::vector<std::vector<int>>();
+}
+
+void Neighbours::square_neighbours(bool closed) {
+    const int L = static_cast<int>(std::lround(std::sqrt(static_cast<double>(m))));
+    if (L * L != m) throw std::invalid_argument("The number of sites must be a perfect square.");
+    neighbours = lists_of(1, L, L, closed);
+}
+
+std::vector<std::vector<int>> Neighbours::getNeighbours() const { return neighbours; }
+
+// ---- BH ------------------------------------------------------------------------------------------------
+
+namespace BH {
+
+int64_t SparseMatrix::rows() const { return sec ? bh_ham_dim(sec->ham) : 0; }
+
+int64_t SparseMatrix::nonZeros() const {
+    if (!sec) return 0;
+    int64_t z = 0;
+    if (J != 0.0) z += bh_ham_nnz(sec->ham);
+    if (U != 0.0 || Mu != 0.0) z += rows();
+    return z;
+}
+
+SparseMatrix SparseMatrix::operator+(const SparseMatrix& o) const {
+    if (!sec) return o;
+    if (!o.sec) return *this;
+    if (sec != o.sec) throw std::invalid_argument("sparse matrices of different sectors cannot be added");
+    return SparseMatrix(sec, J + o.J, U + o.U, Mu + o.Mu);
+}
+
+void SparseMatrix::to_csr(std::vector<int64_t>& row_ptr, std::vector<int32_t>& col, std::vector<double>& val) const {
+    const int64_t D = rows();
+    const int64_t nt = sec ? bh_ham_nnz(sec->ham) : 0;
+    std::vector<int64_t> rp(static_cast<size_t>(D) + 1, 0);
+    std::vector<int32_t> c(static_cast<size_t>(std::max<int64_t>(nt, 1)));
+    std::vector<double> v(c.size()), d(static_cast<size_t>(D));
+    if (sec) bhx::check(bh_ham_copy_csr(sec->ham, rp.data(), c.data(), v.data(), d.data()), "copy_csr");
+    row_ptr.assign(static_cast<size_t>(D) + 1, 0);
+    col.clear();
+    val.clear();
+    const double nshift = sec ? Mu * sec->n : 0.0;
+    for (int64_t r = 0; r < D; ++r) {
+        const double dv = U * d[r] + nshift;
+        bool placed = dv == 0.0;
+        for (int64_t q = (J != 0.0 ? rp[r] : rp[r + 1]); q < rp[r + 1]; ++q) {
+            if (!placed && c[q] > r) {
+                col.push_back(static_cast<int32_t>(r));
+                val.push_back(dv);
+                placed = true;
+            }
+            col.push_back(c[q]);
+            val.push_back(J * v[q]);
+        }
+        if (!placed) {
+            col.push_back(static_cast<int32_t>(r));
+            val.push_back(dv);
+        }
+        row_ptr[r + 1] = static_cast<int64_t>(col.size());
+    }
+}
+
+std::pair<std::vector<double>, bhx::MatrixXd> fixed_set_basis(int m, int n) {
+    bh_basis* b = nullptr;
+    bhx::check(bh_basis_build(bhx::context(), m, n, BH_ORDER_REFERENCE, &b), "fixed_set_basis");
+    const int64_t D = bh_basis_dim(b);
+    std::vector<uint8_t> st(static_cast<size_t>(D * m));
+    std::vector<double> tags(static_cast<size_t>(D));
+    int rc = bh_basis_copy_states(b, st.data());
+    if (rc == BH_OK) rc = bh_basis_copy_tags(b, tags.data());
+    bh_basis_destroy(b);
+    bhx::check(rc, "fixed_set_basis");
+    bhx::MatrixXd basis(m, D);
+    for (int64_t k = 0; k < D; ++k)
+        for (int i = 0; i < m; ++i) basis(i, k) = st[static_cast<size_t>(k * m + i)];
+    return {tags, basis};
+}
+
+SparseMatrix fixed_bosons_hamiltonian(const std::vector<std::vector<int>>& neighbours, const bhx::MatrixXd& basis,
+                                      const std::vector<double>& tags, int m, int n, double J, double U, double mu) {
+    auto s = bhx::sector(neighbours, m, n);
+    const int64_t D = bh_ham_dim(s->ham);
+    if (basis.rows() != m || basis.cols() != D || static_cast<int64_t>(tags.size()) != D)
+        throw std::invalid_argument("basis and tags do not belong to this (m, n) sector");
+    const double eps = std::numeric_limits<double>::epsilon();
+    if (std::abs(J) > eps) return SparseMatrix(s, J, 0.0, 0.0);
+    if (std::abs(U) > eps) return SparseMatrix(s, 0.0, U, 0.0);
+    if (std::abs(mu) > eps) return SparseMatrix(s, 0.0, 0.0, -mu);  // -mu N
+    std::cerr << "Error: At least one of the parameters J, U, mu must be different from zero." << std::endl;
+    return SparseMatrix(s, 0.0, 0.0, 0.0);
+}
+
+}  // namespace BH
+
+// ---- Op --------------------------------------------------------------------------------------------------
+
+namespace Op {
+
+std::vector<std::complex<double>> IRLM_eigen(const BH::SparseMatrix& O, int nb_eigen, bhx::MatrixXcd& eigenvectors) {
+    if (!O.sec) throw std::invalid_argument("empty operator");
+    if (nb_eigen < 1) throw std::invalid_argument("nb_eigen must be positive");
+    const int k = std::min(nb_eigen, 16);
+    bh_lanczos_opts o;
+    bh_lanczos_default_opts(&o);
+    o.k = k;
+    o.prec = bhx::settings().prec;
+    o.want_vectors = 1;
+    std::vector<double> ev(k), rs(k), ee(k);
+    int it = 0, uf = 0, st = 0;
+    bh_lanczos_result r{ev.data(), rs.data(), ee.data(), &it, &uf, &st};
+    bh_ctx* ctx = bhx::context();
+    const int rc = bh_lanczos(ctx, O.sec->ham, 1, &O.J, &O.U, &o, &r);
+    if (rc != BH_OK || st != BH_OK) throw std::runtime_error("Eigenvalue computation failed.");
+    const int64_t D = O.rows();
+    std::vector<double> vec(static_cast<size_t>(k * D));
+    bhx::check(bh_lanczos_copy_vectors(ctx, 1, k, vec.data()), "IRLM_eigen vectors");
+    std::vector<std::complex<double>> out;
+    eigenvectors.resize(D, k);
+    for (int i = 0; i < k; ++i) {
+        out.emplace_back(ev[i] + O.Mu * O.sec->n, 0.0);
+        for (int64_t q = 0; q < D; ++q) eigenvectors(q, i) = vec[static_cast<size_t>(i * D + q)];
+    }
+    return out;
+}
+
+std::vector<double> exact_eigen(const BH::SparseMatrix& O, bhx::MatrixXd& eigenvectors) {
+    const int64_t D = O.rows();
+    if (D > 1500) throw std::invalid_argument("exact_eigen: dense host solve limited to D <= 1500");
+    std::vector<int64_t> rp;
+    std::vector<int32_t> c;
+    std::vector<double> v;
+    O.to_csr(rp, c, v);
+    const int n = static_cast<int>(D);
+    std::vector<double> A(static_cast<size_t>(n) * n, 0.0), w(n), V(static_cast<size_t>(n) * n);
+    for (int r = 0; r < n; ++r)
+        for (int64_t q = rp[r]; q < rp[r + 1]; ++q) A[static_cast<size_t>(r) * n + c[q]] += v[q];
+    bhtri::jacobi_eig(n, A.data(), w.data(), V.data());
+    eigenvectors.resize(D, D);
+    for (int r = 0; r < n; ++r)
+        for (int cc = 0; cc < n; ++cc) eigenvectors(r, cc) = V[static_cast<size_t>(r) * n + cc];
+    return w;
+}
+
+}  // namespace Op
+
+// ---- Analysis ----------------------------------------------------------------------------------------------
+
+namespace Analysis {
+
+std::vector<double> gap_ratios(const std::vector<std::complex<double>>& eigenvalues, int nb_eigen) {
+    const int ne = std::min<int>(nb_eigen, static_cast<int>(eigenvalues.size()));
+    std::vector<double> e(ne);
+    for (int i = 0; i < ne; ++i) e[i] = eigenvalues[i].real();
+    std::sort(e.begin(), e.end());
+    std::vector<double> r;
+    for (int i = 1; i + 1 < ne; ++i) {
+        const double a = e[i] - e[i - 1], b = e[i + 1] - e[i];
+        const double mn = std::min(a, b), mx = std::max(a, b);
+        r.push_back(mx != 0 ? mn / mx : 0.0);
+    }
+    return r;
+}
+
+void exact_parameters(int m, int n, double J, double U, double mu, double s, double r, std::string fixed_param,
+                      const std::string& out) {
+    const double eps = std::numeric_limits<double>::epsilon();
+    if (std::abs(J) < eps && std::abs(U) < eps && std::abs(mu) < eps) {
+        std::cerr << "Error: At least one of the parameters J, U, mu must be different from zero." << std::endl;
+        return;
+    }
+    Neighbours nb(m);
+    nb.chain_neighbours();
+    const auto nei = nb.getNeighbours();
+    auto [tags, basis] = BH::fixed_set_basis(m, n);
+    BH::SparseMatrix JH = BH::fixed_bosons_hamiltonian(nei, basis, tags, m, n, 1, 0, 0);
+    BH::SparseMatrix UH = BH::fixed_bosons_hamiltonian(nei, basis, tags, m, n, 0, 1, 0);
+    BH::SparseMatrix uH = BH::fixed_bosons_hamiltonian(nei, basis, tags, m, n, 0, 0, 1);
+    // parameter ranges and their wiring as src/analysis.cpp:241-256 has them: the first grid axis always
+    // runs over [J, J + r]; the second over [mu, mu + r] (fixed J or fixed mu) or [U, U + r] (fixed U)
+    const double J_min = J, J_max = J + r, mu_min = mu, mu_max = mu + r, U_min = U, U_max = U + r;
+    BH::SparseMatrix Hf, H1, H2;
+    double fixed_value, p1_min, p1_max, p2_min, p2_max;
+    if (fixed_param == "J") {
+        Hf = JH * J, H1 = UH, H2 = uH, fixed_value = J;
+        p1_min = J_min, p1_max = J_max, p2_min = mu_min, p2_max = mu_max;
+    } else if (fixed_param == "U") {
+        Hf = UH * U, H1 = JH, H2 = uH, fixed_value = U;
+        p1_min = J_min, p1_max = J_max, p2_min = U_min, p2_max = U_max;
+    } else {
+        Hf = uH * mu, H1 = JH, H2 = UH, fixed_value = mu;
+        p1_min = J_min, p1_max = J_max, p2_min = mu_min, p2_max = mu_max;
+    }
+    const int n1 = static_cast<int>((p1_max - p1_min) / s) + 1, n2 = static_cast<int>((p2_max - p2_min) / s) + 1;
+    const int npt = n1 * n2;
+    std::vector<double> p1v(npt), p2v(npt), gr(npt);
+    std::vector<BH::SparseMatrix> H(npt);
+    for (int i = 0; i < n1; ++i)
+        for (int j = 0; j < n2; ++j) {
+            const int idx = i * n1 + j;  // the reference's row index (src/analysis.cpp:349); n1 == n2 here
+            p1v[idx] = p1_min + i * s;
+            p2v[idx] = p2_min + j * s;
+            H[idx] = Hf + H1 * p1v[idx] + H2 * p2v[idx];
+        }
+    // distinct (J, U) pairs are solved once, batched 64 per call; N = n I only shifts the levels
+    std::map<std::pair<double, double>, std::vector<double>> lev;
+    for (const auto& h : H) lev[{h.J, h.U}];
+    int nb_eigen = 20;
+    auto sec = JH.sec;
+    for (;;) {
+        const int k = std::min(nb_eigen, 16);
+        std::vector<std::pair<double, double>> keys;
+        for (auto& kv : lev) keys.push_back(kv.first);
+        for (size_t q0 = 0; q0 < keys.size(); q0 += 64) {
+            const int B = static_cast<int>(std::min<size_t>(64, keys.size() - q0));
+            std::vector<double> Jb(B), Ub(B), ev(static_cast<size_t>(B) * k);
+            std::vector<int> st(B);
+            for (int b = 0; b < B; ++b) {
+                Jb[b] = keys[q0 + b].first;
+                Ub[b] = keys[q0 + b].second;
+            }
+            bh_lanczos_opts o;
+            bh_lanczos_default_opts(&o);
+            o.k = k;
+            o.prec = bhx::settings().prec;
+            bh_lanczos_result res{ev.data(), nullptr, nullptr, nullptr, nullptr, st.data()};
+            const int rc = bh_lanczos(bhx::context(), sec->ham, B, Jb.data(), Ub.data(), &o, &res);
+            if (rc != BH_OK && rc != BH_ENOCONV) throw std::runtime_error("Eigenvalue computation failed.");
+            for (int b = 0; b < B; ++b) lev[keys[q0 + b]].assign(ev.begin() + static_cast<int64_t>(b) * k, ev.begin() + static_cast<int64_t>(b + 1) * k);
+        }
+        for (int idx = 0; idx < npt; ++idx) {
+            const auto& L = lev[{H[idx].J, H[idx].U}];
+            std::vector<std::complex<double>> e;
+            for (double x : L)
+                if (std::isfinite(x)) e.emplace_back(x + H[idx].Mu * n, 0.0);
+            const auto rr = gap_ratios(e, static_cast<int>(e.size()));
+            gr[idx] = rr.empty() ? 0.0 : std::accumulate(rr.begin(), rr.end(), 0.0) / rr.size();
+        }
+        // the reference recomputes with 5 more eigenvalues while the gap ratios barely vary
+        // (src/analysis.cpp:364-381); here the level count stops at 16
+        const double mean = std::accumulate(gr.begin(), gr.end(), 0.0) / npt;
+        double var = 0.0;
+        for (double g : gr) var += (g - mean) * (g - mean);
+        var /= npt;
+        if (var > 1e-8 * mean || k == 16) break;
+        nb_eigen += 5;
+    }
+    std::ofstream f(out);
+    f << fixed_param << " " << fixed_value << std::endl;
+    for (int idx = 0; idx < npt; ++idx) f << p1v[idx] << " " << p2v[idx] << " " << gr[idx] << " nan nan" << std::endl;
+}
+
+}  // namespace Analysis

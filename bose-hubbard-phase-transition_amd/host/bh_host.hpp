@@ -1,0 +1,145 @@
+// bh_host.hpp -- C++ host surface of the exact-diagonalisation path with the reference's names.
+//
+// Mirrors the classes and namespaces a caller of the reference uses on this path:
+//   class Neighbours               include/neighbours.hpp:12-62   (src/neighbours.cpp:21-71, 126-128)
+//   BH::fixed_set_basis            include/hamiltonian.hpp:64     (src/hamiltonian.cpp:143-149)
+//   BH::fixed_bosons_hamiltonian   include/hamiltonian.hpp:87     (src/hamiltonian.cpp:238-256)
+//   Op::IRLM_eigen, Op::exact_eigen include/operator.hpp:34, 57   (src/operator.cpp:22-33, 92-101)
+//   Analysis::gap_ratios, Analysis::exact_parameters  include/analysis.hpp (src/analysis.cpp:207-387,433-454)
+// over the C ABI of include/bh_capi.h.  Eigen types are replaced by the small dense containers below
+// (column-major, as Eigen's defaults).  A sparse matrix is held as a linear combination
+// cJ T + cU V + cMu N of one sector's device-resident terms (T hopping, V interaction, N = n I), so that
+// the reference's composition H_fixed + H1 * p1 + H2 * p2 (src/analysis.cpp:311) keeps its meaning and
+// every solve runs on the device.  Errors are thrown as the reference throws them: std::runtime_error
+// from the eigensolvers (src/operator.cpp:28), std::invalid_argument from Neighbours
+// (src/neighbours.cpp:43,80).
+#pragma once
+
+#include <complex>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/bh_capi.h"
+
+namespace bhx {
+
+// dense column-major matrix
+template <typename T>
+struct Matrix {
+    int64_t nrows = 0, ncols = 0;
+    std::vector<T> v;
+    Matrix() = default;
+    Matrix(int64_t r, int64_t c) : nrows(r), ncols(c), v(static_cast<size_t>(r * c), T()) {}
+    int64_t rows() const { return nrows; }
+    int64_t cols() const { return ncols; }
+    T& operator()(int64_t i, int64_t j) { return v[static_cast<size_t>(i + j * nrows)]; }
+    const T& operator()(int64_t i, int64_t j) const { return v[static_cast<size_t>(i + j * nrows)]; }
+    T coeff(int64_t i, int64_t j) const { return (*this)(i, j); }
+    void resize(int64_t r, int64_t c) {
+        nrows = r;
+        ncols = c;
+        v.assign(static_cast<size_t>(r * c), T());
+    }
+};
+using MatrixXd = Matrix<double>;
+using MatrixXcd = Matrix<std::complex<double>>;
+
+// Process-wide runtime of the shims: one device context (device BHX_DEVICE, default 0), the lattice and
+// convention used for the sectors the shims build.
+struct Settings {
+    int device = 0;
+    unsigned flags = 0;      // BH_CONV_* (0: the physical conventions)
+    int prec = BH_PREC_MIXED;
+};
+Settings& settings();
+bh_ctx* context();
+void check(int rc, const char* what);  // throws std::runtime_error with the library's message
+
+// one sector: (neighbour lists, m, n) with its basis in the reference order and its Hamiltonian terms
+struct Sector {
+    int m = 0, n = 0;
+    std::vector<int> off, sites;
+    bh_basis* basis = nullptr;
+    bh_ham* ham = nullptr;
+    ~Sector();
+};
+std::shared_ptr<Sector> sector(const std::vector<std::vector<int>>& neighbours, int m, int n);
+
+}  // namespace bhx
+
+class Neighbours {
+   public:
+    explicit Neighbours(int m);
+    ~Neighbours();
+    // chain of m sites: i -> [i-1 if i > 0, i+1 if i < m-1], then the periodic pair (src/neighbours.cpp:21-34)
+    void chain_neighbours(bool closed = true);
+    // L x L square lattice (m = L^2), left/right/top/bottom with wrap when closed; the reference writes
+    // these lists to a shadowing local (src/neighbours.cpp:45) and returns them empty; here they are filled
+    void square_neighbours(bool closed = true);
+    std::vector<std::vector<int>> getNeighbours() const;
+
+   private:
+    int m;
+    std::vector<std::vector<int>> neighbours;
+};
+
+namespace BH {
+
+// Sparse operator cJ T + cU V + cMu N on one sector (D x D, reference basis order)
+class SparseMatrix {
+   public:
+    SparseMatrix() = default;
+    SparseMatrix(std::shared_ptr<bhx::Sector> s, double cJ, double cU, double cMu) : sec(std::move(s)), J(cJ), U(cU), Mu(cMu) {}
+    int64_t rows() const;
+    int64_t cols() const { return rows(); }
+    int64_t nonZeros() const;  // stored entries of the materialised matrix (off-diagonal plus diagonal)
+    SparseMatrix operator+(const SparseMatrix& o) const;
+    SparseMatrix operator*(double f) const { return SparseMatrix(sec, J * f, U * f, Mu * f); }
+    // CSR materialisation on the host (row_ptr D+1, col, val), rows in the reference basis order
+    void to_csr(std::vector<int64_t>& row_ptr, std::vector<int32_t>& col, std::vector<double>& val) const;
+    std::shared_ptr<bhx::Sector> sec;
+    double J = 0.0, U = 0.0, Mu = 0.0;
+};
+inline SparseMatrix operator*(double f, const SparseMatrix& a) { return a * f; }
+
+// (tags, basis): tags[k] = sum_i n_i ln p_i and column k of the m x D basis, in the reference order
+std::pair<std::vector<double>, bhx::MatrixXd> fixed_set_basis(int m, int n);
+
+// The reference's term selection (src/hamiltonian.cpp:242-255): J != 0 gives the hopping term alone,
+// else U != 0 the interaction term alone, else mu != 0 the chemical term alone, else a message on stderr
+// and the zero matrix.  basis and tags must be the pair fixed_set_basis(m, n) returned.
+SparseMatrix fixed_bosons_hamiltonian(const std::vector<std::vector<int>>& neighbours, const bhx::MatrixXd& basis,
+                                      const std::vector<double>& tags, int m, int n, double J, double U, double mu);
+
+}  // namespace BH
+
+namespace Op {
+
+// Lowest nb_eigen eigenvalues (distinct levels, ascending; at most 16) and eigenvectors (D x nb_eigen)
+// of O by the device Lanczos solver; std::runtime_error("Eigenvalue computation failed.") when it does
+// not converge (src/operator.cpp:26-29).
+std::vector<std::complex<double>> IRLM_eigen(const BH::SparseMatrix& O, int nb_eigen, bhx::MatrixXcd& eigenvectors);
+
+// All eigenvalues (ascending) and eigenvectors of O by a dense host solve (D <= 1500).
+std::vector<double> exact_eigen(const BH::SparseMatrix& O, bhx::MatrixXd& eigenvectors);
+
+}  // namespace Op
+
+namespace Analysis {
+
+// r_i = min(delta_i, delta_{i+1}) / max(delta_i, delta_{i+1}) over the sorted real parts, 0 when the
+// larger gap is 0 (src/analysis.cpp:433-454)
+std::vector<double> gap_ratios(const std::vector<std::complex<double>>& eigenvalues, int nb_eigen);
+
+// The exact sweep of the reference CLI (src/analysis.cpp:207-387): periodic chain of m sites, n bosons,
+// a grid over two parameters with the third fixed, mean gap ratio per grid point, written to `out`
+// (default phase.txt) as "fixed_param fixed_value" then "p1 p2 gap_ratio condensate_fraction coherence"
+// per point.  The single-particle density matrix is not computed here: the last two columns are nan.
+void exact_parameters(int m, int n, double J, double U, double mu, double s, double r, std::string fixed_param,
+                      const std::string& out = "phase.txt");
+
+}  // namespace Analysis
