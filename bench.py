@@ -54,6 +54,54 @@ def cpu_baseline(m, n, J, U, k, gpu_steps, sample_steps=80):
                       f"GPU solve"}
 
 
+def serving_sectors(dmax=5_000_000):
+    """The admissible sectors of the serving mix: chain m in [6, 14] periodic, square 3x3, 3x4, 4x4 periodic, with
+    n in [m-2, m+2] and dimension at most 5e6 (chain 14 and 4x4 have none)."""
+    from math import comb
+
+    out = []
+    geoms = [("chain", m, 0, 0) for m in range(6, 15)] + [("square", 0, 3, 3), ("square", 0, 3, 4), ("square", 0, 4, 4)]
+    for g, m, Lx, Ly in geoms:
+        mm = m if g == "chain" else Lx * Ly
+        for n in range(mm - 2, mm + 3):
+            if n >= 1 and comb(mm + n - 1, n) <= dmax:
+                out.append((g, m, Lx, Ly, n))
+    return out
+
+
+def serving(ctx, nreq, rate, seed=7):
+    """A Poisson stream of `nreq` requests at `rate` req/s offered load: each a seed-drawn admissible sector,
+    J/U log-uniform in [0.01, 1], mu/U uniform in [0, 3], k = 4, mixed precision with FP64 rerun.  Returns
+    p50/p99 latency (submit to result) and completed solves per second over the stream's span."""
+    rng = np.random.default_rng(seed)
+    secs = serving_sectors()
+    with bhx.Server(ctx, lanes=16, k=4, ncv=160) as srv:
+        t0 = time.perf_counter()
+        tnext = t0
+        ids = []
+        for _ in range(nreq):
+            g, m, Lx, Ly, n = secs[rng.integers(len(secs))]
+            J = float(10 ** rng.uniform(-2, 0))
+            mu = float(rng.uniform(0, 3))
+            tnext += rng.exponential(1.0 / rate)
+            dt = tnext - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(srv.submit(g, n, J, 1.0, mu, m=m, Lx=Lx, Ly=Ly))
+        ok = srv.drain(600.0)
+        span = time.perf_counter() - t0
+        st = srv.stats()
+        res = [srv.poll(i) for i in ids]
+    bad = sum(1 for r in res if r is None or r.status != 0)
+    reruns = sum(1 for r in res if r is not None and r.used_fp64)
+    lat = np.array([r.t_done - r.t_submit for r in res if r is not None])
+    return {"requests": nreq, "offered_rps": rate, "completed": int(st.completed), "failed": bad, "drained": bool(ok),
+            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "solves_per_s": float(st.completed / span), "fp64_results": reruns, "sectors": len(secs),
+            "mix": "chain m=6..13, square 3x3 and 3x4, n in [m-2, m+2], D <= 5e6; J/U log-uniform [0.01, 1], "
+                   "mu/U uniform [0, 3], k=4"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,6 +111,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy kernel of BYTES bytes (calibrates HBM counters) and exit")
+    ap.add_argument("--serve", type=int, default=120, metavar="N",
+                    help="requests of the serving stream at 200 req/s offered load (0: skip)")
     ap.add_argument("--quick", action="store_true",
                     help="time the mixed batch only (no plain FP64 comparator, SpMV table or CPU leg): for profilers")
     args = ap.parse_args()
@@ -140,6 +190,11 @@ def main():
             spmv[f"{name}_B{Bs}"] = {"us": ms * 1e3, "alg_GBs": by / ms / 1e6}
     log("spmv:", json.dumps(spmv))
 
+    serve = None
+    if args.serve > 0 and not args.quick:
+        serve = serving(ctx, args.serve, 200.0)
+        log("serving:", json.dumps(serve))
+
     cpu = None
     if not args.no_cpu and not args.quick:
         cpu = cpu_baseline(m, n, J0, U, k, int(r64.iters[0]))
@@ -164,6 +219,7 @@ def main():
         "speedup_vs_plain_fp64": value * fp64_s,
         "rel_err_E0_vs_fp64": rel0,
         "spmv": spmv,
+        "serving": serve,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": cpu,

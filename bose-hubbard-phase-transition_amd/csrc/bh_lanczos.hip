@@ -1072,11 +1072,12 @@ class LzEngine final : public LzEngineBase {
                 new_locks[b].push_back(i);
                 lock_slot[b].push_back(-1);
             }
-            // Refresh: a locked vector is off by about bound/gap from its eigenvector.  When a close level
-            // surfaces later, the Krylov space kept orthogonal to the old vector rebuilds a second copy of
-            // the locked level from that remainder; a locked level whose Ritz bound has since dropped a
-            // thousandfold gets its vector re-formed from the current Ritz vector.
-            for (int l = 0; l < static_cast<int>(R.lock_theta.size()); ++l) {
+            // Refresh (FP64 storage): a locked vector is off by about bound/gap from its eigenvector.  When a
+            // close level surfaces later, the Krylov space kept orthogonal to the old vector rebuilds a second
+            // copy of the locked level from that remainder; a locked level whose Ritz bound has since dropped a
+            // thousandfold gets its vector re-formed from the current Ritz vector.  FP32-stored runs lock far
+            // earlier (sqrt(eps32) ||T||) and measured slower with refreshes, so they keep their first vector.
+            for (int l = 0; !MIXED && l < static_cast<int>(R.lock_theta.size()); ++l) {
                 int best = -1;
                 for (int i = 0; i < static_cast<int>(an.theta.size()); ++i)
                     if (std::fabs(an.theta[i] - R.lock_theta[l]) <= lock_thr && (best < 0 || an.bound[i] < an.bound[best]))

@@ -27,11 +27,11 @@ Settings& settings() {
 }
 
 namespace {
+// The context is not destroyed by a static destructor: at process exit the HIP runtime may already be torn
+// down (its own exit handlers are registered after this object is built), and the process's device memory is
+// released with the process anyway.  bhx::shutdown() releases it explicitly.
 struct CtxHolder {
     bh_ctx* ctx = nullptr;
-    ~CtxHolder() {
-        if (ctx) bh_ctx_destroy(ctx);
-    }
 };
 CtxHolder& holder() {
     static CtxHolder h;
@@ -51,6 +51,13 @@ bh_ctx* context() {
         if (rc != BH_OK) throw std::runtime_error("bhx: cannot create the device context (status " + std::to_string(rc) + ")");
     }
     return h.ctx;
+}
+
+void shutdown() {
+    std::lock_guard<std::mutex> lk(mu());
+    CtxHolder& h = holder();
+    if (h.ctx) bh_ctx_destroy(h.ctx);
+    h.ctx = nullptr;
 }
 
 void check(int rc, const char* what) {
@@ -156,7 +163,7 @@ void SparseMatrix::to_csr(std::vector<int64_t>& row_ptr, std::vector<int32_t>& c
     const double nshift = sec ? Mu * sec->n : 0.0;
     for (int64_t r = 0; r < D; ++r) {
         const double dv = U * d[r] + nshift;
-        bool placed = dv == 0.0;
+        bool placed = U == 0.0 && Mu == 0.0;  // the diagonal is stored whenever a diagonal term is present
         for (int64_t q = (J != 0.0 ? rp[r] : rp[r + 1]); q < rp[r + 1]; ++q) {
             if (!placed && c[q] > r) {
                 col.push_back(static_cast<int32_t>(r));
@@ -308,9 +315,12 @@ void exact_parameters(int m, int n, double J, double U, double mu, double s, dou
     const int npt = n1 * n2;
     std::vector<double> p1v(npt), p2v(npt), gr(npt);
     std::vector<BH::SparseMatrix> H(npt);
+    // Point counts are truncated as the reference truncates them (src/analysis.cpp:281-282), so rounding can
+    // give the two axes different counts.  Rows are i-major; the reference's index i * n1 + j
+    // (src/analysis.cpp:349) leaves its arrays whenever n2 > n1, and is not reproduced.
     for (int i = 0; i < n1; ++i)
         for (int j = 0; j < n2; ++j) {
-            const int idx = i * n1 + j;  // the reference's row index (src/analysis.cpp:349); n1 == n2 here
+            const int idx = i * n2 + j;
             p1v[idx] = p1_min + i * s;
             p2v[idx] = p2_min + j * s;
             H[idx] = Hf + H1 * p1v[idx] + H2 * p2v[idx];

@@ -57,6 +57,7 @@ struct Settings {
 };
 Settings& settings();
 bh_ctx* context();
+void shutdown();  // releases the context (sectors still held by callers must be released first)
 void check(int rc, const char* what);  // throws std::runtime_error with the library's message
 
 // one sector: (neighbour lists, m, n) with its basis in the reference order and its Hamiltonian terms

@@ -93,8 +93,11 @@ def test_cli_phase_file(tmp_path, fixed):
     head = lines[0].split()
     assert head[0] == fixed and float(head[1]) == {"J": J, "U": U, "u": mu}[fixed]
     rows = [list(map(float, ln.split())) for ln in lines[1:] if ln.strip()]
-    npts = int(r / s) + 1
-    assert len(rows) == npts * npts
+    # point counts truncated as the reference computes them (src/analysis.cpp:281-282)
+    lo2 = {"J": mu, "U": U, "u": mu}[fixed]
+    n1 = int(((J + r) - J) / s) + 1
+    n2 = int(((lo2 + r) - lo2) / s) + 1
+    assert len(rows) == n1 * n2
     off, sites = O.chain_neighbours(m, True)
     h = O.hamiltonian(m, n, off, sites)
     for row in rows:
