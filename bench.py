@@ -54,6 +54,21 @@ def cpu_baseline(m, n, J, U, k, gpu_steps, sample_steps=80):
                       f"GPU solve"}
 
 
+PROFILE = os.path.join(ROOT, "profiles", "k1_lanczos_mixed_b16_chain12.txt")
+
+
+def traffic_from_profile():
+    """HBM bytes per K1 launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE passes on this workload (the
+    committed summary holds the command, the copy-kernel calibration and the figure); None without it."""
+    try:
+        for line in open(PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def serving_sectors(dmax=5_000_000):
     """The admissible sectors of the serving mix: chain m in [6, 14] periodic, square 3x3, 3x4, 4x4 periodic, with
     n in [m-2, m+2] and dimension at most 5e6 (chain 14 and 4x4 have none)."""
@@ -189,6 +204,28 @@ def main():
             ms, by = ham.spmv_bench(prec, Bs, 30)
             spmv[f"{name}_B{Bs}"] = {"us": ms * 1e3, "alg_GBs": by / ms / 1e6}
     log("spmv:", json.dumps(spmv))
+    # SpMV at the 4x4 periodic square lattice, n=10 (D=3,268,760): its vectors and matrix exceed the 256 MB
+    # Infinity Cache, so this is the HBM-roofline SpMV figure.  Besides this build's own algorithmic bytes,
+    # the fraction is also given against the bytes of an FP32-valued CSR (the unit of the workload
+    # definition: nnz (4 + 4) + 4 (D + 1) + 4 D + 2 * 4 D).
+    spmv3 = {}
+    if not args.quick:
+        ham.close()
+        basis.close()
+        off3, sites3 = bhx.lattice("square", Lx=4, Ly=4, periodic=True)
+        b3 = ctx.basis(16, 10, bhx.ORDER_LEX)
+        h3 = b3.hamiltonian(off3, sites3)
+        csr32 = 8.0 * h3.nnz + 4.0 * (h3.D + 1) + 4.0 * h3.D + 8.0 * h3.D
+        csr64 = 12.0 * h3.nnz + 4.0 * (h3.D + 1) + 8.0 * h3.D + 16.0 * h3.D
+        for prec, name, ref_bytes in ((bhx.PREC_MIXED, "mixed", csr32), (bhx.PREC_FP64, "fp64", csr64)):
+            ms, by = h3.spmv_bench(prec, 1, 30)
+            spmv3[f"{name}_B1"] = {"us": ms * 1e3, "alg_GBs": by / ms / 1e6, "frac_of_peak": by / ms / 1e6 / HBM_PEAK_GBS,
+                                   "csr_equiv_GBs": ref_bytes / ms / 1e6,
+                                   "csr_equiv_frac": ref_bytes / ms / 1e6 / HBM_PEAK_GBS}
+        spmv3["D"], spmv3["nnz"] = h3.D, h3.nnz
+        log("spmv 4x4 n=10:", json.dumps(spmv3))
+        h3.close()
+        b3.close()
 
     serve = None
     if args.serve > 0 and not args.quick:
@@ -219,9 +256,12 @@ def main():
         "speedup_vs_plain_fp64": value * fp64_s,
         "rel_err_E0_vs_fp64": rel0,
         "spmv": spmv,
+        "spmv_square4x4_n10": spmv3 or None,
         "serving": serve,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile()},
+        "k1": {"kernel": "k_lz_a_rl (fused Lanczos operator pass)", "launches": k1_count, "avg_us": k1_ms * 1e3,
+               "alg_bytes_per_launch": k1_bytes, "k2_avg_us": upd_total / max(upd_count, 1) * 1e3},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -22,6 +22,12 @@ struct bh_ctx {
     std::recursive_mutex mu;
     int timing = 0;
     std::map<std::string, std::pair<int64_t, double>> kstats;
+    // kernel timings recorded but not yet read back (resolved in bulk, so timing adds no host waits)
+    struct PendingTimer {
+        const char* name;
+        hipEvent_t a, b;
+    };
+    std::vector<PendingTimer> ktimers;
     // grow-only device workspaces reused across calls (Krylov basis, locked vectors, scratch)
     static constexpr int NWS = 8;
     void* ws[NWS] = {};
@@ -94,6 +100,8 @@ struct DevArray {
 };
 
 // kernel timing (HIP events on the launch stream when ctx->timing is on)
+void bh_ktimers_flush(bh_ctx* ctx);  // waits for the recorded events and adds them to kstats
+
 struct KTimer {
     bh_ctx* ctx;
     const char* name;

@@ -43,6 +43,9 @@ constexpr int LZ_GRID_MAX = 1024;  // workgroups that write partial sums
 constexpr int LZ_LMAX = 16;        // locked Ritz vectors per solve
 constexpr int LZ_NVMAX = 16;       // Ritz vectors formed per pass
 constexpr int LZ_RED_WG = 1024;    // reduction workgroup: 16 waves, one value per wave
+constexpr int LZ_GRAPH_STEPS = 8;  // Lanczos steps per captured HIP graph
+constexpr int LZ_K1_TILES = 4;         // row groups per K1 workgroup on the row-lane engine
+constexpr int LZ_K1_GRID_MAX = 32768;  // K1 workgroups at most (rows of partial sums)
 constexpr double LZ_ACCEPT_LEVEL = 1e-6;  // mixed: bound on ||r||^2/gap of the levels above E0, relative
 
 struct LzState {
@@ -55,10 +58,13 @@ struct LzState {
     double* tau;     // [2][LZ_LMAX][B] estimates of |<y_l, v_j>| and |<y_l, v_{j-1}>| (locked y_l)
     double* ltheta;  // [LZ_LMAX][B] Ritz values of the locked vectors
     double* anorm;   // [B] estimate of ||H_b||
+    double* dso;     // [3][LZ_LMAX][B] <y_l, v_j> and <y_l, v_{j-1}> (normalised vectors) for the one-read
+                     //     steps, and the step's coefficients c_l
     int* active;     // [B]
     int* nlock;      // [B]
     int* step;       // [B] Lanczos step of each lane (lanes are admitted and retired independently)
-    int* so;         // [2] orthogonalise against the locked vectors in this step / did in the previous one
+    int* so;         // [2] orthogonalisation against the locked vectors in this step / the previous one:
+                     //     0 none, 1 exact (K1 forms the dot products), 2 one-read (from the carried products)
     double* part;    // [LZ_GRID_MAX][nv] per-workgroup partials
 };
 
@@ -96,6 +102,35 @@ __device__ __forceinline__ void lz_block_put(double v, int q, double* lds, doubl
         part[static_cast<size_t>(blockIdx.x) * nv + q * B + threadIdx.x] = s;
     }
     __syncthreads();
+}
+
+// Per-workgroup sums of NQ values per vector b: v[q] of every lane is summed over the lanes of its b and
+// stored at part[wg * nv + q * B + b].  Up to LZ_PUT_Q values share one LDS round (LZ_PUT_LDS doubles).
+constexpr int LZ_PUT_LDS = 4096;
+
+template <int B, int NQ>
+__device__ __forceinline__ void lz_block_put_multi(const double (&v)[NQ], double* lds, double* part, int nv) {
+    constexpr int QC = (LZ_PUT_LDS / (LZ_WG / 64 * B)) < 1 ? 1 : (LZ_PUT_LDS / (LZ_WG / 64 * B));
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q0 = 0; q0 < NQ; q0 += QC) {
+#pragma unroll
+        for (int q = q0; q < NQ && q < q0 + QC; ++q) {
+            double t = v[q];
+#pragma unroll
+            for (int o = B; o < 64; o <<= 1) t += __shfl_xor(t, o, 64);
+            if (lane < B) lds[(wid * QC + (q - q0)) * B + lane] = t;
+        }
+        __syncthreads();
+        const int nq = (NQ - q0) < QC ? (NQ - q0) : QC;
+        for (int i = threadIdx.x; i < nq * B; i += LZ_WG) {
+            const int q = i / B, bb = i % B;
+            double t = 0.0;
+            for (int w = 0; w < LZ_WG / 64; ++w) t += lds[(w * QC + q) * B + bb];
+            part[static_cast<size_t>(blockIdx.x) * nv + (q0 + q) * B + bb] = t;
+        }
+        __syncthreads();
+    }
 }
 
 // K1 epilogue state shared by both operator engines
@@ -138,7 +173,7 @@ template <typename XT, int B, int FMT, int LM>
 __device__ __forceinline__ K1Epi<XT, B, FMT, LM> k1_setup(const LzArgs<XT>& a) {
     const int b = threadIdx.x % B;  // fixed per thread: LZ_WG is a multiple of B
     const int j = a.st.step[b];
-    const bool so = LM > 0 && a.st.so[0] != 0;
+    const bool so = LM > 0 && a.st.so[0] == 1;
     K1Epi<XT, B, FMT, LM> e{a,
                             a.Xc,
                             a.V + static_cast<int64_t>(j > 0 ? j - 1 : 0) * a.vs,
@@ -203,11 +238,20 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_a_rl(LzArgs<XT> a, int64_t rows_pe
 }
 
 // K2: v_{j+1} = w - alpha v_j - sum_l c_l y_l (stored in slot j+1 and, replacing v_j, in the current-vector
-// buffer; each thread reads its own elements before writing them), partials of ||v_{j+1}||^2.  Block 0
-// records alpha_j.  The locked vectors enter only in the steps flagged for orthogonalisation.
-template <typename XT, int B>
-__global__ void __launch_bounds__(LZ_WG) k_lz_b(LzArgs<XT> a, int lm) {
-    __shared__ double lds[LZ_WG / 64 * B];
+// buffer; each thread reads its own elements before writing them), partials of ||v_{j+1}||^2 and of
+// e_l = <y_l, w - alpha v_j>, from which R2 forms <y_l, v_{j+1}> = (e_l - c_l) / beta_{j+1} (the locked vectors
+// are orthonormal; the cross terms c_m <y_l, y_m> are products of two small quantities).  Block 0 records
+// alpha_j and the c_l.  The coefficient c_l = <y_l, w> - alpha_j <y_l, v_j> comes from
+// K1's dot products in an exact step (so = 1) and, in a one-read step (so = 2), from the products K2 formed in
+// the two previous steps:  <y_l, w> = <H y_l, v_j> - beta_j <y_l, v_{j-1}> ~ theta_l <y_l, v_j> - beta_j
+// <y_l, v_{j-1}>, dropping <r_l, v_j> for the locked vector's residual r_l (in exact arithmetic r_l lies along
+// one earlier Lanczos vector, so the term is at the level of the basis' loss of orthogonality).  A one-read
+// step reads each locked vector once instead of twice (K1 skips them).
+template <typename XT, int B, int LM>
+__global__ void __launch_bounds__(LZ_WG) k_lz_b(LzArgs<XT> a) {
+    constexpr int LL = LM > 0 ? LM : 1;
+    constexpr int NV2 = (1 + LM) * B;
+    __shared__ double lds[LZ_PUT_LDS];
     const int b = threadIdx.x % B;
     const int j = a.st.step[b];
     XT* vj = a.Xc;
@@ -215,15 +259,33 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_b(LzArgs<XT> a, int lm) {
     const bool act = a.st.active[b] != 0 && j < a.maxit;
     const double sj = a.st.scale[j * B + b];
     const double al = a.st.sums[b];
-    const int nl = (act && lm > 0 && a.st.so[0] != 0) ? min(a.st.nlock[b], lm) : 0;
+    const int mode = a.st.so[0];
+    const int nl = (act && LM > 0 && mode != 0) ? min(a.st.nlock[b], LM) : 0;
     if (blockIdx.x == 0 && threadIdx.x < B && act) a.st.alpha[j * B + b] = al;
+    // acc[0]: ||v_{j+1}||^2;  acc[1 + l]: e_l = <y_l, w - alpha v_j>, from which the reduction forms
+    // <y_l, v_{j+1}> = (e_l - c_l) / beta_{j+1} (the locked vectors are orthonormal)
+    double cl[LL], acc[1 + LM];
+#pragma unroll
+    for (int q = 0; q <= LM; ++q) acc[q] = 0.0;
+#pragma unroll
+    for (int l = 0; l < LL; ++l) {
+        cl[l] = 0.0;
+        if (l < nl) {
+            if (mode == 1) {
+                cl[l] = a.st.sums[(1 + l) * B + b] - al * a.st.sums[(1 + LM + l) * B + b];
+            } else {
+                const double bj = j > 0 ? a.st.beta[j * B + b] : 0.0;
+                cl[l] = (a.st.ltheta[l * B + b] - al) * a.st.dso[l * B + b] - bj * a.st.dso[(LZ_LMAX + l) * B + b];
+            }
+        }
+        if (LM > 0 && blockIdx.x == 0 && threadIdx.x < B && act && l < LM) a.st.dso[(2 * LZ_LMAX + l) * B + b] = cl[l];
+    }
     constexpr int ROWS = LZ_WG / B;
-    constexpr int UN = 4;  // rows per thread in flight
+    constexpr int UN = LM > 4 ? 2 : 4;  // rows per thread in flight
     const int64_t ntile = (a.D + ROWS - 1) / ROWS;
     const int64_t per = (ntile + gridDim.x - 1) / gridDim.x;
     const int64_t L = bh_xcd_tile(blockIdx.x, gridDim.x);
     const int64_t t0 = min(L * per, ntile), t1 = min(t0 + per, ntile);
-    double pn = 0.0;
     if (act) {
         for (int64_t t = t0; t < t1; t += UN) {
             double w[UN], v[UN];
@@ -239,12 +301,20 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_b(LzArgs<XT> a, int lm) {
             }
 #pragma unroll
             for (int u = 0; u < UN; ++u) w[u] -= al * v[u] * sj;
-            for (int l = 0; l < nl; ++l) {
-                const XT* yl = a.Y + static_cast<int64_t>(l) * a.vs;
-                const double cl = a.st.sums[(1 + l) * B + b] - al * a.st.sums[(1 + lm + l) * B + b];
+            if constexpr (LM > 0) {
+                double w0[UN];
 #pragma unroll
-                for (int u = 0; u < UN; ++u)
-                    if (ok[u]) w[u] -= cl * to_d(yl[idx[u]]);
+                for (int u = 0; u < UN; ++u) w0[u] = w[u];
+#pragma unroll
+                for (int l = 0; l < LL; ++l) {
+                    if (l >= nl) continue;
+#pragma unroll
+                    for (int u = 0; u < UN; ++u) {
+                        const double y = ok[u] ? to_d(a.Y[static_cast<int64_t>(l) * a.vs + idx[u]]) : 0.0;
+                        acc[1 + l] += y * w0[u];
+                        w[u] -= cl[l] * y;
+                    }
+                }
             }
 #pragma unroll
             for (int u = 0; u < UN; ++u) {
@@ -253,11 +323,16 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_b(LzArgs<XT> a, int lm) {
                 vn[idx[u]] = wr;
                 vj[idx[u]] = wr;
                 const double wd = to_d(wr);
-                pn += wd * wd;
+                acc[0] += wd * wd;
             }
         }
     }
-    lz_block_put<B>(pn, 0, lds, a.st.part, B);
+    if (LM > 0 && mode != 0) {  // uniform
+        lz_block_put_multi<B, 1 + LM>(acc, lds, a.st.part, NV2);
+    } else {
+        double a0[1] = {acc[0]};
+        lz_block_put_multi<B, 1>(a0, lds, a.st.part, NV2);
+    }
 }
 
 // Full reorthogonalisation (small sectors), classical Gram-Schmidt pass over the whole basis: workgroup t
@@ -343,7 +418,9 @@ __global__ void __launch_bounds__(LZ_RED_WG) k_lz_reduce(LzState st, const doubl
     __shared__ int want[LZ_RED_WG / 64];
     const int so_cur = (mode == LZ_RED_PLAIN) ? 1 : st.so[0];
     const int so_prev = (mode == LZ_RED_BETA) ? st.so[1] : 0;
-    const int nvv = (mode == LZ_RED_K1 && !so_cur) ? nmin : nv;
+    // K1: the locked-vector sums exist only in exact steps.  BETA: the norms (v < B) and, when the step
+    // orthogonalised, the carried products <y_l, v_{j+1}> (v = B (1 + l) + b)
+    const int nvv = mode == LZ_RED_K1 ? (so_cur == 1 ? nv : nmin) : (mode == LZ_RED_BETA ? (so_cur ? nv : nmin) : nv);
     int any = 0;
     for (int v = blockIdx.x * waves + (threadIdx.x >> 6); v < nvv; v += gridDim.x * waves) {
         double s = 0.0;
@@ -352,6 +429,20 @@ __global__ void __launch_bounds__(LZ_RED_WG) k_lz_reduce(LzState st, const doubl
         for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
         if (mode != LZ_RED_BETA) {
             if (lane == 0) out[v] = s;
+            continue;
+        }
+        if (v >= B) {
+            // carried product of locked vector l of lane b, normalised by this step's ||v_{j+1}|| (the norm
+            // is summed again here in the same order, so it equals the value the v = b wave computes)
+            const int l = v / B - 1, bb = v % B;
+            double sn = 0.0;
+            for (int w = lane; w < nwg; w += 64) sn += part[static_cast<size_t>(w) * nv + bb];
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) sn += __shfl_xor(sn, o, 64);
+            if (lane == 0 && l < LZ_LMAX && sn > 0.0 && st.active[bb] && l < st.nlock[bb]) {
+                st.dso[(LZ_LMAX + l) * B + bb] = st.dso[l * B + bb];
+                st.dso[l * B + bb] = (s - st.dso[(2 * LZ_LMAX + l) * B + bb]) / std::sqrt(sn);
+            }
             continue;
         }
         // v = b; only lane 0 of this wave writes step[b] in the launch
@@ -387,8 +478,33 @@ __global__ void __launch_bounds__(LZ_RED_WG) k_lz_reduce(LzState st, const doubl
     if (threadIdx.x == 0) {
         int f = 0;
         for (int w = 0; w < waves; ++w) f |= want[w];
-        st.so[0] = (so_always || (so_cur && !so_prev)) ? 1 : f;
+        // an exact step that opened a pair is followed by a second exact step; then one-read steps (default)
+        // or, with the estimate-driven schedule, exact pairs when the estimate asks
+        st.so[0] = (so_cur == 1 && so_prev != 1) ? 1 : (so_always ? 2 : (f ? 1 : 0));
         st.so[1] = so_cur;
+    }
+}
+
+// R1 over many K1 workgroups: one workgroup per value (blockIdx.x = v), each thread summing a strided
+// subset of the partials in a fixed order, then a fixed tree over the workgroup.  Only the alpha sums
+// (v < nmin) unless the step is exact (then also the locked-vector sums).
+template <int B>
+__global__ void __launch_bounds__(LZ_RED_WG) k_lz_reduce_wide(LzState st, const double* __restrict__ part, int nwg,
+                                                              int nv, int nmin, double* __restrict__ out) {
+    __shared__ double red[LZ_RED_WG / 64];
+    const int v = blockIdx.x;
+    const int nvv = st.so[0] == 1 ? nv : nmin;
+    if (v >= nvv) return;  // uniform over the workgroup
+    double s = 0.0;
+    for (int w = threadIdx.x; w < nwg; w += LZ_RED_WG) s += part[static_cast<size_t>(w) * nv + v];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < LZ_RED_WG / 64; ++w) t += red[w];
+        out[v] = t;
     }
 }
 
@@ -724,6 +840,10 @@ class LzEngine final : public LzEngineBase {
         slots = lz_slots(D, o);
         cap = slots - 1;
         fullre = bh_lz_fullre(D, o);
+        {
+            const char* e = std::getenv("BHX_GRAPH");
+            use_graphs = !(e && std::strcmp(e, "0") == 0);
+        }
         chunk = o.chunk > 0 ? o.chunk : 16;
         tol = o.tol > 0 ? o.tol : 1e-10;
         dedup = o.dedup_rel > 0 ? o.dedup_rel : 1e-9;
@@ -758,11 +878,15 @@ class LzEngine final : public LzEngineBase {
         n_beta = static_cast<size_t>(slots + 1) * B;
         const size_t n_sums = static_cast<size_t>(1 + 2 * LZ_LMAX) * B;
         const size_t n_tau = static_cast<size_t>(2 * LZ_LMAX) * B, n_lth = static_cast<size_t>(LZ_LMAX) * B;
-        const size_t nd = n_alpha + 2 * n_beta + 2 * B + n_sums + n_tau + n_lth + B;
+        const size_t nd = n_alpha + 2 * n_beta + 2 * B + n_sums + n_tau + n_lth + B + 3 * n_lth;
         BH_TRY(dalloc(ctx, st_d, nd));
         BH_TRY(dalloc(ctx, st_i, static_cast<size_t>(3 * B + 2)));
         const int nvmax = std::max(static_cast<int>(n_sums), 2 * LZ_NVMAX * B);
-        BH_TRY(dalloc(ctx, part, static_cast<size_t>(LZ_GRID_MAX) * nvmax));
+        // K1 on the row-lane engine: small row tiles (LZ_K1_TILES row groups of LZ_WG / B rows), many
+        // workgroups in flight; its partial sums need a row per workgroup
+        rpw = std::max<int64_t>(static_cast<int64_t>(LZ_WG / B) * LZ_K1_TILES, (D + LZ_K1_GRID_MAX - 1) / LZ_K1_GRID_MAX);
+        grid_rl = static_cast<int>((D + rpw - 1) / rpw);
+        BH_TRY(dalloc(ctx, part, static_cast<size_t>(std::max(LZ_GRID_MAX, grid_rl)) * nvmax));
         BH_TRY(dalloc(ctx, dnorm, static_cast<size_t>(2 * LZ_NVMAX * B)));
         BH_TRY(dalloc(ctx, dnj, B));
         BH_TRY(dalloc(ctx, dno, B));
@@ -776,6 +900,7 @@ class LzEngine final : public LzEngineBase {
         st.tau = st.sums + n_sums;
         st.ltheta = st.tau + n_tau;
         st.anorm = st.ltheta + n_lth;
+        st.dso = st.anorm + B;
         st.active = st_i.p;
         st.nlock = st_i.p + B;
         st.step = st_i.p + 2 * B;
@@ -825,8 +950,6 @@ class LzEngine final : public LzEngineBase {
         if (engine == 1) BH_TRY(bh_ham_rowblocks(h, B, &rblk, &nrblk));
         grid_rb = static_cast<int>(std::min<int64_t>(std::max<int64_t>(nrblk, 1), LZ_GRID_MAX));
         shm_rb = shm + BH_STAGE_PRODUCTS * sizeof(double);
-        rpw = (D + LZ_GRID_MAX - 1) / LZ_GRID_MAX;
-        grid_rl = static_cast<int>((D + rpw - 1) / rpw);
         grid_k1 = engine == 1 ? grid_rb : grid_rl;
         const int64_t ntile_b = (D + LZ_WG / B - 1) / (LZ_WG / B);
         grid_k2 = static_cast<int>(std::min<int64_t>(ntile_b, LZ_GRID_MAX));
@@ -902,13 +1025,31 @@ class LzEngine final : public LzEngineBase {
         return BH_OK;
     }
 
+    ~LzEngine() override {
+        for (auto& g : graphs)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    }
+
+    // Steps are enqueued as replays of captured HIP graphs of LZ_GRAPH_STEPS and of 1 step when kernel
+    // timing is off (BHX_GRAPH=0 disables them): every quantity a step reads -- each lane's step counter, its
+    // scalars, the orthogonalisation flag -- lives in device memory, so one captured step is valid for any
+    // step of any lane.  With timing on, the launches are issued one by one so that each carries its events.
     int run(int steps) override {
         const int LM = lmax_used == 0 ? 0 : (lmax_used <= 4 ? 4 : 16);
-        for (int t = 0; t < steps; ++t) step(LM);
+        const int nsteps = steps;
+        if (use_graphs && !ctx->timing) {
+            while (steps >= LZ_GRAPH_STEPS) {
+                BH_TRY(replay(LM, LZ_GRAPH_STEPS));
+                steps -= LZ_GRAPH_STEPS;
+            }
+            for (; steps > 0; --steps) BH_TRY(replay(LM, 1));
+        } else {
+            for (int t = 0; t < steps; ++t) step(LM);
+        }
         BH_HIP(ctx, hipGetLastError());
         for (int b = 0; b < B; ++b)
             if (act[b]) {
-                const int d = std::min(steps, cap - jl[b]);
+                const int d = std::min(nsteps, cap - jl[b]);
                 jl[b] += d;
                 tot[b] += d;
             }
@@ -1272,6 +1413,32 @@ class LzEngine final : public LzEngineBase {
     }
 
    private:
+    struct Graph {
+        hipGraphExec_t exec = nullptr;
+        int lm = -1, steps = 0;
+    };
+    // launches the captured graph of `ns` steps for orthogonalisation variant LM, capturing it on first use
+    int replay(int LM, int ns) {
+        Graph* g = nullptr;
+        for (auto& q : graphs)
+            if (q.exec && q.lm == LM && q.steps == ns) g = &q;
+        if (!g) {
+            hipGraph_t graph = nullptr;
+            BH_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            for (int t = 0; t < ns; ++t) step(LM);
+            BH_HIP(ctx, hipStreamEndCapture(s, &graph));
+            Graph ng;
+            ng.lm = LM;
+            ng.steps = ns;
+            const hipError_t e = hipGraphInstantiate(&ng.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            BH_HIP(ctx, e);
+            graphs.push_back(ng);
+            g = &graphs.back();
+        }
+        BH_HIP(ctx, hipGraphLaunch(g->exec, s));
+        return BH_OK;
+    }
     // Rayleigh-Ritz over each lane's finished candidates (FP64 vectors Y, HY = H Y): with G = Y^T Y and
     // A = Y^T H Y, the pencil (A, G) is solved on the host after dropping the numerically dependent
     // directions of G (copies of one level that finite-precision Lanczos produces), and Y, HY are replaced
@@ -1369,14 +1536,13 @@ class LzEngine final : public LzEngineBase {
         {
             KTimer kt(ctx, "lanczos_reduce", s);
             const int nv = (1 + 2 * LM) * B;
-            const int waves = LZ_RED_WG / 64;
-            const int g = std::max(1, std::min((nv + waves - 1) / waves, 64));
-            hipLaunchKernelGGL((k_lz_reduce<B>), dim3(g), dim3(LZ_RED_WG), 0, s, st, part.p, grid_k1, nv, LZ_RED_K1, cap,
-                               st.sums, B, 0.0, 0);
+            hipLaunchKernelGGL((k_lz_reduce_wide<B>), dim3(nv), dim3(LZ_RED_WG), 0, s, st, part.p, grid_k1, nv, B, st.sums);
         }
         {
             KTimer kt(ctx, "lanczos_update", s);
-            hipLaunchKernelGGL((k_lz_b<XT, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, a, LM);
+            if (LM == 0) hipLaunchKernelGGL((k_lz_b<XT, B, 0>), dim3(grid_k2), dim3(LZ_WG), 0, s, a);
+            else if (LM == 4) hipLaunchKernelGGL((k_lz_b<XT, B, 4>), dim3(grid_k2), dim3(LZ_WG), 0, s, a);
+            else hipLaunchKernelGGL((k_lz_b<XT, B, 16>), dim3(grid_k2), dim3(LZ_WG), 0, s, a);
         }
         if (fullre) {
             // two classical Gram-Schmidt passes against every stored Lanczos vector of the lane
@@ -1388,8 +1554,8 @@ class LzEngine final : public LzEngineBase {
         }
         {
             KTimer kt(ctx, "lanczos_reduce", s);
-            hipLaunchKernelGGL((k_lz_reduce<B>), dim3(1), dim3(LZ_RED_WG), 0, s, st, part.p, grid_k2, B, LZ_RED_BETA, cap,
-                               nullptr, B, eps_store, lz_so_always());
+            hipLaunchKernelGGL((k_lz_reduce<B>), dim3(1), dim3(LZ_RED_WG), 0, s, st, part.p, grid_k2, (1 + LM) * B, LZ_RED_BETA,
+                               cap, nullptr, B, eps_store, lz_so_always());
         }
     }
     // C holds rows t < max(njb) of [(t * nv + i) * B + b]
@@ -1573,6 +1739,8 @@ class LzEngine final : public LzEngineBase {
     int64_t D = 0, vs = 0;
     int k = 4, maxsteps = 10000, slots = 2, cap = 1, chunk = 16;
     bool fullre = false;  // every step reorthogonalises against the whole basis (small sectors)
+    bool use_graphs = true;
+    std::vector<Graph> graphs;
     double tol = 1e-10, dedup = 1e-9, eps_store = 0, eps23 = 0, scale0 = 0;
     XT* V = nullptr;
     XT* Y = nullptr;
@@ -1635,8 +1803,10 @@ size_t bh_engine_bytes(int64_t D, int prec, int B, const bh_lanczos_opts& o) {
     const bool fr = bh_lz_fullre(D, o);
     const size_t es = (prec == BH_PREC_FP64 || fr) ? 8 : 4;
     const size_t vs = static_cast<size_t>(D) * Bp;
+    const int64_t rpw = std::max<int64_t>(static_cast<int64_t>(LZ_WG / Bp) * LZ_K1_TILES, (D + LZ_K1_GRID_MAX - 1) / LZ_K1_GRID_MAX);
+    const size_t part_rows = static_cast<size_t>(std::max<int64_t>(LZ_GRID_MAX, (D + rpw - 1) / rpw));
     return (static_cast<size_t>(slots) + LZ_LMAX + 1) * vs * es + vs * 8 + static_cast<size_t>(D) * 8 +
-           static_cast<size_t>(LZ_GRID_MAX) * (2 * LZ_NVMAX + 1 + 2 * LZ_LMAX) * Bp * 8 +
+           part_rows * (2 * LZ_NVMAX + 1 + 2 * LZ_LMAX) * Bp * 8 +
            (static_cast<size_t>(slots) + 2) * Bp * 24 + static_cast<size_t>(3 * LZ_LMAX + 1) * Bp * 8 +
            (fr ? static_cast<size_t>(slots) * Bp * 8 : 0);
 }

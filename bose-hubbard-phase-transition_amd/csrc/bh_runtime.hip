@@ -61,19 +61,29 @@ KTimer::KTimer(bh_ctx* c, const char* n, hipStream_t st) : ctx(c), name(n), s(st
 }
 
 KTimer::~KTimer() {
-    if (a && b) {
-        (void)hipEventRecord(b, s);
-        if (hipEventSynchronize(b) == hipSuccess) {
+    if (a && b && hipEventRecord(b, s) == hipSuccess) {
+        ctx->ktimers.push_back({name, a, b});
+        if (ctx->ktimers.size() >= 4096) bh_ktimers_flush(ctx);
+        return;
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+}
+
+void bh_ktimers_flush(bh_ctx* ctx) {
+    for (const auto& t : ctx->ktimers) {
+        if (hipEventSynchronize(t.b) == hipSuccess) {
             float ms = 0.f;
-            if (hipEventElapsedTime(&ms, a, b) == hipSuccess) {
-                auto& st = ctx->kstats[name];
+            if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+                auto& st = ctx->kstats[t.name];
                 st.first += 1;
                 st.second += ms;
             }
         }
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
     }
-    if (a) (void)hipEventDestroy(a);
-    if (b) (void)hipEventDestroy(b);
+    ctx->ktimers.clear();
 }
 
 int64_t bh_binom_host(int n, int k) {
@@ -147,6 +157,7 @@ int bh_ctx_destroy(bh_ctx* ctx) {
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
+        bh_ktimers_flush(ctx);
         for (int i = 0; i < bh_ctx::NWS; ++i) bh_dev_free(ctx, ctx->ws[i], ctx->ws_bytes[i]);
         (void)hipStreamDestroy(ctx->stream);
     }
@@ -173,6 +184,7 @@ int bh_ctx_set_timing(bh_ctx* ctx, int on) {
 int bh_ctx_kernel_stats(bh_ctx* ctx, const char* name, int64_t* count, double* total_ms, int reset) {
     if (!ctx || !name) return BH_EINVAL;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    bh_ktimers_flush(ctx);
     auto it = ctx->kstats.find(name);
     if (count) *count = it == ctx->kstats.end() ? 0 : it->second.first;
     if (total_ms) *total_ms = it == ctx->kstats.end() ? 0.0 : it->second.second;

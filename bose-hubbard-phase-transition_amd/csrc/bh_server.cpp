@@ -159,13 +159,31 @@ int ensure_ham(bh_server* S, Sector& sc) {
     return BH_OK;
 }
 
+// Lanes of a sector's engine: enough for the requests queued when it is built (next power of two), within
+// the memory cap.  A lane of an engine costs about a full lane's bandwidth per step even when idle (a row's B
+// lane values are one contiguous segment), so engines are sized to demand rather than to the maximum; an idle
+// engine with more requests queued than lanes is rebuilt larger.
+int lanes_wanted(bh_server* S, Sector& sc, int p) {
+    const int prec = p == 0 ? BH_PREC_MIXED : BH_PREC_FP64;
+    const int cap = p == 0 ? lanes_for(S, sc.ham->D, prec) : std::min(4, lanes_for(S, sc.ham->D, prec));
+    int L = 1;
+    while (L < cap && L < static_cast<int>(sc.queue[p].size())) L <<= 1;
+    return std::min(L, cap);
+}
+
 int ensure_engine(bh_server* S, Sector& sc, int p) {
-    if (sc.eng[p]) return BH_OK;
+    if (sc.eng[p]) {
+        // rebuild an idle engine that is too small for its queue
+        if (sc.eng[p]->active() > 0 || static_cast<int>(sc.queue[p].size()) <= sc.eng[p]->lanes()) return BH_OK;
+        if (lanes_wanted(S, sc, p) <= sc.eng[p]->lanes()) return BH_OK;
+        sc.eng[p].reset();
+        sc.eng_bytes[p] = 0;
+    }
     BH_TRY(ensure_ham(S, sc));
     const int prec = p == 0 ? BH_PREC_MIXED : BH_PREC_FP64;
     // the FP64 engine serves the reruns of rejected mixed solves: full reorthogonalisation
     const bh_lanczos_opts lo = p == 0 ? S->opts.lanczos : bh_fallback_opts(S->opts.lanczos);
-    const int L = p == 0 ? lanes_for(S, sc.ham->D, prec) : std::min(4, lanes_for(S, sc.ham->D, prec));
+    const int L = lanes_wanted(S, sc, p);
     const size_t need = bh_engine_bytes(sc.ham->D, prec, L, lo) +
                         (prec == BH_PREC_FP64 ? static_cast<size_t>(sc.ham->nnz) * 12 : 0);
     evict_for(S, need, &sc);
