@@ -281,7 +281,7 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_b(LzArgs<XT> a) {
         if (LM > 0 && blockIdx.x == 0 && threadIdx.x < B && act && l < LM) a.st.dso[(2 * LZ_LMAX + l) * B + b] = cl[l];
     }
     constexpr int ROWS = LZ_WG / B;
-    constexpr int UN = LM > 4 ? 2 : 4;  // rows per thread in flight
+    constexpr int UN = LM > 0 ? 2 : 4;  // rows per thread in flight
     const int64_t ntile = (a.D + ROWS - 1) / ROWS;
     const int64_t per = (ntile + gridDim.x - 1) / gridDim.x;
     const int64_t L = bh_xcd_tile(blockIdx.x, gridDim.x);
@@ -394,93 +394,88 @@ __global__ void k_lz_init(XT* __restrict__ V0, const double* __restrict__ r0, in
     if (idx < D * B) V0[idx] = static_cast<XT>(r0[idx / B]);
 }
 
-// Reductions of per-workgroup partials, one wave per value in a fixed order (deterministic):
-//   LZ_RED_PLAIN: out[v] = sum_wg part[wg * nv + v]; values spread over the launch's workgroups
-//   LZ_RED_K1:    as PLAIN for the K1 sums; only the nmin alpha sums when the step does not orthogonalise
-//   LZ_RED_BETA:  v = b: beta_{j+1} = sqrt(sum), scale_{j+1} = 1/beta (active lanes), then step = j + 1.
-//                 The same pass advances, for every locked y_l of the lane, the estimate tau of
-//                 |<y_l, v_{j+1}>| (Parlett & Scott):
-//                   tau_{j+1} = (|theta_l - alpha_j| tau_j + beta_j tau_{j-1} + eps ||H||) / beta_{j+1},
-//                 reset to eps after a step that orthogonalised against y_l.  The next step orthogonalises
-//                 (all lanes, all locked vectors) when some tau exceeds sqrt(eps), and always the step after
-//                 one that started a pair, since v_{j+1} and v_{j+2} both have to be cleaned.  Launched as a
-//                 single workgroup, so that the step and flag writes follow every read of them.
-constexpr int LZ_RED_PLAIN = 0;
-constexpr int LZ_RED_BETA = 1;
-constexpr int LZ_RED_K1 = 2;
-
-template <int B>
-__global__ void __launch_bounds__(LZ_RED_WG) k_lz_reduce(LzState st, const double* __restrict__ part, int nwg, int nv,
-                                                         int mode, int maxit, double* __restrict__ out, int nmin,
-                                                         double eps, int so_always) {
+// Plain reduction of per-workgroup partials: out[v] = sum_wg part[wg * nv + v], one wave per value, lanes
+// striding over the workgroups in a fixed order (deterministic); values spread over the launch's workgroups.
+__global__ void __launch_bounds__(LZ_RED_WG) k_lz_reduce(const double* __restrict__ part, int nwg, int nv,
+                                                         double* __restrict__ out) {
     const int lane = threadIdx.x & 63;
     const int waves = LZ_RED_WG / 64;
-    __shared__ int want[LZ_RED_WG / 64];
-    const int so_cur = (mode == LZ_RED_PLAIN) ? 1 : st.so[0];
-    const int so_prev = (mode == LZ_RED_BETA) ? st.so[1] : 0;
-    // K1: the locked-vector sums exist only in exact steps.  BETA: the norms (v < B) and, when the step
-    // orthogonalised, the carried products <y_l, v_{j+1}> (v = B (1 + l) + b)
-    const int nvv = mode == LZ_RED_K1 ? (so_cur == 1 ? nv : nmin) : (mode == LZ_RED_BETA ? (so_cur ? nv : nmin) : nv);
-    int any = 0;
-    for (int v = blockIdx.x * waves + (threadIdx.x >> 6); v < nvv; v += gridDim.x * waves) {
+    for (int v = blockIdx.x * waves + (threadIdx.x >> 6); v < nv; v += gridDim.x * waves) {
         double s = 0.0;
         for (int w = lane; w < nwg; w += 64) s += part[static_cast<size_t>(w) * nv + v];
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
-        if (mode != LZ_RED_BETA) {
-            if (lane == 0) out[v] = s;
-            continue;
+        if (lane == 0) out[v] = s;
+    }
+}
+
+// R2: beta_{j+1} = ||v_{j+1}||, 1/beta, each active lane's step advanced; the carried products
+// <y_l, v_{j+1}> = (e_l - c_l) / beta_{j+1} and the Parlett-Scott estimates of the locked vectors; the next
+// step's orthogonalisation flag.  One workgroup: phase 1 reduces every value (a wave per value, four
+// independent partial sums per lane, fixed order) into LDS; phase 2 gives each lane to one thread, which
+// reads the step counter before anything is written.
+constexpr int LZ_BETA_MAXV = (1 + LZ_LMAX) * 64;
+
+template <int B>
+__global__ void __launch_bounds__(LZ_RED_WG) k_lz_beta(LzState st, const double* __restrict__ part, int nwg, int nv,
+                                                       int maxit, double eps, int so_always) {
+    __shared__ double sums[LZ_BETA_MAXV];
+    __shared__ int want;
+    const int lane = threadIdx.x & 63;
+    const int waves = LZ_RED_WG / 64;
+    const int so_cur = st.so[0], so_prev = st.so[1];
+    const int nvv = so_cur ? nv : B;
+    if (threadIdx.x == 0) want = 0;
+    for (int v = threadIdx.x >> 6; v < nvv; v += waves) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        int w = lane;
+        for (; w + 192 < nwg; w += 256) {
+            s0 += part[static_cast<size_t>(w) * nv + v];
+            s1 += part[static_cast<size_t>(w + 64) * nv + v];
+            s2 += part[static_cast<size_t>(w + 128) * nv + v];
+            s3 += part[static_cast<size_t>(w + 192) * nv + v];
         }
-        if (v >= B) {
-            // carried product of locked vector l of lane b, normalised by this step's ||v_{j+1}|| (the norm
-            // is summed again here in the same order, so it equals the value the v = b wave computes)
-            const int l = v / B - 1, bb = v % B;
-            double sn = 0.0;
-            for (int w = lane; w < nwg; w += 64) sn += part[static_cast<size_t>(w) * nv + bb];
+        for (; w < nwg; w += 64) s0 += part[static_cast<size_t>(w) * nv + v];
+        double s = (s0 + s1) + (s2 + s3);
 #pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) sn += __shfl_xor(sn, o, 64);
-            if (lane == 0 && l < LZ_LMAX && sn > 0.0 && st.active[bb] && l < st.nlock[bb]) {
-                st.dso[(LZ_LMAX + l) * B + bb] = st.dso[l * B + bb];
-                st.dso[l * B + bb] = (s - st.dso[(2 * LZ_LMAX + l) * B + bb]) / std::sqrt(sn);
-            }
-            continue;
-        }
-        // v = b; only lane 0 of this wave writes step[b] in the launch
-        const int j = st.step[v];
-        if (st.active[v] && j < maxit) {
-            const double nrm = std::sqrt(s);
-            const int nl = st.nlock[v];
-            if (lane < nl && lane < LZ_LMAX) {
-                const size_t i0 = static_cast<size_t>(lane) * B + v, i1 = static_cast<size_t>(LZ_LMAX + lane) * B + v;
+        for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (lane == 0) sums[v] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < B) {
+        const int b = threadIdx.x;
+        const int j = st.step[b];
+        if (st.active[b] && j < maxit) {
+            const double nrm = std::sqrt(sums[b]);
+            const int nl = min(st.nlock[b], LZ_LMAX);
+            const double al = st.sums[b], bj = st.beta[j * B + b];
+            int any = 0;
+            for (int l = 0; l < nl; ++l) {
+                const size_t i0 = static_cast<size_t>(l) * B + b, i1 = static_cast<size_t>(LZ_LMAX + l) * B + b;
+                // Parlett-Scott estimate of |<y_l, v_{j+1}>| (reset after an orthogonalising step)
                 const double tj = st.tau[i0], tjm = st.tau[i1];
-                double tn = eps;
-                if (!so_cur) {
-                    const double al = st.sums[v], bj = st.beta[j * B + v];
-                    const double inj = eps * st.anorm[v];
-                    tn = nrm > 0.0 ? (std::fabs(st.ltheta[i0] - al) * tj + bj * tjm + inj) / nrm : 1.0;
-                }
+                const double tn = so_cur ? eps
+                                         : (nrm > 0.0 ? (std::fabs(st.ltheta[i0] - al) * tj + bj * tjm + eps * st.anorm[b]) / nrm
+                                                      : 1.0);
                 st.tau[i1] = tj;
                 st.tau[i0] = tn;
                 if (tn > std::sqrt(eps)) any = 1;
+                if (so_cur && (1 + l) * B + b < nv && nrm > 0.0) {
+                    st.dso[i1] = st.dso[i0];
+                    st.dso[i0] = (sums[(1 + l) * B + b] - st.dso[(2 * LZ_LMAX + l) * B + b]) / nrm;
+                }
             }
-            if (lane == 0) {
-                st.beta[(j + 1) * B + v] = nrm;
-                st.scale[(j + 1) * B + v] = nrm > 0.0 ? 1.0 / nrm : 0.0;
-                st.step[v] = j + 1;
-            }
+            if (any) atomicOr(&want, 1);
+            st.beta[(j + 1) * B + b] = nrm;
+            st.scale[(j + 1) * B + b] = nrm > 0.0 ? 1.0 / nrm : 0.0;
+            st.step[b] = j + 1;
         }
     }
-    if (mode != LZ_RED_BETA) return;
-    // wave-level OR, then across the workgroup
-    any = __any(any) ? 1 : 0;
-    if (lane == 0) want[threadIdx.x >> 6] = any;
     __syncthreads();
     if (threadIdx.x == 0) {
-        int f = 0;
-        for (int w = 0; w < waves; ++w) f |= want[w];
         // an exact step that opened a pair is followed by a second exact step; then one-read steps (default)
         // or, with the estimate-driven schedule, exact pairs when the estimate asks
-        st.so[0] = (so_cur == 1 && so_prev != 1) ? 1 : (so_always ? 2 : (f ? 1 : 0));
+        st.so[0] = (so_cur == 1 && so_prev != 1) ? 1 : (so_always ? 2 : (want ? 1 : 0));
         st.so[1] = so_cur;
     }
 }
@@ -1528,8 +1523,7 @@ class LzEngine final : public LzEngineBase {
         KTimer kt(ctx, "lanczos_reduce", s);
         const int waves = LZ_RED_WG / 64;
         const int g = std::max(1, std::min((nv + waves - 1) / waves, 64));
-        hipLaunchKernelGGL((k_lz_reduce<B>), dim3(g), dim3(LZ_RED_WG), 0, s, st, part.p, nwg, nv, LZ_RED_PLAIN, cap, out, nv,
-                           0.0, 0);
+        hipLaunchKernelGGL(k_lz_reduce, dim3(g), dim3(LZ_RED_WG), 0, s, part.p, nwg, nv, out);
     }
     void step(int LM) {
         launch_k1(LM);
@@ -1554,8 +1548,8 @@ class LzEngine final : public LzEngineBase {
         }
         {
             KTimer kt(ctx, "lanczos_reduce", s);
-            hipLaunchKernelGGL((k_lz_reduce<B>), dim3(1), dim3(LZ_RED_WG), 0, s, st, part.p, grid_k2, (1 + LM) * B, LZ_RED_BETA,
-                               cap, nullptr, B, eps_store, lz_so_always());
+            hipLaunchKernelGGL((k_lz_beta<B>), dim3(1), dim3(LZ_RED_WG), 0, s, st, part.p, grid_k2, (1 + LM) * B, cap,
+                               eps_store, lz_so_always());
         }
     }
     // C holds rows t < max(njb) of [(t * nv + i) * B + b]
