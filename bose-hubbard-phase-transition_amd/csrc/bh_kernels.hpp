@@ -174,25 +174,31 @@ __device__ __forceinline__ void bh_rowlane_run(int64_t r0, int64_t r1, const int
         const int32_t ks = rp[r], ke = rp[r + 1];
         double acc = 0.0;
         for (int32_t k = ks; k < ke; k += U) {
-            double val[U];
-            int64_t c[U];
+            if constexpr (FMT == BH_FMT_PACKED) {
+                // only the words stay live across the gathers; the values are looked up afterwards
+                uint32_t w[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const bool ok = k + u < ke;
-                if constexpr (FMT == BH_FMT_PACKED) {
-                    const uint32_t w = ok ? words[k + u] : 0u;
-                    c[u] = static_cast<int64_t>(w >> cb);
-                    val[u] = ok ? tab[w & mask] : 0.0;
-                } else {
+                for (int u = 0; u < U; ++u) w[u] = k + u < ke ? words[k + u] : 0u;
+                XT xv[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) xv[u] = k + u < ke ? x[static_cast<int64_t>(w[u] >> cb) * B + bb] : XT(0);
+#pragma unroll
+                for (int u = 0; u < U; ++u) acc += tab[w[u] & mask] * to_d(xv[u]);
+            } else {
+                double val[U];
+                int64_t c[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const bool ok = k + u < ke;
                     c[u] = ok ? static_cast<int64_t>(col64[k + u]) : 0;
                     val[u] = ok ? val64[k + u] : 0.0;
                 }
+                XT xv[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) xv[u] = x[c[u] * B + bb];
+#pragma unroll
+                for (int u = 0; u < U; ++u) acc += val[u] * to_d(xv[u]);
             }
-            XT xv[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) xv[u] = x[c[u] * B + bb];
-#pragma unroll
-            for (int u = 0; u < U; ++u) acc += val[u] * to_d(xv[u]);
         }
         epi(r, bb, acc);
     }

@@ -45,6 +45,7 @@ constexpr int LZ_NVMAX = 16;       // Ritz vectors formed per pass
 constexpr int LZ_RED_WG = 1024;    // reduction workgroup: 16 waves, one value per wave
 constexpr int LZ_GRAPH_STEPS = 8;  // Lanczos steps per captured HIP graph
 constexpr int LZ_K1_TILES = 4;         // row groups per K1 workgroup on the row-lane engine
+constexpr int LZ_K1_U = 4;             // entries of a row in flight per thread in K1 (row-lane engine)
 constexpr int LZ_K1_GRID_MAX = 32768;  // K1 workgroups at most (rows of partial sums)
 constexpr double LZ_ACCEPT_LEVEL = 1e-6;  // mixed: bound on ||r||^2/gap of the levels above E0, relative
 
@@ -233,7 +234,7 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_a_rl(LzArgs<XT> a, int64_t rows_pe
     auto e = k1_setup<XT, B, FMT, LM>(a);
     const int64_t L = bh_xcd_tile(blockIdx.x, gridDim.x);
     const int64_t r0 = min(L * rows_per_wg, a.D), r1 = min(r0 + rows_per_wg, a.D);
-    if (e.act) bh_rowlane_run<XT, B, FMT, LZ_WG, 4>(r0, r1, a.row_ptr, a.words, a.col64, a.val64, tab, a.cb, e.vj, e);
+    if (e.act) bh_rowlane_run<XT, B, FMT, LZ_WG, LZ_K1_U>(r0, r1, a.row_ptr, a.words, a.col64, a.val64, tab, a.cb, e.vj, e);
     k1_finish<XT, B, FMT, LM>(e, lds, a.st.part);
 }
 
