@@ -195,14 +195,15 @@ def main():
     k1_bytes = 4.0 * nnz + 4.0 * (D + 1) + 4.0 * D + B * D * (4 + 4 + 8)
     k1_ms = k1_total / max(k1_count, 1)
     achieved = k1_bytes / (k1_ms * 1e-3) / 1e9
+    k1_tuned = ham.tuned(0, bhx.PREC_MIXED, B)
     log(f"K1: {k1_count} launches, avg {k1_ms * 1e3:.1f} us, {k1_bytes / 1e6:.0f} MB -> {achieved:.0f} GB/s; "
-        f"K2 avg {upd_total / max(upd_count, 1) * 1e3:.1f} us")
+        f"K2 avg {upd_total / max(upd_count, 1) * 1e3:.1f} us; autotuner {k1_tuned}")
 
     spmv = {}
     for prec, name in () if args.quick else ((bhx.PREC_MIXED, "mixed"), (bhx.PREC_FP64, "fp64")):
         for Bs in (1, B):
             ms, by = ham.spmv_bench(prec, Bs, 30)
-            spmv[f"{name}_B{Bs}"] = {"us": ms * 1e3, "alg_GBs": by / ms / 1e6}
+            spmv[f"{name}_B{Bs}"] = {"us": ms * 1e3, "alg_GBs": by / ms / 1e6, "tuned": ham.tuned(1, prec, Bs)}
     log("spmv:", json.dumps(spmv))
     # SpMV at the 4x4 periodic square lattice, n=10 (D=3,268,760): its vectors and matrix exceed the 256 MB
     # Infinity Cache, so this is the HBM-roofline SpMV figure.  Besides this build's own algorithmic bytes,
@@ -221,7 +222,8 @@ def main():
             ms, by = h3.spmv_bench(prec, 1, 30)
             spmv3[f"{name}_B1"] = {"us": ms * 1e3, "alg_GBs": by / ms / 1e6, "frac_of_peak": by / ms / 1e6 / HBM_PEAK_GBS,
                                    "csr_equiv_GBs": ref_bytes / ms / 1e6,
-                                   "csr_equiv_frac": ref_bytes / ms / 1e6 / HBM_PEAK_GBS}
+                                   "csr_equiv_frac": ref_bytes / ms / 1e6 / HBM_PEAK_GBS,
+                                   "tuned": h3.tuned(1, prec, 1)}
         spmv3["D"], spmv3["nnz"] = h3.D, h3.nnz
         log("spmv 4x4 n=10:", json.dumps(spmv3))
         h3.close()
@@ -261,7 +263,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile()},
         "k1": {"kernel": "k_lz_a_rl (fused Lanczos operator pass)", "launches": k1_count, "avg_us": k1_ms * 1e3,
-               "alg_bytes_per_launch": k1_bytes, "k2_avg_us": upd_total / max(upd_count, 1) * 1e3},
+               "alg_bytes_per_launch": k1_bytes, "k2_avg_us": upd_total / max(upd_count, 1) * 1e3,
+               "tuned": k1_tuned},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -123,6 +123,8 @@ def lib():
             "bh_spmv_host": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
             "bh_spmv_bench": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
                                         C.POINTER(C.c_double)]),
+            "bh_ham_tuned": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -331,6 +333,15 @@ class Hamiltonian:
         ms, by = C.c_double(), C.c_double()
         self.ctx.check(lib().bh_spmv_bench(self.ctx.h, self.h, prec, B, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
+
+    def tuned(self, kind: int, prec: int, B: int):
+        """Autotuner choice for (kind 0 = Lanczos operator pass, 1 = spmv; precision; width B):
+        {"variant": v, "candidates": {variant: us}}, or None before the sector was used that way."""
+        v, n = C.c_int(), C.c_int()
+        cv, cu = (C.c_int * 8)(), (C.c_double * 8)()
+        if lib().bh_ham_tuned(self.h, kind, prec, B, C.byref(v), C.byref(n), cv, cu, 8) != OK:
+            return None
+        return {"variant": v.value, "candidates": {cv[i]: cu[i] for i in range(min(n.value, 8))}}
 
     def lanczos(self, J, U, k: int = 4, prec: int = PREC_MIXED, **kw) -> LanczosOut:
         J = np.atleast_1d(np.asarray(J, np.float64))

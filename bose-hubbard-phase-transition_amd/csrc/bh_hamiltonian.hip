@@ -189,11 +189,32 @@ int bh_ham_rowblocks(bh_ham* h, int B, const int32_t** rb, int64_t* nblocks) {
     return BH_OK;
 }
 
-int bh_engine_choice(int B, const bh_ham* h) {
-    static int env = [] {
+extern "C" int bh_ham_tuned(const bh_ham* h, int kind, int prec, int B, int* variant, int* ncand, int* cand_var,
+                            double* cand_us, int cap) {
+    if (!h || kind < 0 || kind > 1) return BH_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(h->ctx->mu);
+    auto it = h->tuned.find(bh_tune_key(kind, prec, B));
+    if (it == h->tuned.end()) return bh_set_error(h->ctx, BH_EINVAL, "sector not used at this (kind, precision, width)");
+    if (variant) *variant = it->second.variant;
+    const int n = static_cast<int>(it->second.cand.size());
+    if (ncand) *ncand = n;
+    for (int i = 0; i < n && i < cap; ++i) {
+        if (cand_var) cand_var[i] = it->second.cand[i].first;
+        if (cand_us) cand_us[i] = it->second.cand[i].second;
+    }
+    return BH_OK;
+}
+
+int bh_engine_env() {
+    static const int env = [] {
         const char* e = std::getenv("BHX_ENGINE");
         return e ? std::atoi(e) : -1;
     }();
+    return env;
+}
+
+int bh_engine_choice(int B, const bh_ham* h) {
+    const int env = bh_engine_env();
     const bool staged_ok = static_cast<int64_t>(h->max_row) * B <= BH_STAGE_PRODUCTS_HOST;
     if (env == 0) return 0;
     if (env == 1) return staged_ok ? 1 : 2;

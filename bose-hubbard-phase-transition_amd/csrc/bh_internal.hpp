@@ -124,6 +124,13 @@ struct bh_basis {
     size_t bytes = 0;
 };
 
+// Autotuner result for one (kernel, precision, batch width) of a sector: the chosen variant and the
+// measured time of every candidate (variant, microseconds per launch); no candidates when not timed.
+struct BhTuned {
+    int variant = 1;
+    std::vector<std::pair<int, double>> cand;
+};
+
 struct bh_ham {
     bh_ctx* ctx = nullptr;
     const bh_basis* basis = nullptr;
@@ -147,7 +154,14 @@ struct bh_ham {
     // row blocks of the LDS-staged engine, per batch width B: first row of each block, then D
     std::map<int, std::pair<int32_t*, int64_t>> rowblocks;
     size_t bytes_rb = 0;
+    // operator-kernel variants chosen by the autotuner, keyed by bh_tune_key(kind, prec, B)
+    std::map<int, BhTuned> tuned;
 };
+
+// key of a tuning result: kind 0 = Lanczos K1, 1 = SpMV
+inline int bh_tune_key(int kind, int prec, int B) { return (kind * 4 + prec) * 128 + B; }
+// false when BHX_TUNE=0 (read once): the default variants are used without timing
+bool bh_tune_enabled();
 
 // row blocks of at most (products / B) entries for batch width B (built once, cached in the handle);
 // returns BH_ERANGE when a row does not fit one block
@@ -155,6 +169,7 @@ int bh_ham_rowblocks(bh_ham* h, int B, const int32_t** rb, int64_t* nblocks);
 // engine selection of the operator kernels: 0 = lanes per row (vector), 1 = LDS-staged row blocks,
 // 2 = one thread per (row, b); automatic unless BHX_ENGINE is set (read once).
 int bh_engine_choice(int B, const bh_ham* h);
+int bh_engine_env();  // BHX_ENGINE (-1 when unset)
 
 // ---- primes and packed occupation states ------------------------------------------------------------
 

@@ -105,7 +105,7 @@ __device__ __forceinline__ int64_t bh_xcd_tile(int64_t wg, int64_t nwg) {
 // takes one block at a time: phase 1 streams the block's entries (coalesced, independent loads and
 // gathers, several per lane in flight) and writes the products value * x[col * B + b] to LDS; phase 2
 // gives each (row, b) pair to one thread, which sums its row's products from LDS in column order and
-// hands the sum to the epilogue.  rb[] holds the first row of every block (and D at the end).
+// hands the sum to the epilogue (as an array of one, the epilogue interface of the row-lane engine).  rb[] holds the first row of every block (and D at the end).
 constexpr int BH_STAGE_PRODUCTS = 2048;  // FP64 products per block (16 KB of LDS)
 
 template <typename XT, int B, int FMT, int WG, typename Epi>
@@ -148,8 +148,8 @@ __device__ __forceinline__ void bh_rowblock_run(int64_t t0, int64_t t1, const in
             const int32_t r = r0 + i / B;
             const int bb = i % B;
             const int32_t a = rp[r] - ks, e = rp[r + 1] - ks;
-            double sum = 0.0;
-            for (int32_t q = a; q < e; ++q) sum += prod[q * B + bb];
+            double sum[1] = {0.0};
+            for (int32_t q = a; q < e; ++q) sum[0] += prod[q * B + bb];
             epi(static_cast<int64_t>(r), bb, sum);
         }
         __syncthreads();
@@ -157,49 +157,71 @@ __device__ __forceinline__ void bh_rowblock_run(int64_t t0, int64_t t1, const in
 }
 
 // ---- row-lane engine ---------------------------------------------------------------------------------
-// One thread per (row, b); B consecutive lanes share a row, so each entry's word is one broadcast load
-// and its gather one contiguous B-element segment.  The entry loop is unrolled by U with all loads of
-// an unrolled step issued before the multiply-adds.  Rows [r0, r1) of a workgroup are walked in
-// groups of WG / B.
-template <typename XT, int B, int FMT, int WG, int U, typename Epi>
+// B / V consecutive threads share a row; each holds V consecutive vectors b0 .. b0 + V - 1 of it, so an
+// entry's word is one broadcast load and its gather one V-element vector load per thread (the row's B
+// elements are contiguous).  A wavefront covers 64 V / B rows.  The entry loop is unrolled by U with all
+// loads of an unrolled step issued before the multiply-adds; only the words stay live across the
+// gathers (the values are looked up afterwards).  Rows [r0, r1) of a workgroup are walked in groups of
+// WG V / B.  The sum of every (row, b) runs over the row's entries in storage order whatever V and U are.
+template <typename T, int V>
+struct alignas(sizeof(T) * V) BhVec {
+    T v[V];
+};
+
+template <int V, typename T>
+__device__ __forceinline__ BhVec<T, V> bh_vload(const T* p) {
+    return *reinterpret_cast<const BhVec<T, V>*>(p);
+}
+
+template <typename XT, int B, int V, int FMT, int WG, int U, typename Epi>
 __device__ __forceinline__ void bh_rowlane_run(int64_t r0, int64_t r1, const int32_t* __restrict__ rp,
                                                const uint32_t* __restrict__ words, const int32_t* __restrict__ col64,
                                                const double* __restrict__ val64, const double* __restrict__ tab, int cb,
                                                const XT* __restrict__ x, Epi& epi) {
+    static_assert(B % V == 0, "V divides B");
+    constexpr int TPR = B / V;  // threads per row
     const uint32_t mask = (1u << cb) - 1u;
-    const int bb = threadIdx.x % B;
-    for (int64_t base = r0; base < r1; base += WG / B) {
-        const int64_t r = base + threadIdx.x / B;
+    const int b0 = (threadIdx.x % TPR) * V;
+    for (int64_t base = r0; base < r1; base += WG / TPR) {
+        const int64_t r = base + threadIdx.x / TPR;
         if (r >= r1) continue;
         const int32_t ks = rp[r], ke = rp[r + 1];
-        double acc = 0.0;
+        double acc[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] = 0.0;
         for (int32_t k = ks; k < ke; k += U) {
+            uint32_t w[U];
+            double val[U];
+            BhVec<XT, V> xv[U];
             if constexpr (FMT == BH_FMT_PACKED) {
-                // only the words stay live across the gathers; the values are looked up afterwards
-                uint32_t w[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) w[u] = k + u < ke ? words[k + u] : 0u;
-                XT xv[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) xv[u] = k + u < ke ? x[static_cast<int64_t>(w[u] >> cb) * B + bb] : XT(0);
+                for (int u = 0; u < U; ++u) {
+                    if (k + u < ke) {
+                        xv[u] = bh_vload<V>(x + static_cast<int64_t>(w[u] >> cb) * B + b0);
+                    } else {
 #pragma unroll
-                for (int u = 0; u < U; ++u) acc += tab[w[u] & mask] * to_d(xv[u]);
+                        for (int v = 0; v < V; ++v) xv[u].v[v] = XT(0);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) val[u] = tab[w[u] & mask];
             } else {
-                double val[U];
-                int64_t c[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const bool ok = k + u < ke;
-                    c[u] = ok ? static_cast<int64_t>(col64[k + u]) : 0;
+                    w[u] = ok ? static_cast<uint32_t>(col64[k + u]) : 0u;
                     val[u] = ok ? val64[k + u] : 0.0;
                 }
-                XT xv[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) xv[u] = x[c[u] * B + bb];
-#pragma unroll
-                for (int u = 0; u < U; ++u) acc += val[u] * to_d(xv[u]);
+                for (int u = 0; u < U; ++u) xv[u] = bh_vload<V>(x + static_cast<int64_t>(w[u]) * B + b0);
             }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int v = 0; v < V; ++v) acc[v] += val[u] * to_d(xv[u].v[v]);
         }
-        epi(r, bb, acc);
+        epi(r, b0, acc);
     }
 }

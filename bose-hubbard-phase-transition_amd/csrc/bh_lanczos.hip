@@ -46,6 +46,8 @@ constexpr int LZ_RED_WG = 1024;    // reduction workgroup: 16 waves, one value p
 constexpr int LZ_GRAPH_STEPS = 8;  // Lanczos steps per captured HIP graph
 constexpr int LZ_K1_TILES = 4;         // row groups per K1 workgroup on the row-lane engine
 constexpr int LZ_K1_U = 4;             // entries of a row in flight per thread in K1 (row-lane engine)
+constexpr int LZ_TUNE_REPS = 3;        // timed launches per K1 variant in the autotuner
+constexpr int LZ_CHECK_MIN = 4;        // fewest steps between two analyses of a lane
 constexpr int LZ_K1_GRID_MAX = 32768;  // K1 workgroups at most (rows of partial sums)
 constexpr double LZ_ACCEPT_LEVEL = 1e-6;  // mixed: bound on ||r||^2/gap of the levels above E0, relative
 
@@ -134,78 +136,95 @@ __device__ __forceinline__ void lz_block_put_multi(const double (&v)[NQ], double
     }
 }
 
-// K1 epilogue state shared by both operator engines
-template <typename XT, int B, int FMT, int LM>
+// Per-workgroup sums of V values per thread, thread t holding vectors b = (t % (B / V)) * V + v, stored at
+// part[wg * nv + q * B + b] (the V = 1 case is lz_block_put).
+template <int B, int V>
+__device__ __forceinline__ void lz_block_put_v(const double (&x)[V], int q, double* lds, double* part, int nv) {
+    constexpr int TPR = B / V;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        double t = x[v];
+#pragma unroll
+        for (int o = TPR; o < 64; o <<= 1) t += __shfl_xor(t, o, 64);
+        if (lane < TPR) lds[wid * B + lane * V + v] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < B) {
+        double s = 0.0;
+        for (int w = 0; w < LZ_WG / 64; ++w) s += lds[w * B + threadIdx.x];
+        part[static_cast<size_t>(blockIdx.x) * nv + q * B + threadIdx.x] = s;
+    }
+    __syncthreads();
+}
+
+// K1 epilogue shared by both operator engines: a thread holds the V vectors b0 .. b0 + V - 1 of its rows.
+// w = J_b T x_b + U_b V x_b - beta_j v_{j-1} (x_b = s_j v_j, the stored vector times its scale) is written
+// to W; the thread accumulates <x_b, w> for alpha.
+template <typename XT, int B, int V, int FMT>
 struct K1Epi {
     const LzArgs<XT>& a;
-    const XT* vj;
-    const XT* vjm;
-    double sj, bj, Jb, Ub;
-    int j, nl;
-    bool act;
-    bool so;  // uniform: this step orthogonalises against the locked vectors
-    double pa = 0.0;
-    double pyw[LM > 0 ? LM : 1] = {};
-    double pyv[LM > 0 ? LM : 1] = {};
-    __device__ __forceinline__ void operator()(int64_t r, int bb, double acc) {
-        if (!act) return;
+    int b0 = 0;
+    double sj[V], bj[V], Jb[V], Ub[V];
+    int64_t om[V];  // offset of lane b0 + v's v_{j-1} in the basis (slot j - 1; slot 0 when j = 0)
+    bool act[V];
+    bool any = false;   // some lane of the thread takes this step
+    bool same = true;   // every lane reads v_{j-1} from one slot (one vector load)
+    double pa[V];
+    __device__ __forceinline__ void operator()(int64_t r, int, const double (&acc)[V]) {
+        if (!any) return;  // the row-block engine calls every thread
         const double d = a.diag32 ? static_cast<double>(a.diag32[r]) : a.diag64[r];
-        const double hx = (FMT == BH_FMT_PACKED) ? -acc : acc;
-        const int64_t idx = r * B + bb;
-        const double xr = to_d(vj[idx]) * sj;
-        double w = Jb * hx * sj + Ub * d * xr;
-        if (j > 0) w -= bj * to_d(vjm[idx]);
-        a.W[idx] = w;
-        pa += xr * w;
-        if constexpr (LM > 0) {
+        const int64_t idx = r * B + b0;
+        const BhVec<XT, V> xv = bh_vload<V>(a.Xc + idx);
+        BhVec<XT, V> xm;
+        if (same) {
+            xm = bh_vload<V>(a.V + om[0] + idx);
+        } else {
 #pragma unroll
-            for (int l = 0; l < LM; ++l) {
-                if (l < nl) {
-                    const double y = to_d(a.Y[static_cast<int64_t>(l) * a.vs + idx]);
-                    pyw[l] += y * w;
-                    pyv[l] += y * xr;
-                }
-            }
+            for (int v = 0; v < V; ++v) xm.v[v] = a.V[om[v] + idx + v];
         }
+        BhVec<double, V> wv;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const double hx = (FMT == BH_FMT_PACKED) ? -acc[v] : acc[v];
+            const double xr = to_d(xv.v[v]) * sj[v];
+            const double w = Jb[v] * hx * sj[v] + Ub[v] * d * xr - bj[v] * to_d(xm.v[v]);
+            wv.v[v] = w;
+            if (act[v]) pa[v] += xr * w;
+        }
+        *reinterpret_cast<BhVec<double, V>*>(a.W + idx) = wv;
     }
 };
 
-template <typename XT, int B, int FMT, int LM>
-__device__ __forceinline__ K1Epi<XT, B, FMT, LM> k1_setup(const LzArgs<XT>& a) {
-    const int b = threadIdx.x % B;  // fixed per thread: LZ_WG is a multiple of B
-    const int j = a.st.step[b];
-    const bool so = LM > 0 && a.st.so[0] == 1;
-    K1Epi<XT, B, FMT, LM> e{a,
-                            a.Xc,
-                            a.V + static_cast<int64_t>(j > 0 ? j - 1 : 0) * a.vs,
-                            a.st.scale[j * B + b],
-                            j > 0 ? a.st.beta[j * B + b] * a.st.scale[(j - 1) * B + b] : 0.0,
-                            a.st.J[b],
-                            a.st.U[b],
-                            j,
-                            (a.st.active[b] && so) ? a.st.nlock[b] : 0,
-                            a.st.active[b] != 0 && j < a.maxit,
-                            so};
-    return e;
-}
-
-template <typename XT, int B, int FMT, int LM>
-__device__ __forceinline__ void k1_finish(K1Epi<XT, B, FMT, LM>& e, double* lds, double* part) {
-    constexpr int NV = (1 + 2 * LM) * B;
-    lz_block_put<B>(e.pa, 0, lds, part, NV);
-    if constexpr (LM > 0) {
-        if (!e.so) return;  // uniform
+template <typename XT, int B, int V, int FMT>
+__device__ __forceinline__ void k1_setup(const LzArgs<XT>& a, K1Epi<XT, B, V, FMT>& e) {
+    e.b0 = (threadIdx.x % (B / V)) * V;  // fixed per thread: LZ_WG is a multiple of B
+    int64_t om0 = -1;
 #pragma unroll
-        for (int l = 0; l < LM; ++l) {
-            lz_block_put<B>(e.pyw[l], 1 + l, lds, part, NV);
-            lz_block_put<B>(e.pyv[l], 1 + LM + l, lds, part, NV);
-        }
+    for (int v = 0; v < V; ++v) {
+        const int b = e.b0 + v;
+        const int j = a.st.step[b];
+        e.act[v] = a.st.active[b] != 0 && j < a.maxit;
+        e.sj[v] = a.st.scale[j * B + b];
+        e.bj[v] = j > 0 ? a.st.beta[j * B + b] * a.st.scale[(j - 1) * B + b] : 0.0;
+        e.Jb[v] = a.st.J[b];
+        e.Ub[v] = a.st.U[b];
+        e.om[v] = static_cast<int64_t>(j > 0 ? j - 1 : 0) * a.vs;
+        e.pa[v] = 0.0;
+        e.any = e.any || e.act[v];
+        if (e.act[v] && om0 < 0) om0 = e.om[v];
+    }
+    // lanes that do not take the step read (and discard) the v_{j-1} slot of the active ones
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        if (!e.act[v] && om0 >= 0) e.om[v] = om0;
+        e.same = e.same && e.om[v] == e.om[0];
     }
 }
 
-// K1 on the LDS-staged row-block engine (B = 1)
-template <typename XT, int B, int FMT, int LM>
-__global__ void __launch_bounds__(LZ_WG) k_lz_a_rb(LzArgs<XT> a, const int32_t* __restrict__ rb, int64_t nblocks) {
+// K1 on the LDS-staged row-block engine (B = 1 in the automatic choice)
+template <typename XT, int B, int FMT>
+__global__ void __launch_bounds__(LZ_WG) k_lz_a_rb(LzArgs<XT> a, const int32_t* __restrict__ rb, int64_t nblocks, int nv) {
     extern __shared__ double smem[];
     double* tab = smem;
     double* prod = smem + (FMT == BH_FMT_PACKED ? (1 << a.cb) : 0);
@@ -214,28 +233,65 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_a_rb(LzArgs<XT> a, const int32_t* 
         for (int i = threadIdx.x; i < (1 << a.cb); i += LZ_WG) tab[i] = a.table[i];
         __syncthreads();
     }
-    auto e = k1_setup<XT, B, FMT, LM>(a);
+    K1Epi<XT, B, 1, FMT> e{a};
+    k1_setup(a, e);
     const int64_t per = (nblocks + gridDim.x - 1) / gridDim.x;
     const int64_t L = bh_xcd_tile(blockIdx.x, gridDim.x);
     const int64_t t0 = min(L * per, nblocks), t1 = min(t0 + per, nblocks);
-    bh_rowblock_run<XT, B, FMT, LZ_WG>(t0, t1, rb, a.row_ptr, a.words, a.col64, a.val64, tab, a.cb, e.vj, prod, e);
-    k1_finish<XT, B, FMT, LM>(e, lds, a.st.part);
+    bh_rowblock_run<XT, B, FMT, LZ_WG>(t0, t1, rb, a.row_ptr, a.words, a.col64, a.val64, tab, a.cb, a.Xc, prod, e);
+    lz_block_put_v<B, 1>(e.pa, 0, lds, a.st.part, nv);
 }
 
-// K1 on the row-lane engine (one thread per (row, b), contiguous row ranges per workgroup)
-template <typename XT, int B, int FMT, int LM>
-__global__ void __launch_bounds__(LZ_WG) k_lz_a_rl(LzArgs<XT> a, int64_t rows_per_wg) {
+// K1 on the row-lane engine (B / V threads per row, contiguous row ranges per workgroup)
+template <typename XT, int B, int V, int FMT>
+__global__ void __launch_bounds__(LZ_WG) k_lz_a_rl(LzArgs<XT> a, int64_t rows_per_wg, int nv) {
     extern __shared__ double tab[];
     __shared__ double lds[LZ_WG / 64 * B];
     if constexpr (FMT == BH_FMT_PACKED) {
         for (int i = threadIdx.x; i < (1 << a.cb); i += LZ_WG) tab[i] = a.table[i];
         __syncthreads();
     }
-    auto e = k1_setup<XT, B, FMT, LM>(a);
+    K1Epi<XT, B, V, FMT> e{a};
+    k1_setup(a, e);
     const int64_t L = bh_xcd_tile(blockIdx.x, gridDim.x);
     const int64_t r0 = min(L * rows_per_wg, a.D), r1 = min(r0 + rows_per_wg, a.D);
-    if (e.act) bh_rowlane_run<XT, B, FMT, LZ_WG, LZ_K1_U>(r0, r1, a.row_ptr, a.words, a.col64, a.val64, tab, a.cb, e.vj, e);
-    k1_finish<XT, B, FMT, LM>(e, lds, a.st.part);
+    if (e.any)
+        bh_rowlane_run<XT, B, V, FMT, LZ_WG, LZ_K1_U>(r0, r1, a.row_ptr, a.words, a.col64, a.val64, tab, a.cb, a.Xc, e);
+    lz_block_put_v<B, V>(e.pa, 0, lds, a.st.part, nv);
+}
+
+// Dot products of an exact orthogonalisation step (so = 1): partials of <y_l, w> and <y_l, s_j v_j> for the
+// locked vectors l < nlock, at part[wg * nv + (1 + l) * B + b] and part[wg * nv + (1 + LM + l) * B + b] over
+// the same rows of partials as K1 (which writes q = 0).  Other steps leave at once.
+template <typename XT, int B, int LM>
+__global__ void __launch_bounds__(LZ_WG) k_lz_sodots(LzArgs<XT> a, int64_t rows_per_wg, int nv) {
+    if (a.st.so[0] != 1) return;  // uniform
+    __shared__ double lds[LZ_PUT_LDS];
+    const int b = threadIdx.x % B;
+    const int j = a.st.step[b];
+    const bool act = a.st.active[b] != 0 && j < a.maxit;
+    const int nl = act ? min(a.st.nlock[b], LM) : 0;
+    const double sj = a.st.scale[j * B + b];
+    double acc[2 * LM];
+#pragma unroll
+    for (int q = 0; q < 2 * LM; ++q) acc[q] = 0.0;
+    const int64_t L = bh_xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t r0 = min(L * rows_per_wg, a.D), r1 = min(r0 + rows_per_wg, a.D);
+    if (nl > 0) {
+        for (int64_t r = r0 + threadIdx.x / B; r < r1; r += LZ_WG / B) {
+            const int64_t idx = r * B + b;
+            const double w = a.W[idx], xr = to_d(a.Xc[idx]) * sj;
+#pragma unroll
+            for (int l = 0; l < LM; ++l) {
+                if (l < nl) {
+                    const double y = to_d(a.Y[static_cast<int64_t>(l) * a.vs + idx]);
+                    acc[l] += y * w;
+                    acc[LM + l] += y * xr;
+                }
+            }
+        }
+    }
+    lz_block_put_multi<B, 2 * LM>(acc, lds, a.st.part + B, nv);
 }
 
 // K2: v_{j+1} = w - alpha v_j - sum_l c_l y_l (stored in slot j+1 and, replacing v_j, in the current-vector
@@ -767,6 +823,10 @@ struct RunHost {
     double tnorm = 0.0;
     double tnorm0 = 0.0;                 // ||T|| at the first analysis (divergence guard)
     bool exhausted = false;              // the run ended on an invariant Krylov space
+    // convergence history for the check schedule: max over the unconverged wanted levels of
+    // log(bound / threshold) at the previous analysis, and the lane's total steps then (-1: none)
+    double prev_gap = 0.0;
+    int prev_tot = -1;
 };
 
 struct Analysis {
@@ -942,11 +1002,12 @@ class LzEngine final : public LzEngineBase {
                        cap,
                        st};
         shm = FMT == BH_FMT_PACKED ? (static_cast<size_t>(1) << h->code_bits) * sizeof(double) : 0;
-        engine = bh_engine_choice(B, h) == 1 ? 1 : 2;  // staged row blocks, or one thread per (row, b)
-        if (engine == 1) BH_TRY(bh_ham_rowblocks(h, B, &rblk, &nrblk));
+        // K1 variant before the tuner has run: staged row blocks, or one thread per (row, b)
+        staged_ok = static_cast<int64_t>(h->max_row) * B <= BH_STAGE_PRODUCTS_HOST;
+        if (staged_ok) BH_TRY(bh_ham_rowblocks(h, B, &rblk, &nrblk));
         grid_rb = static_cast<int>(std::min<int64_t>(std::max<int64_t>(nrblk, 1), LZ_GRID_MAX));
         shm_rb = shm + BH_STAGE_PRODUCTS * sizeof(double);
-        grid_k1 = engine == 1 ? grid_rb : grid_rl;
+        set_k1(bh_engine_choice(B, h) == 1 && staged_ok ? 0 : 1);
         const int64_t ntile_b = (D + LZ_WG / B - 1) / (LZ_WG / B);
         grid_k2 = static_cast<int>(std::min<int64_t>(ntile_b, LZ_GRID_MAX));
         rh.assign(B, RunHost());
@@ -1031,6 +1092,7 @@ class LzEngine final : public LzEngineBase {
     // scalars, the orthogonalisation flag -- lives in device memory, so one captured step is valid for any
     // step of any lane.  With timing on, the launches are issued one by one so that each carries its events.
     int run(int steps) override {
+        if (!tuned) BH_TRY(tune_k1());
         const int LM = lmax_used == 0 ? 0 : (lmax_used <= 4 ? 4 : 16);
         const int nsteps = steps;
         if (use_graphs && !ctx->timing) {
@@ -1168,6 +1230,24 @@ class LzEngine final : public LzEngineBase {
                 R.tnorm = an.tnorm;
                 done_now[b] = 1;
                 return;
+            }
+            // Next analysis: where the slowest unconverged wanted level is predicted to meet its threshold,
+            // extrapolating log(bound / threshold) linearly in steps from the lane's previous analysis
+            // (Lanczos bounds fall at least geometrically once a level has emerged, so the prediction errs
+            // late).  Never later than the default interval, at least LZ_CHECK_MIN steps ahead.
+            if (static_cast<int>(lev.size()) == k) {
+                double gap = 0.0;
+                for (size_t q = 0; q < lev.size(); ++q)
+                    if (lbound[q] >= cthr[q]) gap = std::fmax(gap, std::log(lbound[q] / cthr[q]));
+                if (R.prev_tot >= 0 && tot[b] > R.prev_tot && R.prev_gap > gap) {
+                    const double rate = (R.prev_gap - gap) / (tot[b] - R.prev_tot);
+                    const double d = std::ceil(1.05 * gap / rate) + 1.0;
+                    if (d < nextc[b] - j) nextc[b] = j + std::max(LZ_CHECK_MIN, static_cast<int>(d));
+                }
+                R.prev_gap = gap;
+                R.prev_tot = tot[b];
+            } else {
+                R.prev_tot = -1;
             }
             if (full) {
                 // thick restart: keep the lowest distinct Ritz pairs (one per cluster of copies)
@@ -1507,18 +1587,99 @@ class LzEngine final : public LzEngineBase {
         e = std::min<int64_t>(e, static_cast<int64_t>(jl[b]) + std::max(0, maxsteps - tot[b]));
         return e;
     }
-    void launch_k1(int LM) {
-        KTimer kt(ctx, "lanczos_spmv", s);
-#define K1L(LMV)                                                                                                       \
-    if (LM == LMV) {                                                                                                   \
-        if (engine == 1)                                                                                               \
-            hipLaunchKernelGGL((k_lz_a_rb<XT, B, FMT, LMV>), dim3(grid_rb), dim3(LZ_WG), shm_rb, s, a, rblk, nrblk);   \
-        else                                                                                                           \
-            hipLaunchKernelGGL((k_lz_a_rl<XT, B, FMT, LMV>), dim3(grid_rl), dim3(LZ_WG), shm, s, a, rpw);              \
-        return;                                                                                                        \
+    // rows of a K1 workgroup for variant V (row-lane engine): LZ_K1_TILES passes of LZ_WG V / B rows, more
+    // when the grid would exceed LZ_K1_GRID_MAX
+    int64_t k1_rows(int V) const {
+        return std::max<int64_t>(static_cast<int64_t>(LZ_WG * V / B) * LZ_K1_TILES, (D + LZ_K1_GRID_MAX - 1) / LZ_K1_GRID_MAX);
     }
-        K1L(0) K1L(4) K1L(16)
-#undef K1L
+    void set_k1(int var) {
+        k1v = var;
+        grid_k1 = var == 0 ? grid_rb : static_cast<int>((D + k1_rows(var) - 1) / k1_rows(var));
+        rpw_so = (D + grid_k1 - 1) / grid_k1;
+    }
+    // K1 variant `var` (0: staged row blocks; V = 1, 2, 4: row-lane engine, V vectors per thread)
+    void launch_k1_var(int var, int nv) {
+        if (var == 0) {
+            hipLaunchKernelGGL((k_lz_a_rb<XT, B, FMT>), dim3(grid_rb), dim3(LZ_WG), shm_rb, s, a, rblk, nrblk, nv);
+            return;
+        }
+        const int64_t rows = k1_rows(var);
+        const dim3 g(static_cast<unsigned>((D + rows - 1) / rows));
+        if constexpr (B >= 4 && sizeof(XT) * 4 <= 16) {
+            if (var == 4) {
+                hipLaunchKernelGGL((k_lz_a_rl<XT, B, 4, FMT>), g, dim3(LZ_WG), shm, s, a, rows, nv);
+                return;
+            }
+        }
+        if constexpr (B >= 2) {
+            if (var == 2) {
+                hipLaunchKernelGGL((k_lz_a_rl<XT, B, 2, FMT>), g, dim3(LZ_WG), shm, s, a, rows, nv);
+                return;
+            }
+        }
+        hipLaunchKernelGGL((k_lz_a_rl<XT, B, 1, FMT>), g, dim3(LZ_WG), shm, s, a, rows, nv);
+    }
+    void launch_k1(int LM) {
+        const int nv = (1 + 2 * LM) * B;
+        {
+            KTimer kt(ctx, "lanczos_spmv", s);
+            launch_k1_var(k1v, nv);
+        }
+        if (LM > 0) {
+            KTimer kt(ctx, "lanczos_sodots", s);
+            if (LM == 4) hipLaunchKernelGGL((k_lz_sodots<XT, B, 4>), dim3(grid_k1), dim3(LZ_WG), 0, s, a, rpw_so, nv);
+            else hipLaunchKernelGGL((k_lz_sodots<XT, B, 16>), dim3(grid_k1), dim3(LZ_WG), 0, s, a, rpw_so, nv);
+        }
+    }
+    // Autotuner of K1, run once per (sector, precision, batch width) on the engine's first steps (the
+    // result is kept in the sector's handle): every variant that fits is timed with HIP events over a
+    // warm-up and LZ_TUNE_REPS launches on the admitted lanes (K1 writes only W and the partial sums, which
+    // the real step overwrites), and the fastest is kept.  The variants differ in rows per wavefront
+    // (64 V / B on the row-lane engine) and in the staging of the gathers; each (row, b) sums its entries in
+    // storage order in all of them (the row-lane variants give identical sums, the staged engine rounds each
+    // product before adding it).  BHX_TUNE=0 keeps the default variant.
+    int tune_k1() {
+        tuned = true;
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        const int key = bh_tune_key(0, MIXED ? BH_PREC_MIXED : BH_PREC_FP64, B);
+        auto it = h->tuned.find(key);
+        if (it != h->tuned.end()) {
+            set_k1(it->second.variant);
+            return BH_OK;
+        }
+        std::vector<int> cand;
+        if (staged_ok && rblk) cand.push_back(0);
+        cand.push_back(1);
+        if (B >= 2) cand.push_back(2);
+        if (B >= 4 && sizeof(XT) * 4 <= 16) cand.push_back(4);
+        BhTuned t;
+        t.variant = k1v;
+        if (bh_tune_enabled() && cand.size() > 1) {
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            BH_HIP(ctx, hipEventCreate(&e0));
+            BH_HIP(ctx, hipEventCreate(&e1));
+            double best = 0.0;
+            for (int var : cand) {
+                launch_k1_var(var, B);
+                BH_HIP(ctx, hipEventRecord(e0, s));
+                for (int r = 0; r < LZ_TUNE_REPS; ++r) launch_k1_var(var, B);
+                BH_HIP(ctx, hipEventRecord(e1, s));
+                BH_HIP(ctx, hipEventSynchronize(e1));
+                float ms = 0.f;
+                BH_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+                const double us = 1e3 * ms / LZ_TUNE_REPS;
+                t.cand.push_back({var, us});
+                if (best == 0.0 || us < best) {
+                    best = us;
+                    t.variant = var;
+                }
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+        h->tuned[key] = t;
+        set_k1(t.variant);
+        return BH_OK;
     }
     void reduce_plain(int nwg, int nv, double* out) {
         KTimer kt(ctx, "lanczos_reduce", s);
@@ -1748,8 +1909,9 @@ class LzEngine final : public LzEngineBase {
     size_t n_alpha = 0, n_beta = 0, bytes = 0;
     LzState st{};
     LzArgs<XT> a{};
-    int engine = 2, grid_rb = 1, grid_rl = 1, grid_k1 = 1, grid_k2 = 1;
-    int64_t rpw = 1, nrblk = 0;
+    int k1v = 1, grid_rb = 1, grid_rl = 1, grid_k1 = 1, grid_k2 = 1;
+    bool staged_ok = false, tuned = false;
+    int64_t rpw = 1, rpw_so = 1, nrblk = 0;
     const int32_t* rblk = nullptr;
     size_t shm = 0, shm_rb = 0;
     std::vector<RunHost> rh;

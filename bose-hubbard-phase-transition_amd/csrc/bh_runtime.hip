@@ -1,5 +1,6 @@
 // bh_runtime.hip -- context, errors, budgeted device memory, kernel timing, lattices.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "bh_internal.hpp"
@@ -84,6 +85,14 @@ void bh_ktimers_flush(bh_ctx* ctx) {
         (void)hipEventDestroy(t.b);
     }
     ctx->ktimers.clear();
+}
+
+bool bh_tune_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("BHX_TUNE");
+        return !(e && std::strcmp(e, "0") == 0);
+    }();
+    return on;
 }
 
 int64_t bh_binom_host(int n, int k) {

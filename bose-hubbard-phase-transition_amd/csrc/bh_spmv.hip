@@ -50,15 +50,15 @@ __global__ void __launch_bounds__(BH_WG) k_spmv_rb(SpmvArgs<XT> a, const int32_t
         __syncthreads();
     }
     const int64_t t = a.xcd ? bh_xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
-    auto epi = [&](int64_t r, int bb, double sum) {
-        const double hx = (FMT == BH_FMT_PACKED) ? -sum : sum;
+    auto epi = [&](int64_t r, int bb, const double (&sum)[1]) {
+        const double hx = (FMT == BH_FMT_PACKED) ? -sum[0] : sum[0];
         const double d = a.diag32 ? static_cast<double>(a.diag32[r]) : a.diag64[r];
         a.y[r * B + bb] = static_cast<XT>(a.J[bb] * hx + a.U[bb] * d * to_d(a.x[r * B + bb]));
     };
     bh_rowblock_run<XT, B, FMT, BH_WG>(t, t + 1, rb, a.row_ptr, a.words, a.col64, a.val64, tab, a.cb, a.x, prod, epi);
 }
 
-template <typename XT, int B, int FMT>
+template <typename XT, int B, int V, int FMT>
 __global__ void __launch_bounds__(BH_WG) k_spmv_rl(SpmvArgs<XT> a, int64_t rows_per_wg) {
     extern __shared__ double tab[];
     if constexpr (FMT == BH_FMT_PACKED) {
@@ -67,25 +67,36 @@ __global__ void __launch_bounds__(BH_WG) k_spmv_rl(SpmvArgs<XT> a, int64_t rows_
     }
     const int64_t t = a.xcd ? bh_xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
     const int64_t r0 = min(t * rows_per_wg, a.D), r1 = min(r0 + rows_per_wg, a.D);
-    auto epi = [&](int64_t r, int bb, double sum) {
-        const double hx = (FMT == BH_FMT_PACKED) ? -sum : sum;
+    auto epi = [&](int64_t r, int b0, const double (&acc)[V]) {
         const double d = a.diag32 ? static_cast<double>(a.diag32[r]) : a.diag64[r];
-        a.y[r * B + bb] = static_cast<XT>(a.J[bb] * hx + a.U[bb] * d * to_d(a.x[r * B + bb]));
+        const BhVec<XT, V> xv = bh_vload<V>(a.x + r * B + b0);
+        BhVec<XT, V> yv;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const double hx = (FMT == BH_FMT_PACKED) ? -acc[v] : acc[v];
+            yv.v[v] = static_cast<XT>(a.J[b0 + v] * hx + a.U[b0 + v] * d * to_d(xv.v[v]));
+        }
+        *reinterpret_cast<BhVec<XT, V>*>(a.y + r * B + b0) = yv;
     };
-    bh_rowlane_run<XT, B, FMT, BH_WG, 4>(r0, r1, a.row_ptr, a.words, a.col64, a.val64, tab, a.cb, a.x, epi);
+    bh_rowlane_run<XT, B, V, FMT, BH_WG, 4>(r0, r1, a.row_ptr, a.words, a.col64, a.val64, tab, a.cb, a.x, epi);
 }
 
+// row-lane engine with V vectors per thread (V * sizeof(XT) <= 16 bytes, V <= B)
 template <typename XT, int FMT>
-static void launch_spmv_rl_t(const SpmvArgs<XT>& a, int B, hipStream_t s) {
+static void launch_spmv_rl_t(const SpmvArgs<XT>& a, int B, int V, hipStream_t s) {
     const size_t shm = FMT == BH_FMT_PACKED ? (static_cast<size_t>(1) << a.cb) * sizeof(double) : 0;
-    const int64_t rows_per_wg = BH_WG / B * 4;
+    const int64_t rows_per_wg = static_cast<int64_t>(BH_WG) * V / B * 4;
     const unsigned grid = static_cast<unsigned>((a.D + rows_per_wg - 1) / rows_per_wg);
-#define L(BB)                                                                                              \
-    if (B == BB) {                                                                                        \
-        hipLaunchKernelGGL((k_spmv_rl<XT, BB, FMT>), dim3(grid), dim3(BH_WG), shm, s, a, rows_per_wg);    \
-        return;                                                                                           \
+#define L(BB, VV)                                                                                              \
+    if (B == BB && V == VV) {                                                                                 \
+        hipLaunchKernelGGL((k_spmv_rl<XT, BB, VV, FMT>), dim3(grid), dim3(BH_WG), shm, s, a, rows_per_wg);    \
+        return;                                                                                               \
     }
-    L(1) L(2) L(4) L(8) L(16) L(32) L(64)
+    L(1, 1) L(2, 1) L(4, 1) L(8, 1) L(16, 1) L(32, 1) L(64, 1)
+    L(2, 2) L(4, 2) L(8, 2) L(16, 2) L(32, 2) L(64, 2)
+    if constexpr (sizeof(XT) == 4) {
+        L(4, 4) L(8, 4) L(16, 4) L(32, 4) L(64, 4)
+    }
 #undef L
 }
 
@@ -110,31 +121,77 @@ void bh_launch_spmv_rb(const SpmvArgs<double>& a, int B, int fmt, const int32_t*
     else launch_spmv_rb_t<double, BH_FMT_CSR64>(a, B, rb, nb, s);
 }
 
-// one operator application with the engine chosen for (h, B)
-int bh_apply(bh_ctx* ctx, bh_ham* h, const SpmvArgs<double>* a64, const SpmvArgs<float>* a32, int fmt, int B,
-             hipStream_t s) {
-    const int eng = bh_engine_choice(B, h);
-    if (eng == 2) {
-        if (a64) {
-            if (fmt == BH_FMT_PACKED) launch_spmv_rl_t<double, BH_FMT_PACKED>(*a64, B, s);
-            else launch_spmv_rl_t<double, BH_FMT_CSR64>(*a64, B, s);
-        } else {
-            launch_spmv_rl_t<float, BH_FMT_PACKED>(*a32, B, s);
-        }
-        return BH_OK;
-    }
-    if (eng == 1) {
+// SpMV variants: 0 = LDS-staged row blocks; V = 1, 2, 4 = row-lane engine with V vectors per thread
+// (64 V / B rows per wavefront); 100 + G = G lanes per row (64 / (G B) rows per wavefront)
+static int spmv_launch(bh_ham* h, int var, const SpmvArgs<double>* a64, const SpmvArgs<float>* a32, int fmt, int B,
+                       hipStream_t s) {
+    if (var == 0) {
         const int32_t* rb = nullptr;
         int64_t nb = 0;
         BH_TRY(bh_ham_rowblocks(h, B, &rb, &nb));
         if (a64) bh_launch_spmv_rb(*a64, B, fmt, rb, nb, s);
         else bh_launch_spmv_rb(*a32, B, rb, nb, s);
+    } else if (var >= 100) {
+        if (a64) bh_launch_spmv(*a64, B, var - 100, fmt, s);
+        else bh_launch_spmv(*a32, B, var - 100, s);
+    } else if (a64) {
+        if (fmt == BH_FMT_PACKED) launch_spmv_rl_t<double, BH_FMT_PACKED>(*a64, B, var, s);
+        else launch_spmv_rl_t<double, BH_FMT_CSR64>(*a64, B, var, s);
     } else {
-        const int G = bh_pick_lanes(h, B);
-        if (a64) bh_launch_spmv(*a64, B, G, fmt, s);
-        else bh_launch_spmv(*a32, B, G, s);
+        launch_spmv_rl_t<float, BH_FMT_PACKED>(*a32, B, var, s);
     }
     return BH_OK;
+}
+
+// One operator application with the variant chosen for (h, precision, B).  The first application at a
+// (precision, B) runs the autotuner unless BHX_TUNE=0 or BHX_ENGINE is set: every variant that fits is
+// timed with HIP events over a warm-up and three applications to the caller's own x and y (y = H x is
+// idempotent), and the fastest is kept in the handle.  Under stream capture the default variant is used.
+int bh_apply(bh_ctx* ctx, bh_ham* h, const SpmvArgs<double>* a64, const SpmvArgs<float>* a32, int fmt, int B,
+             hipStream_t s) {
+    const int prec = a32 ? BH_PREC_MIXED : (fmt == BH_FMT_CSR64 ? BH_PREC_FP64 : 2);
+    const int key = bh_tune_key(1, prec, B);
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    auto it = h->tuned.find(key);
+    if (it != h->tuned.end()) return spmv_launch(h, it->second.variant, a64, a32, fmt, B, s);
+    const int eng = bh_engine_choice(B, h);
+    BhTuned t;
+    t.variant = eng == 2 ? 1 : (eng == 1 ? 0 : 100 + bh_pick_lanes(h, B));
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    BH_HIP(ctx, hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone) return spmv_launch(h, t.variant, a64, a32, fmt, B, s);
+    if (bh_tune_enabled() && bh_engine_env() < 0) {
+        std::vector<int> cand;
+        if (static_cast<int64_t>(h->max_row) * B <= BH_STAGE_PRODUCTS_HOST) cand.push_back(0);
+        cand.push_back(1);
+        if (B >= 2) cand.push_back(2);
+        if (B >= 4 && a32) cand.push_back(4);
+        const int G = bh_pick_lanes(h, B);
+        if (G > 1) cand.push_back(100 + G);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        BH_HIP(ctx, hipEventCreate(&e0));
+        BH_HIP(ctx, hipEventCreate(&e1));
+        double best = 0.0;
+        for (int var : cand) {
+            BH_TRY(spmv_launch(h, var, a64, a32, fmt, B, s));
+            BH_HIP(ctx, hipEventRecord(e0, s));
+            for (int r = 0; r < 3; ++r) BH_TRY(spmv_launch(h, var, a64, a32, fmt, B, s));
+            BH_HIP(ctx, hipEventRecord(e1, s));
+            BH_HIP(ctx, hipEventSynchronize(e1));
+            float ms = 0.f;
+            BH_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+            const double us = 1e3 * ms / 3;
+            t.cand.push_back({var, us});
+            if (best == 0.0 || us < best) {
+                best = us;
+                t.variant = var;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    h->tuned[key] = t;
+    return spmv_launch(h, t.variant, a64, a32, fmt, B, s);
 }
 
 template <typename XT, int FMT>

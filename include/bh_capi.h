@@ -120,6 +120,15 @@ int bh_ham_copy_csr(const bh_ham* h, int64_t* row_ptr, int32_t* col, double* val
 int bh_ham_copy_packed(const bh_ham* h, uint32_t* words, int* code_bits);
 /* Materialise the FP64 CSR (needed once before BH_PREC_FP64 use; done lazily otherwise). */
 int bh_ham_prepare(bh_ham* h, int prec);
+/* No counterpart in the reference (its product has one fixed kernel): the operator-kernel variant the
+ * autotuner chose for this sector at (kind, prec, B), once the sector has been used that way (kind 0: the
+ * Lanczos operator pass at engine width B, kind 1: bh_spmv).  *variant: 0 = LDS-staged row blocks;
+ * V = 1, 2, 4 = one row per B / V threads holding V vectors each (64 V / B rows per wavefront);
+ * 100 + G = G threads per (row, vector).  cand_var / cand_us (capacity cap, may be NULL) receive each
+ * timed candidate and its microseconds per launch, *ncand their number (0 when the default variant was
+ * kept untimed: BHX_TUNE=0, BHX_ENGINE set, or a single candidate).  BH_EINVAL when not used yet. */
+int bh_ham_tuned(const bh_ham* h, int kind, int prec, int B, int* variant, int* ncand, int* cand_var, double* cand_us,
+                 int cap);
 
 /* ---- operator application ---------------------------------------------------------------------- */
 /* Replaces the sparse product of SparseGenMatProd::perform_op
