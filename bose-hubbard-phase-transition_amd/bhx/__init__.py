@@ -125,6 +125,7 @@ def lib():
                                         C.POINTER(C.c_double)]),
             "bh_ham_tuned": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                        C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int]),
+            "bh_ham_set_tuned": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -342,6 +343,10 @@ class Hamiltonian:
         if lib().bh_ham_tuned(self.h, kind, prec, B, C.byref(v), C.byref(n), cv, cu, 8) != OK:
             return None
         return {"variant": v.value, "candidates": {cv[i]: cu[i] for i in range(min(n.value, 8))}}
+
+    def set_tuned(self, kind: int, prec: int, B: int, variant: int):
+        """Pins the operator variant for (kind, precision, width) instead of the autotuner's choice."""
+        self.ctx.check(lib().bh_ham_set_tuned(self.h, kind, prec, B, variant))
 
     def lanczos(self, J, U, k: int = 4, prec: int = PREC_MIXED, **kw) -> LanczosOut:
         J = np.atleast_1d(np.asarray(J, np.float64))

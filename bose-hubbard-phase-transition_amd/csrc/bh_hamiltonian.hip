@@ -21,6 +21,7 @@
 #include <cstring>
 
 #include "bh_scan.hpp"
+#include "bh_spmv.hpp"
 
 constexpr int HB_BLOCK = 64;  // one wave per block: per-thread LDS row buffers stay small
 
@@ -202,6 +203,26 @@ extern "C" int bh_ham_tuned(const bh_ham* h, int kind, int prec, int B, int* var
         if (cand_var) cand_var[i] = it->second.cand[i].first;
         if (cand_us) cand_us[i] = it->second.cand[i].second;
     }
+    return BH_OK;
+}
+
+extern "C" int bh_ham_set_tuned(bh_ham* h, int kind, int prec, int B, int variant) {
+    if (!h || kind < 0 || kind > 1) return BH_EINVAL;
+    if (!bh_is_pow2_le64(B) || (prec != BH_PREC_FP64 && prec != BH_PREC_MIXED))
+        return bh_set_error(h->ctx, BH_EINVAL, "set_tuned: B a power of two <= 64, precision FP64 or MIXED");
+    bool ok = false;
+    if (variant == 0) ok = static_cast<int64_t>(h->max_row) * B <= BH_STAGE_PRODUCTS_HOST;
+    else if (variant == 1 || variant == 2) ok = variant <= B;
+    else if (variant == 4) ok = B >= 4 && prec == BH_PREC_MIXED;
+    else if (variant > 100 && kind == 1) {
+        const int G = variant - 100;
+        ok = (G & (G - 1)) == 0 && G * B <= 64;
+    }
+    if (!ok) return bh_set_error(h->ctx, BH_EINVAL, "set_tuned: variant does not fit this sector and width");
+    std::lock_guard<std::recursive_mutex> lk(h->ctx->mu);
+    BhTuned t;
+    t.variant = variant;
+    h->tuned[bh_tune_key(kind, prec, B)] = t;
     return BH_OK;
 }
 

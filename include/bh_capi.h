@@ -129,6 +129,11 @@ int bh_ham_prepare(bh_ham* h, int prec);
  * kept untimed: BHX_TUNE=0, BHX_ENGINE set, or a single candidate).  BH_EINVAL when not used yet. */
 int bh_ham_tuned(const bh_ham* h, int kind, int prec, int B, int* variant, int* ncand, int* cand_var, double* cand_us,
                  int cap);
+/* No counterpart in the reference: pins the variant used for (kind, prec, B) on this sector instead of the
+ * autotuner's choice (engines already running keep theirs).  BH_EINVAL when the variant does not fit:
+ * 0 needs the longest row times B within the 2048-product stage; V in {1, 2, 4} needs V <= B and
+ * 4 * V <= 16 / element size (V = 4 with FP32 vectors only); 100 + G (kind 1 only) needs G B <= 64. */
+int bh_ham_set_tuned(bh_ham* h, int kind, int prec, int B, int variant);
 
 /* ---- operator application ---------------------------------------------------------------------- */
 /* Replaces the sparse product of SparseGenMatProd::perform_op
