@@ -226,6 +226,24 @@ def main():
                                    "tuned": h3.tuned(1, prec, 1)}
         spmv3["D"], spmv3["nnz"] = h3.D, h3.nnz
         log("spmv 4x4 n=10:", json.dumps(spmv3))
+        # the 4x4 workload itself: J/U = 0.05 and 0.2 solved together in the FP32-mixed form (k = 4); its
+        # operator pass K1 at width 2 is this sector's SpMV inside the solver
+        J3 = np.array([0.05, 0.2])
+        h3.lanczos(J3, 1.0, k=4, prec=bhx.PREC_MIXED)
+        ctx.set_timing(True)
+        ctx.kernel_stats("lanczos_spmv", reset=True)
+        t3 = time.perf_counter()
+        r3 = h3.lanczos(J3, 1.0, k=4, prec=bhx.PREC_MIXED)
+        t3 = time.perf_counter() - t3
+        ctx.set_timing(False)
+        c3, tot3 = ctx.kernel_stats("lanczos_spmv")
+        k1b3 = 4.0 * h3.nnz + 4.0 * (h3.D + 1) + 4.0 * h3.D + 2 * h3.D * (4 + 4 + 8)
+        k1us3 = tot3 / max(c3, 1) * 1e3
+        spmv3["lanczos_mixed_B2"] = {"s": t3, "iters": r3.iters.tolist(), "E0": r3.evals[:, 0].tolist(),
+                                     "fp64_reruns": int(r3.used_fp64.sum()), "k1_us": k1us3,
+                                     "k1_alg_GBs": k1b3 / k1us3 / 1e3, "k1_frac_of_peak": k1b3 / k1us3 / 1e3 / HBM_PEAK_GBS,
+                                     "tuned": h3.tuned(0, bhx.PREC_MIXED, 2)}
+        log("lanczos 4x4 n=10:", json.dumps(spmv3["lanczos_mixed_B2"]))
         h3.close()
         b3.close()
 

@@ -214,6 +214,7 @@ extern "C" int bh_ham_set_tuned(bh_ham* h, int kind, int prec, int B, int varian
     if (variant == 0) ok = static_cast<int64_t>(h->max_row) * B <= BH_STAGE_PRODUCTS_HOST;
     else if (variant == 1 || variant == 2) ok = variant <= B;
     else if (variant == 4) ok = B >= 4 && prec == BH_PREC_MIXED;
+    else if (variant == 8) ok = kind == 0 && B >= 8 && prec == BH_PREC_MIXED;
     else if (variant > 100 && kind == 1) {
         const int G = variant - 100;
         ok = (G & (G - 1)) == 0 && G * B <= 64;

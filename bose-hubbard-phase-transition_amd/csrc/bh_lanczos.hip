@@ -158,18 +158,27 @@ __device__ __forceinline__ void lz_block_put_v(const double (&x)[V], int q, doub
     __syncthreads();
 }
 
+// Per-lane scalars of a K1 step, staged in LDS at the start of the kernel (registers stay free for the
+// gathers): s_j, J_b s_j, U_b, beta_j s_{j-1}, the offset of v_{j-1} (slot j - 1; slot 0 when j = 0) and
+// whether the lane takes the step.
+template <int B>
+struct K1Shared {
+    double sj[B], jbs[B], ub[B], bj[B];
+    int64_t om[B];
+    int act[B];
+};
+
 // K1 epilogue shared by both operator engines: a thread holds the V vectors b0 .. b0 + V - 1 of its rows.
 // w = J_b T x_b + U_b V x_b - beta_j v_{j-1} (x_b = s_j v_j, the stored vector times its scale) is written
 // to W; the thread accumulates <x_b, w> for alpha.
 template <typename XT, int B, int V, int FMT>
 struct K1Epi {
     const LzArgs<XT>& a;
+    const K1Shared<B>& ls;
     int b0 = 0;
-    double sj[V], bj[V], Jb[V], Ub[V];
-    int64_t om[V];  // offset of lane b0 + v's v_{j-1} in the basis (slot j - 1; slot 0 when j = 0)
-    bool act[V];
+    int64_t om0 = 0;    // the common v_{j-1} offset when `same`
     bool any = false;   // some lane of the thread takes this step
-    bool same = true;   // every lane reads v_{j-1} from one slot (one vector load)
+    bool same = true;   // every lane that takes the step reads v_{j-1} from one slot (one vector load)
     double pa[V];
     __device__ __forceinline__ void operator()(int64_t r, int, const double (&acc)[V]) {
         if (!any) return;  // the row-block engine calls every thread
@@ -178,48 +187,53 @@ struct K1Epi {
         const BhVec<XT, V> xv = bh_vload<V>(a.Xc + idx);
         BhVec<XT, V> xm;
         if (same) {
-            xm = bh_vload<V>(a.V + om[0] + idx);
+            xm = bh_vload<V>(a.V + om0 + idx);
         } else {
 #pragma unroll
-            for (int v = 0; v < V; ++v) xm.v[v] = a.V[om[v] + idx + v];
+            for (int v = 0; v < V; ++v) xm.v[v] = a.V[ls.om[b0 + v] + idx + v];
         }
         BhVec<double, V> wv;
 #pragma unroll
         for (int v = 0; v < V; ++v) {
+            const int b = b0 + v;
             const double hx = (FMT == BH_FMT_PACKED) ? -acc[v] : acc[v];
-            const double xr = to_d(xv.v[v]) * sj[v];
-            const double w = Jb[v] * hx * sj[v] + Ub[v] * d * xr - bj[v] * to_d(xm.v[v]);
+            const double xr = to_d(xv.v[v]) * ls.sj[b];
+            const double w = ls.jbs[b] * hx + ls.ub[b] * d * xr - ls.bj[b] * to_d(xm.v[v]);
             wv.v[v] = w;
-            if (act[v]) pa[v] += xr * w;
+            if (ls.act[b]) pa[v] += xr * w;
         }
         *reinterpret_cast<BhVec<double, V>*>(a.W + idx) = wv;
     }
 };
 
+// fills the LDS lane table (threads < B) and the thread's view of its V lanes; ends with a barrier
 template <typename XT, int B, int V, int FMT>
-__device__ __forceinline__ void k1_setup(const LzArgs<XT>& a, K1Epi<XT, B, V, FMT>& e) {
+__device__ __forceinline__ void k1_setup(const LzArgs<XT>& a, K1Shared<B>& ls, K1Epi<XT, B, V, FMT>& e) {
+    if (threadIdx.x < B) {
+        const int b = threadIdx.x;
+        const int j = a.st.step[b];
+        const double sj = a.st.scale[j * B + b];
+        ls.sj[b] = sj;
+        ls.jbs[b] = a.st.J[b] * sj;
+        ls.ub[b] = a.st.U[b];
+        ls.bj[b] = j > 0 ? a.st.beta[j * B + b] * a.st.scale[(j - 1) * B + b] : 0.0;
+        ls.om[b] = static_cast<int64_t>(j > 0 ? j - 1 : 0) * a.vs;
+        ls.act[b] = a.st.active[b] != 0 && j < a.maxit;
+    }
+    __syncthreads();
     e.b0 = (threadIdx.x % (B / V)) * V;  // fixed per thread: LZ_WG is a multiple of B
     int64_t om0 = -1;
 #pragma unroll
     for (int v = 0; v < V; ++v) {
-        const int b = e.b0 + v;
-        const int j = a.st.step[b];
-        e.act[v] = a.st.active[b] != 0 && j < a.maxit;
-        e.sj[v] = a.st.scale[j * B + b];
-        e.bj[v] = j > 0 ? a.st.beta[j * B + b] * a.st.scale[(j - 1) * B + b] : 0.0;
-        e.Jb[v] = a.st.J[b];
-        e.Ub[v] = a.st.U[b];
-        e.om[v] = static_cast<int64_t>(j > 0 ? j - 1 : 0) * a.vs;
         e.pa[v] = 0.0;
-        e.any = e.any || e.act[v];
-        if (e.act[v] && om0 < 0) om0 = e.om[v];
+        if (ls.act[e.b0 + v]) {
+            e.any = true;
+            if (om0 < 0) om0 = ls.om[e.b0 + v];
+            e.same = e.same && ls.om[e.b0 + v] == om0;
+        }
     }
-    // lanes that do not take the step read (and discard) the v_{j-1} slot of the active ones
-#pragma unroll
-    for (int v = 0; v < V; ++v) {
-        if (!e.act[v] && om0 >= 0) e.om[v] = om0;
-        e.same = e.same && e.om[v] == e.om[0];
-    }
+    // lanes that do not take the step read (and discard) the active lanes' v_{j-1} slot
+    e.om0 = om0 < 0 ? 0 : om0;
 }
 
 // K1 on the LDS-staged row-block engine (B = 1 in the automatic choice)
@@ -233,8 +247,9 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_a_rb(LzArgs<XT> a, const int32_t* 
         for (int i = threadIdx.x; i < (1 << a.cb); i += LZ_WG) tab[i] = a.table[i];
         __syncthreads();
     }
-    K1Epi<XT, B, 1, FMT> e{a};
-    k1_setup(a, e);
+    __shared__ K1Shared<B> ls;
+    K1Epi<XT, B, 1, FMT> e{a, ls};
+    k1_setup(a, ls, e);
     const int64_t per = (nblocks + gridDim.x - 1) / gridDim.x;
     const int64_t L = bh_xcd_tile(blockIdx.x, gridDim.x);
     const int64_t t0 = min(L * per, nblocks), t1 = min(t0 + per, nblocks);
@@ -244,20 +259,33 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_a_rb(LzArgs<XT> a, const int32_t* 
 
 // K1 on the row-lane engine (B / V threads per row, contiguous row ranges per workgroup)
 template <typename XT, int B, int V, int FMT>
-__global__ void __launch_bounds__(LZ_WG) k_lz_a_rl(LzArgs<XT> a, int64_t rows_per_wg, int nv) {
+__device__ __forceinline__ void k1_rowlane(const LzArgs<XT>& a, int64_t rows_per_wg, int nv) {
     extern __shared__ double tab[];
     __shared__ double lds[LZ_WG / 64 * B];
     if constexpr (FMT == BH_FMT_PACKED) {
         for (int i = threadIdx.x; i < (1 << a.cb); i += LZ_WG) tab[i] = a.table[i];
-        __syncthreads();
     }
-    K1Epi<XT, B, V, FMT> e{a};
-    k1_setup(a, e);
+    __shared__ K1Shared<B> ls;
+    K1Epi<XT, B, V, FMT> e{a, ls};
+    k1_setup(a, ls, e);
     const int64_t L = bh_xcd_tile(blockIdx.x, gridDim.x);
     const int64_t r0 = min(L * rows_per_wg, a.D), r1 = min(r0 + rows_per_wg, a.D);
     if (e.any)
         bh_rowlane_run<XT, B, V, FMT, LZ_WG, LZ_K1_U>(r0, r1, a.row_ptr, a.words, a.col64, a.val64, tab, a.cb, a.Xc, e);
     lz_block_put_v<B, V>(e.pa, 0, lds, a.st.part, nv);
+}
+
+// V <= 4: at most 80 VGPRs (six waves per SIMD)
+template <typename XT, int B, int V, int FMT>
+__global__ void __launch_bounds__(LZ_WG) __attribute__((amdgpu_waves_per_eu(6))) k_lz_a_rl(LzArgs<XT> a, int64_t rows_per_wg,
+                                                                                            int nv) {
+    k1_rowlane<XT, B, V, FMT>(a, rows_per_wg, nv);
+}
+
+// V = 8 (two 16-byte gathers per entry and thread)
+template <typename XT, int B, int V, int FMT>
+__global__ void __launch_bounds__(LZ_WG) k_lz_a_rl8(LzArgs<XT> a, int64_t rows_per_wg, int nv) {
+    k1_rowlane<XT, B, V, FMT>(a, rows_per_wg, nv);
 }
 
 // Dot products of an exact orthogonalisation step (so = 1): partials of <y_l, w> and <y_l, s_j v_j> for the
@@ -1597,7 +1625,7 @@ class LzEngine final : public LzEngineBase {
         grid_k1 = var == 0 ? grid_rb : static_cast<int>((D + k1_rows(var) - 1) / k1_rows(var));
         rpw_so = (D + grid_k1 - 1) / grid_k1;
     }
-    // K1 variant `var` (0: staged row blocks; V = 1, 2, 4: row-lane engine, V vectors per thread)
+    // K1 variant `var` (0: staged row blocks; V = 1, 2, 4, 8: row-lane engine, V vectors per thread)
     void launch_k1_var(int var, int nv) {
         if (var == 0) {
             hipLaunchKernelGGL((k_lz_a_rb<XT, B, FMT>), dim3(grid_rb), dim3(LZ_WG), shm_rb, s, a, rblk, nrblk, nv);
@@ -1605,6 +1633,12 @@ class LzEngine final : public LzEngineBase {
         }
         const int64_t rows = k1_rows(var);
         const dim3 g(static_cast<unsigned>((D + rows - 1) / rows));
+        if constexpr (B >= 8 && sizeof(XT) == 4) {
+            if (var == 8) {
+                hipLaunchKernelGGL((k_lz_a_rl8<XT, B, 8, FMT>), g, dim3(LZ_WG), shm, s, a, rows, nv);
+                return;
+            }
+        }
         if constexpr (B >= 4 && sizeof(XT) * 4 <= 16) {
             if (var == 4) {
                 hipLaunchKernelGGL((k_lz_a_rl<XT, B, 4, FMT>), g, dim3(LZ_WG), shm, s, a, rows, nv);
@@ -1652,6 +1686,7 @@ class LzEngine final : public LzEngineBase {
         cand.push_back(1);
         if (B >= 2) cand.push_back(2);
         if (B >= 4 && sizeof(XT) * 4 <= 16) cand.push_back(4);
+        if (B >= 8 && sizeof(XT) == 4) cand.push_back(8);
         BhTuned t;
         t.variant = k1v;
         if (bh_tune_enabled() && cand.size() > 1) {

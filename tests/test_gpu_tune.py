@@ -1,13 +1,13 @@
 """Operator-kernel variants and their autotuner.
 
-Every variant the autotuner can choose (LDS-staged row blocks; B / V threads per row with V vectors each;
-G threads per (row, vector)) is pinned in turn and checked against the oracle's product, with the
+Every variant the autotuner can choose (LDS-staged row blocks; B / V threads per row with V vectors each,
+V = 1, 2, 4 and, in the Lanczos pass, 8; G threads per (row, vector)) is pinned in turn and checked against the oracle's product, with the
 tolerances of the operator tests: FP64 within 1e-13, FP32-stored vectors within 2e-6 relative.  The
 row-lane variants sum every (row, vector) over its entries in the same order with the same arithmetic,
 so their products must agree bitwise with each other.  In a Lanczos solve the variants also partition the
 rows differently among workgroups, which changes the order of the alpha partial sums: whole solves with
-each variant pinned must give the same levels within 1e-9 relative (the solver's accuracy, after its FP64
-finish).  The autotuner's record must name the fastest of the candidates it timed."""
+each variant pinned must give the same levels to the solver's accuracy (E0 within 1e-9 relative, the levels
+above it within 1e-9 in FP64 and 1e-5 with FP32-stored vectors).  The autotuner's record must name the fastest of the candidates it timed."""
 import numpy as np
 import pytest
 
@@ -29,7 +29,7 @@ def ctx():
 def variants(h, kind, prec, B):
     """every variant bh_ham_set_tuned accepts for this sector and width"""
     out = []
-    for v in (0, 1, 2, 4) + ((102, 104, 108, 116) if kind == SPMV else ()):
+    for v in (0, 1, 2, 4) + ((102, 104, 108, 116) if kind == SPMV else (8,)):
         try:
             h.set_tuned(kind, prec, B, v)
             out.append(v)
@@ -117,10 +117,17 @@ def test_lanczos_k1_variants(ctx, prec):
         assert h.tuned(K1, prec, 8)["variant"] == v
         assert np.all(r.status == 0), v
         res[v] = r
-    assert {1, 2} <= set(res) and ((4 in res) == (prec == bhx.PREC_MIXED))
-    scale = np.abs(res[1].evals).max()
+    assert {1, 2} <= set(res) and ((4 in res) == (8 in res) == (prec == bhx.PREC_MIXED))
+    # the accuracy bar of the Lanczos tests: E0 within 1e-9 relative; the levels above it within 1e-9 in FP64
+    # and 1e-5 with FP32-stored vectors (lanes rerun in FP64 are held to 1e-9)
+    def close(e, want, fp64):
+        err = np.abs(e - want) / np.abs(want)
+        return err[0] <= 1e-9 and np.all(err[1:] <= (1e-9 if fp64 else 1e-5))
+
     for v, r in res.items():
-        assert np.allclose(r.evals, res[1].evals, rtol=1e-9, atol=1e-9 * scale), v
-    assert np.allclose(res[1].evals[3], ref, rtol=1e-9, atol=1e-9)
+        for i in range(len(J)):
+            both64 = prec == bhx.PREC_FP64 or (r.used_fp64[i] and res[1].used_fp64[i])
+            assert close(r.evals[i], res[1].evals[i], both64), (v, i)
+    assert close(res[1].evals[3], ref, prec == bhx.PREC_FP64 or res[1].used_fp64[3])
     h.close()
     b.close()
