@@ -168,21 +168,26 @@ int bh_ham_rowblocks(bh_ham* h, int B, const int32_t** rb, int64_t* nblocks) {
     std::vector<int32_t> rp(static_cast<size_t>(h->D + 1));
     BH_HIP(ctx, hipMemcpyAsync(rp.data(), h->row_ptr, rp.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
     BH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // (first row, first entry) of every block, then (D, nnz): one load gives a block's whole extent
     std::vector<int32_t> blocks;
-    blocks.push_back(0);
+    auto add = [&](int64_t r) {
+        blocks.push_back(static_cast<int32_t>(r));
+        blocks.push_back(rp[r]);
+    };
+    add(0);
     int64_t start = 0;
     for (int64_t r = 0; r < h->D; ++r) {
         if (rp[r + 1] - rp[start] > cap) {
-            blocks.push_back(static_cast<int32_t>(r));
+            add(r);
             start = r;
         }
     }
-    blocks.push_back(static_cast<int32_t>(h->D));
+    add(h->D);
     DevArray<int32_t> d;
     BH_TRY(d.alloc(ctx, blocks.size()));
     BH_HIP(ctx, hipMemcpyAsync(d.p, blocks.data(), blocks.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     BH_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    const int64_t nb = static_cast<int64_t>(blocks.size()) - 1;
+    const int64_t nb = static_cast<int64_t>(blocks.size()) / 2 - 1;
     h->bytes_rb += blocks.size() * 4;
     h->rowblocks[B] = {d.release(), nb};
     *rb = h->rowblocks[B].first;
@@ -215,6 +220,7 @@ extern "C" int bh_ham_set_tuned(bh_ham* h, int kind, int prec, int B, int varian
     else if (variant == 1 || variant == 2) ok = variant <= B;
     else if (variant == 4) ok = B >= 4 && prec == BH_PREC_MIXED;
     else if (variant == 8) ok = kind == 0 && B >= 8 && prec == BH_PREC_MIXED;
+    else if (variant == 17 || variant == 33 || variant == 18 || variant == 34) ok = kind == 0 && (variant & 15) <= B && B <= 16;
     else if (variant > 100 && kind == 1) {
         const int G = variant - 100;
         ok = (G & (G - 1)) == 0 && G * B <= 64;
@@ -246,7 +252,7 @@ int bh_engine_choice(int B, const bh_ham* h) {
 
 static int free_ham(bh_ham* h) {
     bh_ctx* ctx = h->ctx;
-    for (auto& kv : h->rowblocks) bh_dev_free(ctx, kv.second.first, static_cast<size_t>(kv.second.second + 1) * 4);
+    for (auto& kv : h->rowblocks) bh_dev_free(ctx, kv.second.first, static_cast<size_t>(kv.second.second + 1) * 8);
     const size_t tab = (static_cast<size_t>(1) << h->code_bits);
     bh_dev_free(ctx, h->row_ptr, static_cast<size_t>(h->D + 1) * sizeof(int32_t));
     bh_dev_free(ctx, h->words, static_cast<size_t>(h->nnz ? h->nnz : 1) * sizeof(uint32_t));

@@ -151,7 +151,8 @@ struct bh_ham {
     // spectral bounds: V in [vmin, vmax]; the largest row sum of |T| (Gershgorin radius of T)
     double vmin = 0.0, vmax = 0.0, tmax = 0.0;
     std::vector<double> vlevels;  // distinct values of V ascending (host, filled on first use)
-    // row blocks of the LDS-staged engine, per batch width B: first row of each block, then D
+    // row blocks of the LDS-staged engine, per batch width B: (first row, first entry) of each block, then
+    // (D, nnz)
     std::map<int, std::pair<int32_t*, int64_t>> rowblocks;
     size_t bytes_rb = 0;
     // operator-kernel variants chosen by the autotuner, keyed by bh_tune_key(kind, prec, B)

@@ -104,8 +104,9 @@ __device__ __forceinline__ int64_t bh_xcd_tile(int64_t wg, int64_t nwg) {
 // A row block is a run of consecutive rows holding at most BH_STAGE_PRODUCTS / B entries.  A workgroup
 // takes one block at a time: phase 1 streams the block's entries (coalesced, independent loads and
 // gathers, several per lane in flight) and writes the products value * x[col * B + b] to LDS; phase 2
-// gives each (row, b) pair to one thread, which sums its row's products from LDS in column order and
-// hands the sum to the epilogue (as an array of one, the epilogue interface of the row-lane engine).  rb[] holds the first row of every block (and D at the end).
+// gives each (row, b) pair to one thread, which sums its row's products from LDS (four interleaved
+// partial sums in a fixed order) and hands the sum to the epilogue (as an array of one, the epilogue
+// interface of the row-lane engine).  rb[] holds (first row, first entry) of every block, then (D, nnz).
 constexpr int BH_STAGE_PRODUCTS = 2048;  // FP64 products per block (16 KB of LDS)
 
 template <typename XT, int B, int FMT, int WG, typename Epi>
@@ -118,8 +119,8 @@ __device__ __forceinline__ void bh_rowblock_run(int64_t t0, int64_t t1, const in
     constexpr int PER = CAP / WG;           // products per thread in phase 1
     const uint32_t mask = (1u << cb) - 1u;
     for (int64_t t = t0; t < t1; ++t) {
-        const int32_t r0 = rb[t], r1 = rb[t + 1];
-        const int32_t ks = rp[r0], ke = rp[r1];
+        const int2 b0 = *reinterpret_cast<const int2*>(rb + 2 * t), b1 = *reinterpret_cast<const int2*>(rb + 2 * t + 2);
+        const int32_t r0 = b0.x, ks = b0.y, r1 = b1.x, ke = b1.y;
         const int np = (ke - ks) * B;
         double v[PER];
 #pragma unroll
@@ -148,8 +149,17 @@ __device__ __forceinline__ void bh_rowblock_run(int64_t t0, int64_t t1, const in
             const int32_t r = r0 + i / B;
             const int bb = i % B;
             const int32_t a = rp[r] - ks, e = rp[r + 1] - ks;
-            double sum[1] = {0.0};
-            for (int32_t q = a; q < e; ++q) sum[0] += prod[q * B + bb];
+            // four interleaved partial sums (independent LDS reads), combined in a fixed order
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            int32_t q = a;
+            for (; q + 3 < e; q += 4) {
+                s0 += prod[q * B + bb];
+                s1 += prod[(q + 1) * B + bb];
+                s2 += prod[(q + 2) * B + bb];
+                s3 += prod[(q + 3) * B + bb];
+            }
+            for (; q < e; ++q) s0 += prod[q * B + bb];
+            double sum[1] = {(s0 + s1) + (s2 + s3)};
             epi(static_cast<int64_t>(r), bb, sum);
         }
         __syncthreads();

@@ -45,9 +45,9 @@ constexpr int LZ_NVMAX = 16;       // Ritz vectors formed per pass
 constexpr int LZ_RED_WG = 1024;    // reduction workgroup: 16 waves, one value per wave
 constexpr int LZ_GRAPH_STEPS = 8;  // Lanczos steps per captured HIP graph
 constexpr int LZ_K1_TILES = 4;         // row groups per K1 workgroup on the row-lane engine
-constexpr int LZ_K1_U = 4;             // entries of a row in flight per thread in K1 (row-lane engine)
 constexpr int LZ_TUNE_REPS = 3;        // timed launches per K1 variant in the autotuner
 constexpr int LZ_CHECK_MIN = 4;        // fewest steps between two analyses of a lane
+constexpr int64_t LZ_PART_BYTES = 64ll << 20;  // partial sums of the staged K1 (more workgroups than grid_rl)
 constexpr int LZ_K1_GRID_MAX = 32768;  // K1 workgroups at most (rows of partial sums)
 constexpr double LZ_ACCEPT_LEVEL = 1e-6;  // mixed: bound on ||r||^2/gap of the levels above E0, relative
 
@@ -258,7 +258,7 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_a_rb(LzArgs<XT> a, const int32_t* 
 }
 
 // K1 on the row-lane engine (B / V threads per row, contiguous row ranges per workgroup)
-template <typename XT, int B, int V, int FMT>
+template <typename XT, int B, int V, int FMT, int U>
 __device__ __forceinline__ void k1_rowlane(const LzArgs<XT>& a, int64_t rows_per_wg, int nv) {
     extern __shared__ double tab[];
     __shared__ double lds[LZ_WG / 64 * B];
@@ -271,21 +271,22 @@ __device__ __forceinline__ void k1_rowlane(const LzArgs<XT>& a, int64_t rows_per
     const int64_t L = bh_xcd_tile(blockIdx.x, gridDim.x);
     const int64_t r0 = min(L * rows_per_wg, a.D), r1 = min(r0 + rows_per_wg, a.D);
     if (e.any)
-        bh_rowlane_run<XT, B, V, FMT, LZ_WG, LZ_K1_U>(r0, r1, a.row_ptr, a.words, a.col64, a.val64, tab, a.cb, a.Xc, e);
+        bh_rowlane_run<XT, B, V, FMT, LZ_WG, U>(r0, r1, a.row_ptr, a.words, a.col64, a.val64, tab, a.cb, a.Xc, e);
     lz_block_put_v<B, V>(e.pa, 0, lds, a.st.part, nv);
 }
 
-// V <= 4: at most 80 VGPRs (six waves per SIMD)
+// V <= 4, U = 4: at most 80 VGPRs (six waves per SIMD)
 template <typename XT, int B, int V, int FMT>
 __global__ void __launch_bounds__(LZ_WG) __attribute__((amdgpu_waves_per_eu(6))) k_lz_a_rl(LzArgs<XT> a, int64_t rows_per_wg,
                                                                                             int nv) {
-    k1_rowlane<XT, B, V, FMT>(a, rows_per_wg, nv);
+    k1_rowlane<XT, B, V, FMT, 4>(a, rows_per_wg, nv);
 }
 
-// V = 8 (two 16-byte gathers per entry and thread)
-template <typename XT, int B, int V, int FMT>
-__global__ void __launch_bounds__(LZ_WG) k_lz_a_rl8(LzArgs<XT> a, int64_t rows_per_wg, int nv) {
-    k1_rowlane<XT, B, V, FMT>(a, rows_per_wg, nv);
+// wide variants: V = 8 (two 16-byte gathers per entry and thread) or U = 8, 16 entries of a row in flight
+// (long rows at small B)
+template <typename XT, int B, int V, int FMT, int U>
+__global__ void __launch_bounds__(LZ_WG) k_lz_a_rlw(LzArgs<XT> a, int64_t rows_per_wg, int nv) {
+    k1_rowlane<XT, B, V, FMT, U>(a, rows_per_wg, nv);
 }
 
 // Dot products of an exact orthogonalisation step (so = 1): partials of <y_l, w> and <y_l, s_j v_j> for the
@@ -970,7 +971,13 @@ class LzEngine final : public LzEngineBase {
         // workgroups in flight; its partial sums need a row per workgroup
         rpw = std::max<int64_t>(static_cast<int64_t>(LZ_WG / B) * LZ_K1_TILES, (D + LZ_K1_GRID_MAX - 1) / LZ_K1_GRID_MAX);
         grid_rl = static_cast<int>((D + rpw - 1) / rpw);
-        BH_TRY(dalloc(ctx, part, static_cast<size_t>(std::max(LZ_GRID_MAX, grid_rl)) * nvmax));
+        // K1 on the staged row-block engine: a workgroup per block up to LZ_K1_GRID_MAX workgroups, within
+        // LZ_PART_BYTES of partial sums
+        staged_ok = static_cast<int64_t>(h->max_row) * B <= BH_STAGE_PRODUCTS_HOST;
+        if (staged_ok) BH_TRY(bh_ham_rowblocks(h, B, &rblk, &nrblk));
+        const int64_t rb_cap = std::max<int64_t>(grid_rl, LZ_PART_BYTES / (static_cast<int64_t>(nvmax) * 8));
+        grid_rb = static_cast<int>(std::min<int64_t>({std::max<int64_t>(nrblk, 1), LZ_K1_GRID_MAX, rb_cap}));
+        BH_TRY(dalloc(ctx, part, static_cast<size_t>(std::max({LZ_GRID_MAX, grid_rl, grid_rb})) * nvmax));
         BH_TRY(dalloc(ctx, dnorm, static_cast<size_t>(2 * LZ_NVMAX * B)));
         BH_TRY(dalloc(ctx, dnj, B));
         BH_TRY(dalloc(ctx, dno, B));
@@ -1031,9 +1038,6 @@ class LzEngine final : public LzEngineBase {
                        st};
         shm = FMT == BH_FMT_PACKED ? (static_cast<size_t>(1) << h->code_bits) * sizeof(double) : 0;
         // K1 variant before the tuner has run: staged row blocks, or one thread per (row, b)
-        staged_ok = static_cast<int64_t>(h->max_row) * B <= BH_STAGE_PRODUCTS_HOST;
-        if (staged_ok) BH_TRY(bh_ham_rowblocks(h, B, &rblk, &nrblk));
-        grid_rb = static_cast<int>(std::min<int64_t>(std::max<int64_t>(nrblk, 1), LZ_GRID_MAX));
         shm_rb = shm + BH_STAGE_PRODUCTS * sizeof(double);
         set_k1(bh_engine_choice(B, h) == 1 && staged_ok ? 0 : 1);
         const int64_t ntile_b = (D + LZ_WG / B - 1) / (LZ_WG / B);
@@ -1615,17 +1619,21 @@ class LzEngine final : public LzEngineBase {
         e = std::min<int64_t>(e, static_cast<int64_t>(jl[b]) + std::max(0, maxsteps - tot[b]));
         return e;
     }
-    // rows of a K1 workgroup for variant V (row-lane engine): LZ_K1_TILES passes of LZ_WG V / B rows, more
-    // when the grid would exceed LZ_K1_GRID_MAX
-    int64_t k1_rows(int V) const {
-        return std::max<int64_t>(static_cast<int64_t>(LZ_WG * V / B) * LZ_K1_TILES, (D + LZ_K1_GRID_MAX - 1) / LZ_K1_GRID_MAX);
+    // K1 variants: 0 = staged row blocks; V + 16 u (V = 1, 2, 4, 8 vectors per thread, u = 0, 1, 2 for
+    // U = 4, 8, 16 entries of a row in flight) = row-lane engine
+    static int k1_V(int var) { return var & 15; }
+    static int k1_U(int var) { return 4 << (var >> 4); }
+    // rows of a K1 workgroup on the row-lane engine: LZ_K1_TILES passes of LZ_WG V / B rows, more when the
+    // grid would exceed LZ_K1_GRID_MAX
+    int64_t k1_rows(int var) const {
+        return std::max<int64_t>(static_cast<int64_t>(LZ_WG * k1_V(var) / B) * LZ_K1_TILES,
+                                 (D + LZ_K1_GRID_MAX - 1) / LZ_K1_GRID_MAX);
     }
     void set_k1(int var) {
         k1v = var;
         grid_k1 = var == 0 ? grid_rb : static_cast<int>((D + k1_rows(var) - 1) / k1_rows(var));
         rpw_so = (D + grid_k1 - 1) / grid_k1;
     }
-    // K1 variant `var` (0: staged row blocks; V = 1, 2, 4, 8: row-lane engine, V vectors per thread)
     void launch_k1_var(int var, int nv) {
         if (var == 0) {
             hipLaunchKernelGGL((k_lz_a_rb<XT, B, FMT>), dim3(grid_rb), dim3(LZ_WG), shm_rb, s, a, rblk, nrblk, nv);
@@ -1633,12 +1641,19 @@ class LzEngine final : public LzEngineBase {
         }
         const int64_t rows = k1_rows(var);
         const dim3 g(static_cast<unsigned>((D + rows - 1) / rows));
-        if constexpr (B >= 8 && sizeof(XT) == 4) {
-            if (var == 8) {
-                hipLaunchKernelGGL((k_lz_a_rl8<XT, B, 8, FMT>), g, dim3(LZ_WG), shm, s, a, rows, nv);
-                return;
+#define K1W(VV, UU)                                                                                    \
+    if (var == VV + 16 * (UU == 4 ? 0 : (UU == 8 ? 1 : 2))) {                                          \
+        hipLaunchKernelGGL((k_lz_a_rlw<XT, B, VV, FMT, UU>), g, dim3(LZ_WG), shm, s, a, rows, nv);     \
+        return;                                                                                        \
+    }
+        if constexpr (B >= 8 && sizeof(XT) == 4) K1W(8, 4)
+        if constexpr (B <= 16) {
+            K1W(1, 8) K1W(1, 16)
+            if constexpr (B >= 2) {
+                K1W(2, 8) K1W(2, 16)
             }
         }
+#undef K1W
         if constexpr (B >= 4 && sizeof(XT) * 4 <= 16) {
             if (var == 4) {
                 hipLaunchKernelGGL((k_lz_a_rl<XT, B, 4, FMT>), g, dim3(LZ_WG), shm, s, a, rows, nv);
@@ -1669,7 +1684,8 @@ class LzEngine final : public LzEngineBase {
     // result is kept in the sector's handle): every variant that fits is timed with HIP events over a
     // warm-up and LZ_TUNE_REPS launches on the admitted lanes (K1 writes only W and the partial sums, which
     // the real step overwrites), and the fastest is kept.  The variants differ in rows per wavefront
-    // (64 V / B on the row-lane engine) and in the staging of the gathers; each (row, b) sums its entries in
+    // (64 V / B on the row-lane engine), in the entries of a row in flight per thread and in the staging of
+    // the gathers; each (row, b) sums its entries in
     // storage order in all of them (the row-lane variants give identical sums, the staged engine rounds each
     // product before adding it).  BHX_TUNE=0 keeps the default variant.
     int tune_k1() {
@@ -1687,6 +1703,13 @@ class LzEngine final : public LzEngineBase {
         if (B >= 2) cand.push_back(2);
         if (B >= 4 && sizeof(XT) * 4 <= 16) cand.push_back(4);
         if (B >= 8 && sizeof(XT) == 4) cand.push_back(8);
+        // more entries in flight per thread for long rows at small widths
+        const double mean_row = D ? static_cast<double>(h->nnz) / static_cast<double>(D) : 0.0;
+        if (B <= 16)
+            for (int V = 1; V <= std::min(B, 2); V *= 2) {
+                if (mean_row > 6.0) cand.push_back(V + 16);
+                if (mean_row > 12.0) cand.push_back(V + 32);
+            }
         BhTuned t;
         t.variant = k1v;
         if (bh_tune_enabled() && cand.size() > 1) {

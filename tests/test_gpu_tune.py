@@ -1,7 +1,8 @@
 """Operator-kernel variants and their autotuner.
 
 Every variant the autotuner can choose (LDS-staged row blocks; B / V threads per row with V vectors each,
-V = 1, 2, 4 and, in the Lanczos pass, 8; G threads per (row, vector)) is pinned in turn and checked against the oracle's product, with the
+V = 1, 2, 4 and, in the Lanczos pass, 8, or V = 1, 2 with 8 or 16 entries of a row in flight; G threads per
+(row, vector)) is pinned in turn and checked against the oracle's product, with the
 tolerances of the operator tests: FP64 within 1e-13, FP32-stored vectors within 2e-6 relative.  The
 row-lane variants sum every (row, vector) over its entries in the same order with the same arithmetic,
 so their products must agree bitwise with each other.  In a Lanczos solve the variants also partition the
@@ -29,7 +30,7 @@ def ctx():
 def variants(h, kind, prec, B):
     """every variant bh_ham_set_tuned accepts for this sector and width"""
     out = []
-    for v in (0, 1, 2, 4) + ((102, 104, 108, 116) if kind == SPMV else (8,)):
+    for v in (0, 1, 2, 4) + ((102, 104, 108, 116) if kind == SPMV else (8, 17, 33, 18, 34)):
         try:
             h.set_tuned(kind, prec, B, v)
             out.append(v)
