@@ -247,6 +247,20 @@ def main():
         h3.close()
         b3.close()
 
+    # batched phase-diagram sweep: chain m=10 periodic, sectors n = 9, 10, 11, 32 log-spaced J/U in
+    # [0.01, 0.4], 64 mu/U in [0, 3], k = 4; block width chosen per sector by the tuner
+    sweep = None
+    if not args.quick:
+        Jsw = 0.01 * 40.0 ** (np.arange(32) / 31.0)
+        musw = 3.0 * np.arange(64) / 63.0
+        ctx.sweep("chain", 10, Jsw, 1.0, musw, m=10, k=4)
+        t0 = time.perf_counter()
+        rsw = ctx.sweep("chain", 10, Jsw, 1.0, musw, m=10, k=4)
+        tsw = time.perf_counter() - t0
+        sweep = {"s": tsw, "solves": 3 * Jsw.size, "solves_per_s": 3 * Jsw.size / tsw, "width": rsw["width"].tolist(),
+                 "fp64_reruns": int(rsw["used_fp64"].sum()), "all_ok": bool(np.all(rsw["status"] == 0))}
+        log("sweep chain m=10 n=9/10/11:", json.dumps(sweep))
+
     serve = None
     if args.serve > 0 and not args.quick:
         serve = serving(ctx, args.serve, 200.0)
@@ -277,6 +291,7 @@ def main():
         "rel_err_E0_vs_fp64": rel0,
         "spmv": spmv,
         "spmv_square4x4_n10": spmv3 or None,
+        "sweep_chain10": sweep,
         "serving": serve,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile()},

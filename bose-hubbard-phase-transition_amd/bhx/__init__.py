@@ -49,13 +49,13 @@ class LanczosOpts(C.Structure):
 class SweepSpec(C.Structure):
     _fields_ = [("kind", C.c_int), ("Lx", C.c_int), ("Ly", C.c_int), ("periodic", C.c_int), ("n", C.c_int),
                 ("nJ", C.c_int), ("J", C.c_void_p), ("U", C.c_double), ("nmu", C.c_int), ("mu", C.c_void_p),
-                ("flags", C.c_uint), ("order", C.c_int)]
+                ("flags", C.c_uint), ("order", C.c_int), ("width", C.c_int)]
 
 
 class SweepOut(C.Structure):
     _fields_ = [("E", C.c_void_p), ("gap_ratio", C.c_void_p), ("density", C.c_void_p),
                 ("compressibility", C.c_void_p), ("mu_plus", C.c_void_p), ("mu_minus", C.c_void_p),
-                ("status", C.c_void_p), ("used_fp64", C.c_void_p)]
+                ("status", C.c_void_p), ("used_fp64", C.c_void_p), ("width", C.c_void_p)]
 
 
 class Request(C.Structure):
@@ -220,12 +220,15 @@ class Context:
         return Basis(self, m, n, order)
 
     def sweep(self, geometry: str, n: int, J, U: float, mu, m: int = 0, Lx: int = 0, Ly: int = 0,
-              periodic: bool = True, k: int = 4, prec: int = PREC_MIXED, flags: int = 0, order: int = ORDER_LEX, **kw):
-        """Batched phase-diagram sweep over the sectors n-1, n, n+1 (see bh_sweep in bh_capi.h)."""
+              periodic: bool = True, k: int = 4, prec: int = PREC_MIXED, flags: int = 0, order: int = ORDER_LEX,
+              width: int = 0, **kw):
+        """Batched phase-diagram sweep over the sectors n-1, n, n+1 (see bh_sweep in bh_capi.h); width 0
+        lets the block-width tuner choose the batch width per sector (reported in res["width"])."""
         J = np.ascontiguousarray(np.atleast_1d(J), np.float64)
         mu = np.ascontiguousarray(np.atleast_1d(mu), np.float64)
         kind, a, b = (0, m, 1) if geometry == "chain" else (1, Lx, Ly)
-        sp = SweepSpec(kind, a, b, int(periodic), n, J.size, _ptr(J).value, U, mu.size, _ptr(mu).value, flags, order)
+        sp = SweepSpec(kind, a, b, int(periodic), n, J.size, _ptr(J).value, U, mu.size, _ptr(mu).value, flags, order,
+                       width)
         o = LanczosOpts()
         lib().bh_lanczos_default_opts(C.byref(o))
         o.k, o.prec = k, prec
@@ -235,9 +238,10 @@ class Context:
         res = {"E": np.full((3, nJ, k), np.nan), "gap_ratio": np.full((3, nJ), np.nan),
                "density": np.full((nJ, nmu), np.nan), "compressibility": np.full((nJ, nmu), np.nan),
                "mu_plus": np.full(nJ, np.nan), "mu_minus": np.full(nJ, np.nan),
-               "status": np.zeros((3, nJ), np.int32), "used_fp64": np.zeros((3, nJ), np.int32)}
+               "status": np.zeros((3, nJ), np.int32), "used_fp64": np.zeros((3, nJ), np.int32),
+               "width": np.zeros(3, np.int32)}
         out = SweepOut(*[_ptr(res[f]).value for f in ("E", "gap_ratio", "density", "compressibility", "mu_plus",
-                                                      "mu_minus", "status", "used_fp64")])
+                                                      "mu_minus", "status", "used_fp64", "width")])
         self.check(lib().bh_sweep(self.h, C.byref(sp), C.byref(o), C.byref(out)))
         return res
 

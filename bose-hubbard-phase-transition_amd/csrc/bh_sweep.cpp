@@ -2,10 +2,11 @@
 //
 // Replaces the grid loop of Analysis::calculate_and_save (src/analysis.cpp:266-387): there every grid
 // point merges three sparse matrices (:311) and runs its own IRLM (:314) on an OpenMP thread (:302).
-// Here each sector's CSR is built once, all J points of a sector go through bh_lanczos as one batch
-// (one CSR pass per Lanczos step for the whole batch), and mu enters afterwards as the N-dependent
-// shift -mu N.  The gap ratio follows Analysis::gap_ratios (src/analysis.cpp:433-454).
+// Here each sector's CSR is built once, the J points of a sector go through bh_lanczos in batches of
+// a width chosen per sector (one CSR pass per Lanczos step for the whole batch), and mu enters afterwards
+// as the N-dependent shift -mu N.  The gap ratio follows Analysis::gap_ratios (src/analysis.cpp:433-454).
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "../../include/bh_capi.h"
@@ -35,11 +36,43 @@ struct Owned {
     }
 };
 
+// Block-width tuner of a sector: the operator is timed at widths 8, 16, 32, 64 (up to the batch that
+// holds every J point) and the narrowest width whose cost per vector is within 10% of the best is kept.
+// A wider batch amortises the matrix stream over more vectors; a narrower one leaves fewer lanes idle
+// while the slowest solve of the batch finishes and groups neighbouring J, which converge alike.
+int sweep_width(bh_ctx* ctx, bh_ham* h, int prec, int nJ, int* W) {
+    int top = 1;
+    while (top < nJ) top <<= 1;
+    if (top <= 8) {
+        *W = top;
+        return BH_OK;
+    }
+    std::vector<int> ws;
+    std::vector<double> cost;
+    double best = 0.0;
+    for (int w = 8; w <= top && w <= 64; w <<= 1) {
+        double ms = 0.0, bytes = 0.0;
+        const int rc = bh_spmv_bench(ctx, h, prec, w, 5, &ms, &bytes);
+        if (rc != BH_OK) return rc;
+        ws.push_back(w);
+        cost.push_back(ms / w);
+        if (best == 0.0 || ms / w < best) best = ms / w;
+    }
+    *W = top;
+    for (size_t i = 0; i < ws.size(); ++i)
+        if (cost[i] <= 1.1 * best) {
+            *W = ws[i];
+            break;
+        }
+    return BH_OK;
+}
+
 }  // namespace
 
 extern "C" int bh_sweep(bh_ctx* ctx, const bh_sweep_spec* sp, const bh_lanczos_opts* opts, bh_sweep_out* out) {
     if (!ctx || !sp || !opts || !out || !sp->J || (sp->nmu > 0 && !sp->mu)) return BH_EINVAL;
     if (sp->nJ < 1 || sp->nJ > 64 || sp->nmu < 0 || sp->n < 0) return BH_EINVAL;
+    if (sp->width < 0 || sp->width > 64 || (sp->width & (sp->width - 1)) != 0) return BH_EINVAL;
     const int m = sp->kind == 0 ? sp->Lx : sp->Lx * sp->Ly;
     if (m < 1 || m > 32) return BH_EINVAL;
     std::vector<int> off(m + 1), sites(4 * m + 4);
@@ -49,6 +82,7 @@ extern "C" int bh_sweep(bh_ctx* ctx, const bh_sweep_spec* sp, const bh_lanczos_o
     std::vector<double> Uv(nJ, sp->U);
     for (int q = 0; q < 3; ++q) {
         const int N = sp->n - 1 + q;
+        if (out->width) out->width[q] = 0;
         for (int i = 0; i < nJ; ++i) {
             if (out->status) out->status[q * nJ + i] = BH_OK;
             if (out->used_fp64) out->used_fp64[q * nJ + i] = 0;
@@ -61,11 +95,22 @@ extern "C" int bh_sweep(bh_ctx* ctx, const bh_sweep_spec* sp, const bh_lanczos_o
         if (rc != BH_OK) return rc;
         rc = bh_hamiltonian_build(ctx, o.b, off.data(), sites.data(), sp->flags, &o.h);
         if (rc != BH_OK) return rc;
+        int W = sp->width;
+        if (W == 0) {
+            rc = sweep_width(ctx, o.h, opts->prec, nJ, &W);
+            if (rc != BH_OK) return rc;
+        }
+        if (out->width) out->width[q] = W;
         std::vector<double> ev(static_cast<size_t>(nJ) * k), rs(ev.size()), ee(ev.size());
         std::vector<int> it(nJ), uf(nJ), st(nJ);
-        bh_lanczos_result r{ev.data(), rs.data(), ee.data(), it.data(), uf.data(), st.data()};
-        rc = bh_lanczos(ctx, o.h, nJ, sp->J, Uv.data(), opts, &r);
-        if (rc != BH_OK && rc != BH_ENOCONV) return rc;
+        for (int c0 = 0; c0 < nJ; c0 += W) {
+            const int nb = nJ - c0 < W ? nJ - c0 : W;
+            const size_t e0 = static_cast<size_t>(c0) * k;
+            bh_lanczos_result r{ev.data() + e0, rs.data() + e0, ee.data() + e0, it.data() + c0, uf.data() + c0,
+                                st.data() + c0};
+            rc = bh_lanczos(ctx, o.h, nb, sp->J + c0, Uv.data() + c0, opts, &r);
+            if (rc != BH_OK && rc != BH_ENOCONV) return rc;
+        }
         for (int i = 0; i < nJ; ++i) {
             for (int l = 0; l < k; ++l) out->E[(q * nJ + i) * k + l] = ev[static_cast<size_t>(i) * k + l];
             out->gap_ratio[q * nJ + i] = gap_ratio(&ev[static_cast<size_t>(i) * k], k);

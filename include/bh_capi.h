@@ -211,13 +211,16 @@ typedef struct bh_sweep_spec {
     int Lx, Ly;          /* chain: Lx = m */
     int periodic;
     int n;               /* central boson number */
-    int nJ;              /* J points (<= 64), solved as one batch per sector */
+    int nJ;              /* J points (<= 64), solved in batches of `width` per sector */
     const double* J;
     double U;
     int nmu;
     const double* mu;
     unsigned flags;      /* BH_CONV_* */
     int order;           /* basis order used for the solves (BH_ORDER_LEX is the fastest) */
+    int width;           /* J points per batched solve: 0 = chosen per sector by timing the operator at
+                          * widths 8 .. 64 (the narrowest within 10% of the best cost per vector), else a
+                          * power of two <= 64 */
 } bh_sweep_spec;
 
 typedef struct bh_sweep_out {
@@ -229,6 +232,7 @@ typedef struct bh_sweep_out {
     double* mu_minus;        /* [nJ] */
     int* status;             /* [3][nJ] */
     int* used_fp64;          /* [3][nJ] (may be NULL) */
+    int* width;              /* [3] batch width used per sector (may be NULL; 0 for an empty sector) */
 } bh_sweep_out;
 
 int bh_sweep(bh_ctx* ctx, const bh_sweep_spec* spec, const bh_lanczos_opts* opts, bh_sweep_out* out);

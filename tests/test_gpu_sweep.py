@@ -71,6 +71,7 @@ def test_sweep_full_size(ctx):
     mu = 3.0 * np.arange(64) / 63.0
     res = ctx.sweep("chain", n, J, 1.0, mu, m=m, k=4)
     assert np.all(res["status"] == 0)
+    assert all(w in (8, 16, 32) for w in res["width"])  # block width chosen per sector by the tuner
     off, sites = O.chain_neighbours(m, True)
     for q in (0, 2):
         b = ctx.basis(m, n - 1 + q, bhx.ORDER_LEX)
@@ -86,3 +87,20 @@ def test_sweep_full_size(ctx):
     # the charge gap closes as J/U grows (1D Mott transition near J/U ~ 0.3)
     gap = res["mu_plus"] - res["mu_minus"]
     assert gap[0] > 0.9 and gap[-1] < gap[0] / 3
+
+
+def test_sweep_fixed_width_equals_tuned(ctx):
+    """Batches of a fixed width 8 give the levels of the tuned width (E0 within 1e-9, the levels above
+    within 1e-5: FP32-stored solves) and report the width used per sector."""
+    m, n = 10, 10
+    J = 0.01 * 40.0 ** (np.arange(20) / 19.0)
+    mu = np.array([0.5, 1.5])
+    a = ctx.sweep("chain", n, J, 1.0, mu, m=m, k=4)
+    b = ctx.sweep("chain", n, J, 1.0, mu, m=m, k=4, width=8)
+    assert list(b["width"]) == [8, 8, 8] and all(w in (8, 16, 32) for w in a["width"])
+    assert np.all(a["status"] == 0) and np.all(b["status"] == 0)
+    e0a, e0b = a["E"][:, :, 0], b["E"][:, :, 0]
+    assert np.all(np.abs(e0a - e0b) <= 1e-9 * np.abs(e0b))
+    assert np.allclose(a["E"], b["E"], rtol=1e-5, atol=1e-9)
+    with pytest.raises(bhx.BHError):
+        ctx.sweep("chain", n, J, 1.0, mu, m=m, k=4, width=12)
