@@ -13,10 +13,19 @@ struct LaneResult {
     std::vector<double> vecs;  // k * D (when requested)
 };
 
+// host-side time profile of an engine (BHX_LZ_PROFILE=1 prints it per batch solve)
+struct LzProfile {
+    int checks = 0;
+    double wait_ms = 0.0;    // host waiting for the device at the analyses
+    double host_ms = 0.0;    // host analyses (the device idles unless other engines have work)
+    double finish_ms = 0.0;  // FP64 finish, wall time
+};
+
 // B lanes over one Hamiltonian; each lane runs one solve (J_b, U_b) with its own step counter, so lanes
 // are admitted and retired independently.  All work goes to the engine's stream.
 class LzEngineBase {
    public:
+    LzProfile prof;
     virtual ~LzEngineBase() = default;
     virtual int admit(int slot, double J, double U) = 0;
     virtual int run(int steps) = 0;                     // enqueue steps for the active lanes

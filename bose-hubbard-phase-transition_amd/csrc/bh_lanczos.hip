@@ -25,6 +25,7 @@
 // Mixed precision: Krylov and locked vectors stored FP32, W and every reduction FP64; the matrix
 // values come exactly from the FP64 table of the packed format.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -47,6 +48,7 @@ constexpr int LZ_GRAPH_STEPS = 8;  // Lanczos steps per captured HIP graph
 constexpr int LZ_K1_TILES = 4;         // row groups per K1 workgroup on the row-lane engine
 constexpr int LZ_TUNE_REPS = 3;        // timed launches per K1 variant in the autotuner
 constexpr int LZ_CHECK_MIN = 4;        // fewest steps between two analyses of a lane
+constexpr int64_t LZ_SPEC_MIN_WORK = 4ll << 20;  // D B from which analyses overlap speculative steps
 constexpr int64_t LZ_PART_BYTES = 64ll << 20;  // partial sums of the staged K1 (more workgroups than grid_rl)
 constexpr int LZ_K1_GRID_MAX = 32768;  // K1 workgroups at most (rows of partial sums)
 constexpr double LZ_ACCEPT_LEVEL = 1e-6;  // mixed: bound on ||r||^2/gap of the levels above E0, relative
@@ -827,6 +829,16 @@ int lz_so_always() {
     return d;
 }
 
+// Steps enqueued at an analysis before the host analyses (the device runs them meanwhile): one captured
+// graph's worth by default, BHX_SPEC=0 turns the overlap off.
+int lz_spec_steps() {
+    static int d = [] {
+        const char* e = std::getenv("BHX_SPEC");
+        return (e && std::strcmp(e, "0") == 0) ? 0 : LZ_GRAPH_STEPS;
+    }();
+    return d;
+}
+
 // start vector of the solver (Park-Miller sequence x <- 16807 x mod (2^31 - 1), seed 0 -> 1, values
 // x / (2^31 - 1) - 0.5), shared by every lane
 std::vector<double> lz_start_vector(int64_t D, uint64_t seed) {
@@ -1047,6 +1059,8 @@ class LzEngine final : public LzEngineBase {
         act.assign(B, 0);
         nlock.assign(B, 0);
         jl.assign(B, 0);
+        jan.assign(B, 0);
+        tot_an.assign(B, 0);
         tot.assign(B, 0);
         kept.assign(B, 0);
         Jb.assign(B, 0.0);
@@ -1076,7 +1090,8 @@ class LzEngine final : public LzEngineBase {
         int st = -1;
         for (int b = 0; b < B; ++b)
             if (act[b]) {
-                const int d = std::max(1, next_event(b) - jl[b]);
+                // with speculative steps a lane may already be past its next analysis: check at once
+                const int d = std::max(speculate() ? 0 : 1, next_event(b) - jl[b]);
                 st = st < 0 ? d : std::min(st, d);
             }
         return st < 0 ? chunk : st;
@@ -1156,14 +1171,32 @@ class LzEngine final : public LzEngineBase {
         hb.resize(static_cast<size_t>(jmax + 1) * B);
         BH_HIP(ctx, hipMemcpyAsync(ha.data(), st.alpha, ha.size() * 8, hipMemcpyDeviceToHost, s));
         BH_HIP(ctx, hipMemcpyAsync(hb.data(), st.beta, hb.size() * 8, hipMemcpyDeviceToHost, s));
+        const auto tw0 = std::chrono::steady_clock::now();
         BH_HIP(ctx, hipStreamSynchronize(s));
+        const auto tw1 = std::chrono::steady_clock::now();
+        // The device is idle from here on.  Steps for the active lanes are enqueued before the analysis so
+        // that the device works while the host analyses.  Each lane is analysed at its own scheduled step
+        // counts up to the synced ones (the coefficients are all there), so its analysis points do not
+        // depend on when the checks happen.  A lane that a decision changes at an earlier step than the one
+        // it has reached -- a lock -- is rolled back to that step first (its step counter and current
+        // vector; the steps after it only wrote later basis slots); a lane that retires keeps its first
+        // basis vectors untouched; a lane whose basis is full runs no speculative step.
+        jsync = jl;
+        totsync = tot;
+        if (speculate()) BH_TRY(run(lz_spec_steps()));
+        struct HostClock {  // adds the host time of this analysis to the engine's profile on return
+            LzEngine* e;
+            std::chrono::steady_clock::time_point t;
+            ~HostClock() { e->prof.host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count(); }
+        } hc{this, tw1};
+        prof.wait_ms += std::chrono::duration<double, std::milli>(tw1 - tw0).count();
+        prof.checks += 1;
         std::vector<std::vector<int>> new_locks(B), lock_slot(B);  // Ritz index, Y slot (-1: append)
         std::vector<Analysis> ans(B);
         std::vector<int> done_now(B, 0), restart_now(B, 0);
-        auto analyse_run = [&](int b) {
+        // analysis of lane b at step j (total steps tj); true when it retires or restarts the lane
+        auto analyse_at = [&](int b, int j, int tj) -> bool {
             RunHost& R = rh[b];
-            const int j = jl[b];
-            if (j < next_event(b)) return;
             nextc[b] = j + std::max(chunk, j / 8);
             R.alpha.assign(j, 0.0);
             R.beta.assign(j + 1, 0.0);
@@ -1199,18 +1232,18 @@ class LzEngine final : public LzEngineBase {
             }
             if (broken) {
                 R.done = true;
-                R.iters = tot[b];
+                R.iters = tj;
                 R.status = BH_ENOCONV;
                 R.theta.clear();
                 R.s.clear();
                 done_now[b] = 1;
-                return;
+                return true;
             }
             if (R.tnorm0 == 0.0) R.tnorm0 = an.tnorm;
             const bool exhausted = jr < j || bnext <= std::fmax(1e-12, 100.0 * eps_store) * std::fmax(an.tnorm, 1e-300) ||
                                    j >= D;
             if (lz_debug()) {
-                std::fprintf(stderr, "[lz] j=%d tot=%d lane=%d tnorm=%.3e bnext=%.3e locks=%d:", j, tot[b], b, an.tnorm, bnext,
+                std::fprintf(stderr, "[lz] j=%d tot=%d lane=%d tnorm=%.3e bnext=%.3e locks=%d:", j, tj, b, an.tnorm, bnext,
                              nlock[b]);
                 for (size_t i = 0; i < an.theta.size() && i < 8; ++i)
                     std::fprintf(stderr, " (%.10f %.1e)", an.theta[i], an.bound[i]);
@@ -1248,10 +1281,10 @@ class LzEngine final : public LzEngineBase {
                 cthr[q] = std::fmax(tol * std::fmax(eps23, std::fabs(an.theta[lev[q]])), floor);
                 if (lbound[q] >= cthr[q] && !exhausted) conv = false;
             }
-            if (conv || tot[b] >= maxsteps) {
+            if (conv || tj >= maxsteps) {
                 R.done = true;
                 R.exhausted = exhausted;
-                R.iters = tot[b];
+                R.iters = tj;
                 R.status = conv && static_cast<int>(lev.size()) == k ? BH_OK : BH_ENOCONV;
                 R.theta.clear();
                 R.s.clear();
@@ -1261,7 +1294,7 @@ class LzEngine final : public LzEngineBase {
                 }
                 R.tnorm = an.tnorm;
                 done_now[b] = 1;
-                return;
+                return true;
             }
             // Next analysis: where the slowest unconverged wanted level is predicted to meet its threshold,
             // extrapolating log(bound / threshold) linearly in steps from the lane's previous analysis
@@ -1271,13 +1304,13 @@ class LzEngine final : public LzEngineBase {
                 double gap = 0.0;
                 for (size_t q = 0; q < lev.size(); ++q)
                     if (lbound[q] >= cthr[q]) gap = std::fmax(gap, std::log(lbound[q] / cthr[q]));
-                if (R.prev_tot >= 0 && tot[b] > R.prev_tot && R.prev_gap > gap) {
-                    const double rate = (R.prev_gap - gap) / (tot[b] - R.prev_tot);
+                if (R.prev_tot >= 0 && tj > R.prev_tot && R.prev_gap > gap) {
+                    const double rate = (R.prev_gap - gap) / (tj - R.prev_tot);
                     const double d = std::ceil(1.05 * gap / rate) + 1.0;
                     if (d < nextc[b] - j) nextc[b] = j + std::max(LZ_CHECK_MIN, static_cast<int>(d));
                 }
                 R.prev_gap = gap;
-                R.prev_tot = tot[b];
+                R.prev_tot = tj;
             } else {
                 R.prev_tot = -1;
             }
@@ -1299,11 +1332,11 @@ class LzEngine final : public LzEngineBase {
                     P.s.push_back(an.s[i]);
                 }
                 restart_now[b] = 1;
-                return;
+                return true;
             }
             // selective orthogonalisation: lock converged wanted-region Ritz vectors (Parlett and Scott:
             // bound below sqrt(eps) ||T||); not needed when every step reorthogonalises fully
-            if (fullre) return;
+            if (fullre) return false;
             const double lock_thr = std::sqrt(eps_store) * an.tnorm;
             for (size_t q = 0; q < lev.size(); ++q) {
                 const int i = lev[q];
@@ -1338,11 +1371,23 @@ class LzEngine final : public LzEngineBase {
                 new_locks[b].push_back(best);
                 lock_slot[b].push_back(l);
             }
+            return false;
+        };
+        auto analyse_run = [&](int b) {
+            for (;;) {
+                const int e = std::min({nextc[b], cap, jsync[b] + std::max(0, maxsteps - totsync[b])});
+                if (e > jsync[b]) return;
+                const int tj = totsync[b] - (jsync[b] - e);
+                jan[b] = e;
+                tot_an[b] = tj;
+                if (analyse_at(b, e, tj) || !new_locks[b].empty()) return;
+            }
         };
         {
-            std::vector<int> todo;
+            std::vector<int> todo;  // lanes with an analysis due
             for (int b = 0; b < B; ++b)
-                if (act[b]) todo.push_back(b);
+                if (act[b] && std::min({nextc[b], cap, jsync[b] + std::max(0, maxsteps - totsync[b])}) <= jsync[b])
+                    todo.push_back(b);
             const int nth = std::max(1, std::min(static_cast<int>(todo.size()),
                                                  static_cast<int>(std::thread::hardware_concurrency())));
             if (nth <= 1) {
@@ -1365,6 +1410,23 @@ class LzEngine final : public LzEngineBase {
             }
             if (!new_locks[b].empty()) any_lock = true;
             if (restart_now[b]) any_restart = true;
+        }
+        // roll lanes back to the step of their decision
+        bool rolled = false;
+        for (int b = 0; b < B; ++b) {
+            const bool decided = done_now[b] || !new_locks[b].empty() || restart_now[b];
+            if (!decided || jan[b] >= jl[b]) continue;
+            jl[b] = jan[b];
+            tot[b] = tot_an[b];
+            if (done_now[b]) continue;  // retired: only the host counters
+            BH_HIP(ctx, hipMemcpy2DAsync(Xc + b, B * sizeof(XT), V + static_cast<int64_t>(jl[b]) * vs + b, B * sizeof(XT),
+                                         sizeof(XT), static_cast<size_t>(D), hipMemcpyDeviceToDevice, s));
+            rolled = true;
+        }
+        if (rolled) {
+            step_h = jl;
+            BH_HIP(ctx, hipMemcpyAsync(st.step, step_h.data(), B * sizeof(int), hipMemcpyHostToDevice, s));
+            BH_HIP(ctx, hipStreamSynchronize(s));
         }
         if (any_lock) BH_TRY(lock(new_locks, lock_slot, ans));
         if (any_restart) BH_TRY(restart(restart_now));
@@ -1619,6 +1681,11 @@ class LzEngine final : public LzEngineBase {
         e = std::min<int64_t>(e, static_cast<int64_t>(jl[b]) + std::max(0, maxsteps - tot[b]));
         return e;
     }
+    // Speculative steps pay when they keep the device busy for a good part of a host analysis (a few
+    // milliseconds): on for engines of at least LZ_SPEC_MIN_WORK vector elements (D B), whose steps take
+    // hundreds of microseconds; smaller engines analyse at the scheduled steps only.  The rule depends on
+    // the sizes alone, so the sequence of analyses and its results are reproducible.
+    bool speculate() const { return lz_spec_steps() > 0 && vs >= LZ_SPEC_MIN_WORK; }
     // K1 variants: 0 = staged row blocks; V + 16 u (V = 1, 2, 4, 8 vectors per thread, u = 0, 1, 2 for
     // U = 4, 8, 16 entries of a row in flight) = row-lane engine
     static int k1_V(int var) { return var & 15; }
@@ -1882,10 +1949,10 @@ class LzEngine final : public LzEngineBase {
         std::vector<int> njb(B, 0);
         for (int b = 0; b < B; ++b) {
             if (new_locks[b].empty()) continue;
-            njb[b] = jl[b];
+            njb[b] = static_cast<int>(ans[b].s[new_locks[b][0]].size());  // the analysed step count
             for (int i = 0; i < nv && i < static_cast<int>(new_locks[b].size()); ++i) {
                 const std::vector<double>& sv = ans[b].s[new_locks[b][i]];
-                for (int t = 0; t < static_cast<int>(sv.size()) && t < jl[b]; ++t)
+                for (int t = 0; t < static_cast<int>(sv.size()) && t < njb[b]; ++t)
                     C[(static_cast<size_t>(t) * nv + i) * B + b] = sv[t] * hsc[static_cast<size_t>(t) * B + b];
             }
         }
@@ -1974,6 +2041,9 @@ class LzEngine final : public LzEngineBase {
     size_t shm = 0, shm_rb = 0;
     std::vector<RunHost> rh;
     std::vector<int> act, nlock, jl, tot, kept, nextc;
+    std::vector<int> jsync, totsync;  // step counts at the last analysis' device sync
+    std::vector<int> jan, tot_an;     // step counts of each lane's last analysis
+    std::vector<int> step_h;          // host copy of the step counters written at a rollback
     std::vector<double> Jb, Ub, hlo, hhi, ha, hb;
     double an0[B] = {};
     Restart plan[B];
@@ -2068,7 +2138,17 @@ static int lanczos_batch(bh_ctx* ctx, bh_ham* h, int prec, const double* J, cons
     }
     for (int b = 0; b < nreal; ++b) all.push_back(b);
     std::vector<LaneResult> out;
+    const auto tf0 = std::chrono::steady_clock::now();
     BH_TRY(e->finish(all, out, vecs != nullptr));
+    e->prof.finish_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tf0).count();
+    static const bool prof_on = [] {
+        const char* v = std::getenv("BHX_LZ_PROFILE");
+        return v && std::strcmp(v, "1") == 0;
+    }();
+    if (prof_on)
+        std::fprintf(stderr, "[lz-profile] D=%lld lanes=%d fp64=%d checks=%d wait=%.1f ms host=%.1f ms finish=%.1f ms\n",
+                     static_cast<long long>(h->D), nreal, e->fp64() ? 1 : 0, e->prof.checks, e->prof.wait_ms,
+                     e->prof.host_ms, e->prof.finish_ms);
     const int k = o.k;
     if (vecs) vecs->assign(static_cast<size_t>(nreal) * k * h->D, 0.0);
     for (int b = 0; b < nreal; ++b) {
