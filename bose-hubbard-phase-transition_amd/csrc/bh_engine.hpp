@@ -17,8 +17,12 @@ struct LaneResult {
 struct LzProfile {
     int checks = 0;
     double wait_ms = 0.0;    // host waiting for the device at the analyses
-    double host_ms = 0.0;    // host analyses (the device idles unless other engines have work)
+    double host_ms = 0.0;    // host side of the analyses: decisions, locks, restarts, retirements
+    double analysis_ms = 0.0;  // of which the tridiagonal analyses of the lanes
     double finish_ms = 0.0;  // FP64 finish, wall time
+    double run_ms = 0.0;     // enqueueing steps, graph captures included
+    double capture_ms = 0.0; // capturing and instantiating step graphs
+    double setup_ms = 0.0;   // engine construction and admission
 };
 
 // B lanes over one Hamiltonian; each lane runs one solve (J_b, U_b) with its own step counter, so lanes

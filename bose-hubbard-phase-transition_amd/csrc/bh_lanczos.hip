@@ -45,6 +45,7 @@ constexpr int LZ_LMAX = 16;        // locked Ritz vectors per solve
 constexpr int LZ_NVMAX = 16;       // Ritz vectors formed per pass
 constexpr int LZ_RED_WG = 1024;    // reduction workgroup: 16 waves, one value per wave
 constexpr int LZ_GRAPH_STEPS = 8;  // Lanczos steps per captured HIP graph
+constexpr int LZ_K2_GRID = 1024;   // K2 workgroups at most (rows of partial sums for R2; 512 and 2048 measured slower)
 constexpr int LZ_K1_TILES = 4;         // row groups per K1 workgroup on the row-lane engine
 constexpr int LZ_TUNE_REPS = 3;        // timed launches per K1 variant in the autotuner
 constexpr int LZ_CHECK_MIN = 4;        // fewest steps between two analyses of a lane
@@ -499,61 +500,84 @@ __global__ void __launch_bounds__(LZ_RED_WG) k_lz_reduce(const double* __restric
 
 // R2: beta_{j+1} = ||v_{j+1}||, 1/beta, each active lane's step advanced; the carried products
 // <y_l, v_{j+1}> = (e_l - c_l) / beta_{j+1} and the Parlett-Scott estimates of the locked vectors; the next
-// step's orthogonalisation flag.  One workgroup: phase 1 reduces every value (a wave per value, four
-// independent partial sums per lane, fixed order) into LDS; phase 2 gives each lane to one thread, which
-// reads the step counter before anything is written.
+// step's orthogonalisation flag.  One workgroup: phase 1 reduces every value (coalesced reads, a fixed
+// order) into LDS; phase 2 updates the locked-vector
+// quantities with a thread per (vector, lane), then each lane's beta and step counter with a thread per lane
+// (the step counters are read by all of phase 2a before any is written).
 constexpr int LZ_BETA_MAXV = (1 + LZ_LMAX) * 64;
+constexpr int LZ_BETA_ACC = 8;  // independent partial sums per thread in the reduction
 
 template <int B>
 __global__ void __launch_bounds__(LZ_RED_WG) k_lz_beta(LzState st, const double* __restrict__ part, int nwg, int nv,
                                                        int maxit, double eps, int so_always) {
     __shared__ double sums[LZ_BETA_MAXV];
+    __shared__ double red[LZ_RED_WG];
     __shared__ int want;
-    const int lane = threadIdx.x & 63;
-    const int waves = LZ_RED_WG / 64;
     const int so_cur = st.so[0], so_prev = st.so[1];
     const int nvv = so_cur ? nv : B;
     if (threadIdx.x == 0) want = 0;
-    for (int v = threadIdx.x >> 6; v < nvv; v += waves) {
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        int w = lane;
-        for (; w + 192 < nwg; w += 256) {
-            s0 += part[static_cast<size_t>(w) * nv + v];
-            s1 += part[static_cast<size_t>(w + 64) * nv + v];
-            s2 += part[static_cast<size_t>(w + 128) * nv + v];
-            s3 += part[static_cast<size_t>(w + 192) * nv + v];
-        }
-        for (; w < nwg; w += 64) s0 += part[static_cast<size_t>(w) * nv + v];
-        double s = (s0 + s1) + (s2 + s3);
+    // Values v0 .. v0 + nvc - 1 at a time: thread t sums value v0 + t % nvc over the rows g, g + G, ... with
+    // g = t / nvc, G = LZ_RED_WG / nvc, so that consecutive threads read consecutive values of one row
+    // (coalesced); the G partial sums of a value are then added in the order of g.
+    for (int v0 = 0; v0 < nvv; v0 += LZ_RED_WG) {
+        const int nvc = min(LZ_RED_WG, nvv - v0);
+        const int G = LZ_RED_WG / nvc;
+        const int t = threadIdx.x;
+        if (t < G * nvc) {
+            const int v = v0 + t % nvc, g = t / nvc;
+            double acc[LZ_BETA_ACC];
 #pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
-        if (lane == 0) sums[v] = s;
+            for (int q = 0; q < LZ_BETA_ACC; ++q) acc[q] = 0.0;
+            for (int w0 = g; w0 < nwg; w0 += LZ_BETA_ACC * G) {
+#pragma unroll
+                for (int q = 0; q < LZ_BETA_ACC; ++q) {
+                    const int w = w0 + q * G;
+                    if (w < nwg) acc[q] += part[static_cast<size_t>(w) * nv + v];
+                }
+            }
+#pragma unroll
+            for (int h = LZ_BETA_ACC / 2; h >= 1; h >>= 1)
+#pragma unroll
+                for (int q = 0; q < h; ++q) acc[q] += acc[q + h];
+            red[t] = acc[0];
+        }
+        __syncthreads();
+        if (t < nvc) {
+            double s = 0.0;
+            for (int g = 0; g < G; ++g) s += red[g * nvc + t];
+            sums[v0 + t] = s;
+        }
+        __syncthreads();
     }
     __syncthreads();
+    // phase 2a: one thread per (locked vector l, lane b), all in parallel
+    for (int q = threadIdx.x; q < LZ_LMAX * B; q += LZ_RED_WG) {
+        const int b = q % B, l = q / B;
+        const int j = st.step[b];
+        if (!st.active[b] || j >= maxit || l >= min(st.nlock[b], LZ_LMAX)) continue;
+        const double nrm = std::sqrt(sums[b]);
+        const double al = st.sums[b], bj = st.beta[j * B + b];
+        const size_t i0 = static_cast<size_t>(l) * B + b, i1 = static_cast<size_t>(LZ_LMAX + l) * B + b;
+        // Parlett-Scott estimate of |<y_l, v_{j+1}>| (reset after an orthogonalising step)
+        const double tj = st.tau[i0], tjm = st.tau[i1];
+        const double tn = so_cur ? eps
+                                 : (nrm > 0.0 ? (std::fabs(st.ltheta[i0] - al) * tj + bj * tjm + eps * st.anorm[b]) / nrm
+                                              : 1.0);
+        st.tau[i1] = tj;
+        st.tau[i0] = tn;
+        if (tn > std::sqrt(eps)) atomicOr(&want, 1);
+        if (so_cur && (1 + l) * B + b < nv && nrm > 0.0) {
+            st.dso[i1] = st.dso[i0];
+            st.dso[i0] = (sums[(1 + l) * B + b] - st.dso[(2 * LZ_LMAX + l) * B + b]) / nrm;
+        }
+    }
+    __syncthreads();  // every thread of 2a has read the step counters
+    // phase 2b: one thread per lane
     if (threadIdx.x < B) {
         const int b = threadIdx.x;
         const int j = st.step[b];
         if (st.active[b] && j < maxit) {
             const double nrm = std::sqrt(sums[b]);
-            const int nl = min(st.nlock[b], LZ_LMAX);
-            const double al = st.sums[b], bj = st.beta[j * B + b];
-            int any = 0;
-            for (int l = 0; l < nl; ++l) {
-                const size_t i0 = static_cast<size_t>(l) * B + b, i1 = static_cast<size_t>(LZ_LMAX + l) * B + b;
-                // Parlett-Scott estimate of |<y_l, v_{j+1}>| (reset after an orthogonalising step)
-                const double tj = st.tau[i0], tjm = st.tau[i1];
-                const double tn = so_cur ? eps
-                                         : (nrm > 0.0 ? (std::fabs(st.ltheta[i0] - al) * tj + bj * tjm + eps * st.anorm[b]) / nrm
-                                                      : 1.0);
-                st.tau[i1] = tj;
-                st.tau[i0] = tn;
-                if (tn > std::sqrt(eps)) any = 1;
-                if (so_cur && (1 + l) * B + b < nv && nrm > 0.0) {
-                    st.dso[i1] = st.dso[i0];
-                    st.dso[i0] = (sums[(1 + l) * B + b] - st.dso[(2 * LZ_LMAX + l) * B + b]) / nrm;
-                }
-            }
-            if (any) atomicOr(&want, 1);
             st.beta[(j + 1) * B + b] = nrm;
             st.scale[(j + 1) * B + b] = nrm > 0.0 ? 1.0 / nrm : 0.0;
             st.step[b] = j + 1;
@@ -645,6 +669,7 @@ __global__ void __launch_bounds__(256) k_lz_restart(XT* V, XT* __restrict__ Xc, 
 constexpr int LZ_RZ_R = 4;       // elements per thread (strided by the workgroup size)
 constexpr int LZ_RZ_G = 8;       // outputs per launch
 constexpr int LZ_RZ_LDS = 4096;  // coefficients staged per pass (32 KB)
+constexpr int LZ_RZ_T = 4;       // Krylov slots loaded together
 
 template <typename XT, typename OT, int B>
 __global__ void __launch_bounds__(256) k_lz_ritz(const XT* __restrict__ V, int64_t vs, const int* __restrict__ njb, int njmax,
@@ -670,7 +695,30 @@ __global__ void __launch_bounds__(256) k_lz_ritz(const XT* __restrict__ V, int64
         }
         __syncthreads();
         const int te = min(t1, nj);
-        for (int t = t0; t < te; ++t) {
+        int t = t0;
+        // LZ_RZ_T slots per iteration: all their loads are issued before the multiply-adds (the sums still
+        // run over t in order)
+        for (; t + LZ_RZ_T <= te; t += LZ_RZ_T) {
+            XT v[LZ_RZ_T][LZ_RZ_R];
+#pragma unroll
+            for (int q = 0; q < LZ_RZ_T; ++q)
+#pragma unroll
+                for (int u = 0; u < LZ_RZ_R; ++u) {
+                    const int64_t idx = base + u * 256;
+                    v[q][u] = idx < vs ? V[static_cast<int64_t>(t + q) * vs + idx] : XT(0);
+                }
+#pragma unroll
+            for (int q = 0; q < LZ_RZ_T; ++q) {
+                const double* ct = cs + (t + q - t0) * per_t + b;
+#pragma unroll
+                for (int i = 0; i < LZ_RZ_G; ++i) {
+                    const double c = ct[i * B];
+#pragma unroll
+                    for (int u = 0; u < LZ_RZ_R; ++u) acc[u][i] += to_d(v[q][u]) * c;
+                }
+            }
+        }
+        for (; t < te; ++t) {
             double v[LZ_RZ_R];
 #pragma unroll
             for (int u = 0; u < LZ_RZ_R; ++u) {
@@ -989,7 +1037,7 @@ class LzEngine final : public LzEngineBase {
         if (staged_ok) BH_TRY(bh_ham_rowblocks(h, B, &rblk, &nrblk));
         const int64_t rb_cap = std::max<int64_t>(grid_rl, LZ_PART_BYTES / (static_cast<int64_t>(nvmax) * 8));
         grid_rb = static_cast<int>(std::min<int64_t>({std::max<int64_t>(nrblk, 1), LZ_K1_GRID_MAX, rb_cap}));
-        BH_TRY(dalloc(ctx, part, static_cast<size_t>(std::max({LZ_GRID_MAX, grid_rl, grid_rb})) * nvmax));
+        BH_TRY(dalloc(ctx, part, static_cast<size_t>(std::max({LZ_GRID_MAX, LZ_K2_GRID, grid_rl, grid_rb})) * nvmax));
         BH_TRY(dalloc(ctx, dnorm, static_cast<size_t>(2 * LZ_NVMAX * B)));
         BH_TRY(dalloc(ctx, dnj, B));
         BH_TRY(dalloc(ctx, dno, B));
@@ -1053,7 +1101,7 @@ class LzEngine final : public LzEngineBase {
         shm_rb = shm + BH_STAGE_PRODUCTS * sizeof(double);
         set_k1(bh_engine_choice(B, h) == 1 && staged_ok ? 0 : 1);
         const int64_t ntile_b = (D + LZ_WG / B - 1) / (LZ_WG / B);
-        grid_k2 = static_cast<int>(std::min<int64_t>(ntile_b, LZ_GRID_MAX));
+        grid_k2 = static_cast<int>(std::min<int64_t>(ntile_b, LZ_K2_GRID));
         rh.assign(B, RunHost());
         nextc.assign(B, 0);
         act.assign(B, 0);
@@ -1169,9 +1217,9 @@ class LzEngine final : public LzEngineBase {
         if (jmax == 0) return BH_OK;
         ha.resize(static_cast<size_t>(jmax) * B);
         hb.resize(static_cast<size_t>(jmax + 1) * B);
+        const auto tw0 = std::chrono::steady_clock::now();
         BH_HIP(ctx, hipMemcpyAsync(ha.data(), st.alpha, ha.size() * 8, hipMemcpyDeviceToHost, s));
         BH_HIP(ctx, hipMemcpyAsync(hb.data(), st.beta, hb.size() * 8, hipMemcpyDeviceToHost, s));
-        const auto tw0 = std::chrono::steady_clock::now();
         BH_HIP(ctx, hipStreamSynchronize(s));
         const auto tw1 = std::chrono::steady_clock::now();
         // The device is idle from here on.  Steps for the active lanes are enqueued before the analysis so
@@ -1384,6 +1432,14 @@ class LzEngine final : public LzEngineBase {
             }
         };
         {
+            const auto ta0 = std::chrono::steady_clock::now();
+            struct AnaClock {
+                LzEngine* e;
+                std::chrono::steady_clock::time_point t;
+                ~AnaClock() {
+                    e->prof.analysis_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+                }
+            } ac{this, ta0};
             std::vector<int> todo;  // lanes with an analysis due
             for (int b = 0; b < B; ++b)
                 if (act[b] && std::min({nextc[b], cap, jsync[b] + std::max(0, maxsteps - totsync[b])}) <= jsync[b])
@@ -1593,6 +1649,14 @@ class LzEngine final : public LzEngineBase {
         for (auto& q : graphs)
             if (q.exec && q.lm == LM && q.steps == ns) g = &q;
         if (!g) {
+            const auto tc0 = std::chrono::steady_clock::now();
+            struct CaptureClock {
+                LzEngine* e;
+                std::chrono::steady_clock::time_point t;
+                ~CaptureClock() {
+                    e->prof.capture_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+                }
+            } cc{this, tc0};
             hipGraph_t graph = nullptr;
             BH_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             for (int t = 0; t < ns; ++t) step(LM);
@@ -2089,7 +2153,7 @@ size_t bh_engine_bytes(int64_t D, int prec, int B, const bh_lanczos_opts& o) {
     const size_t es = (prec == BH_PREC_FP64 || fr) ? 8 : 4;
     const size_t vs = static_cast<size_t>(D) * Bp;
     const int64_t rpw = std::max<int64_t>(static_cast<int64_t>(LZ_WG / Bp) * LZ_K1_TILES, (D + LZ_K1_GRID_MAX - 1) / LZ_K1_GRID_MAX);
-    const size_t part_rows = static_cast<size_t>(std::max<int64_t>(LZ_GRID_MAX, (D + rpw - 1) / rpw));
+    const size_t part_rows = static_cast<size_t>(std::max<int64_t>({LZ_GRID_MAX, LZ_K2_GRID, (D + rpw - 1) / rpw}));
     return (static_cast<size_t>(slots) + LZ_LMAX + 1) * vs * es + vs * 8 + static_cast<size_t>(D) * 8 +
            part_rows * (2 * LZ_NVMAX + 1 + 2 * LZ_LMAX) * Bp * 8 +
            (static_cast<size_t>(slots) + 2) * Bp * 24 + static_cast<size_t>(3 * LZ_LMAX + 1) * Bp * 8 +
@@ -2128,12 +2192,16 @@ bool bh_lane_accepted(int status, const double* ev, const double* ee, int k, dou
 static int lanczos_batch(bh_ctx* ctx, bh_ham* h, int prec, const double* J, const double* U, int nreal,
                          const bh_lanczos_opts& o, bh_lanczos_result* res, std::vector<double>* vecs) {
     int rc = BH_OK;
+    const auto ts0 = std::chrono::steady_clock::now();
     auto e = bh_make_engine(ctx, h, prec, nreal, o, ctx->stream, true, &rc);
     if (!e) return rc;
     for (int b = 0; b < nreal; ++b) BH_TRY(e->admit(b, J[b], U[b]));
+    e->prof.setup_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts0).count();
     std::vector<int> fin, all;
     while (e->active() > 0) {
+        const auto tr0 = std::chrono::steady_clock::now();
         BH_TRY(e->run(e->suggest_chunk()));
+        e->prof.run_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tr0).count();
         BH_TRY(e->check(fin));
     }
     for (int b = 0; b < nreal; ++b) all.push_back(b);
@@ -2146,9 +2214,12 @@ static int lanczos_batch(bh_ctx* ctx, bh_ham* h, int prec, const double* J, cons
         return v && std::strcmp(v, "1") == 0;
     }();
     if (prof_on)
-        std::fprintf(stderr, "[lz-profile] D=%lld lanes=%d fp64=%d checks=%d wait=%.1f ms host=%.1f ms finish=%.1f ms\n",
-                     static_cast<long long>(h->D), nreal, e->fp64() ? 1 : 0, e->prof.checks, e->prof.wait_ms,
-                     e->prof.host_ms, e->prof.finish_ms);
+        std::fprintf(stderr,
+                     "[lz-profile] D=%lld lanes=%d fp64=%d checks=%d setup=%.1f ms run=%.1f ms (capture %.1f) wait=%.1f ms "
+                     "host=%.1f ms (analysis %.1f) finish=%.1f ms\n",
+                     static_cast<long long>(h->D), nreal, e->fp64() ? 1 : 0, e->prof.checks, e->prof.setup_ms,
+                     e->prof.run_ms, e->prof.capture_ms, e->prof.wait_ms, e->prof.host_ms, e->prof.analysis_ms,
+                     e->prof.finish_ms);
     const int k = o.k;
     if (vecs) vecs->assign(static_cast<size_t>(nreal) * k * h->D, 0.0);
     for (int b = 0; b < nreal; ++b) {
