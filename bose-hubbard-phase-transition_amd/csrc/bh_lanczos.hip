@@ -743,6 +743,29 @@ __global__ void __launch_bounds__(256) k_lz_ritz(const XT* __restrict__ V, int64
     }
 }
 
+// Y[dst[i][b]][r][b] = src[i][r][b] for the new locked vectors (dst -1: none)
+template <typename XT, int B>
+__global__ void __launch_bounds__(256) k_lz_put_locked(const XT* __restrict__ src, XT* __restrict__ Y, int64_t vs, int nv,
+                                                       const int* __restrict__ dst) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (idx >= vs) return;
+    const int b = static_cast<int>(idx % B);
+    for (int i = 0; i < nv; ++i) {
+        const int d = dst[i * B + b];
+        if (d >= 0) Y[static_cast<int64_t>(d) * vs + idx] = src[static_cast<int64_t>(i) * vs + idx];
+    }
+}
+
+// Xc[r][b] = V[slot[b]][r][b] for the lanes rolled back to step slot[b] (slot -1: lane untouched)
+template <typename XT, int B>
+__global__ void __launch_bounds__(256) k_lz_restore_current(const XT* __restrict__ V, XT* __restrict__ Xc, int64_t vs,
+                                                            const int* __restrict__ slot) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (idx >= vs) return;
+    const int d = slot[idx % B];
+    if (d >= 0) Xc[idx] = V[static_cast<int64_t>(d) * vs + idx];
+}
+
 // per-(i, b) partial sums over rows of [nv][D][B] arrays (reduced by k_lz_reduce mode 0):
 //   mode 0: value 2i -> <y,y>, 2i+1 -> <y,Hy>;  mode 1: value i -> ||Hy - theta y||^2;  mode 2: i -> <y,y>
 template <typename YT, int B>
@@ -786,35 +809,41 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_dots(const YT* __restrict__ Y, con
 
 // Rayleigh-Ritz of the finished candidates, row i of the Gram matrices: per lane b, partials of
 // value q < nv: <Y_i, Y_q> and value nv + q: <Y_i, HY_q> (reduced by k_lz_reduce mode 0)
-template <int B>
-__global__ void __launch_bounds__(LZ_WG) k_lz_gram_row(const double* __restrict__ Y, const double* __restrict__ HY, int64_t D,
-                                                       int nv, int i, double* part) {
-    __shared__ double lds[LZ_WG / 64 * B];
+// Rows i0 .. i0 + NI - 1 of the Gram matrices of the finish, per lane: G(i, q) = <y_i, y_q> and
+// A(i, q) = <y_i, H y_q> for q < nv <= NV, as per-workgroup partials of the values ((i - i0) 2 + w) NV + q
+// (w = 0: G, 1: A); each thread reads its NI row elements once and every column element once.
+template <int B, int NV, int NI>
+__global__ void __launch_bounds__(LZ_WG) k_lz_gram_rows(const double* __restrict__ Y, const double* __restrict__ HY, int64_t D,
+                                                        int nv, int i0, double* part) {
+    __shared__ double lds[LZ_PUT_LDS];
     const int b = threadIdx.x % B;
     constexpr int ROWS = LZ_WG / B;
     const int64_t vs = D * B;
     const int64_t ntile = (D + ROWS - 1) / ROWS;
     const int64_t per = (ntile + gridDim.x - 1) / gridDim.x;
     const int64_t t0 = min(static_cast<int64_t>(blockIdx.x) * per, ntile), t1 = min(t0 + per, ntile);
-    double gy[LZ_NVMAX], gh[LZ_NVMAX];
+    double g[NI * 2 * NV];
 #pragma unroll
-    for (int q = 0; q < LZ_NVMAX; ++q) gy[q] = gh[q] = 0.0;
+    for (int q = 0; q < NI * 2 * NV; ++q) g[q] = 0.0;
     for (int64_t t = t0; t < t1; ++t) {
         const int64_t r = t * ROWS + threadIdx.x / B;
         if (r >= D) continue;
         const int64_t idx = r * B + b;
-        const double yi = Y[static_cast<int64_t>(i) * vs + idx];
+        double yi[NI];
 #pragma unroll
-        for (int q = 0; q < LZ_NVMAX; ++q)
+        for (int ii = 0; ii < NI; ++ii) yi[ii] = i0 + ii < nv ? Y[static_cast<int64_t>(i0 + ii) * vs + idx] : 0.0;
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
             if (q < nv) {
-                gy[q] += yi * Y[static_cast<int64_t>(q) * vs + idx];
-                gh[q] += yi * HY[static_cast<int64_t>(q) * vs + idx];
+                const double yq = Y[static_cast<int64_t>(q) * vs + idx], hq = HY[static_cast<int64_t>(q) * vs + idx];
+#pragma unroll
+                for (int ii = 0; ii < NI; ++ii) {
+                    g[(ii * 2) * NV + q] += yi[ii] * yq;
+                    g[(ii * 2 + 1) * NV + q] += yi[ii] * hq;
+                }
             }
     }
-    for (int q = 0; q < nv; ++q) {
-        lz_block_put<B>(gy[q], q, lds, part, 2 * nv * B);
-        lz_block_put<B>(gh[q], nv + q, lds, part, 2 * nv * B);
-    }
+    lz_block_put_multi<B, NI * 2 * NV>(g, lds, part, NI * 2 * NV * B);
 }
 
 // in place, per lane b: X[c][r][b] = sum_{i < nin} X[i][r][b] Z[(i nout + c) B + b] for c < nout (nout <= nin;
@@ -1040,6 +1069,7 @@ class LzEngine final : public LzEngineBase {
         BH_TRY(dalloc(ctx, part, static_cast<size_t>(std::max({LZ_GRID_MAX, LZ_K2_GRID, grid_rl, grid_rb})) * nvmax));
         BH_TRY(dalloc(ctx, dnorm, static_cast<size_t>(2 * LZ_NVMAX * B)));
         BH_TRY(dalloc(ctx, dnj, B));
+        BH_TRY(dalloc(ctx, lock_map, static_cast<size_t>(LZ_NVMAX) * B));
         BH_TRY(dalloc(ctx, dno, B));
         if (fullre) BH_TRY(dalloc(ctx, dd, static_cast<size_t>(slots) * B));
         st.alpha = st_d.p;
@@ -1469,19 +1499,22 @@ class LzEngine final : public LzEngineBase {
         }
         // roll lanes back to the step of their decision
         bool rolled = false;
+        std::vector<int> src_slot(B, -1);  // current vector of lane b restored from this slot
         for (int b = 0; b < B; ++b) {
             const bool decided = done_now[b] || !new_locks[b].empty() || restart_now[b];
             if (!decided || jan[b] >= jl[b]) continue;
             jl[b] = jan[b];
             tot[b] = tot_an[b];
             if (done_now[b]) continue;  // retired: only the host counters
-            BH_HIP(ctx, hipMemcpy2DAsync(Xc + b, B * sizeof(XT), V + static_cast<int64_t>(jl[b]) * vs + b, B * sizeof(XT),
-                                         sizeof(XT), static_cast<size_t>(D), hipMemcpyDeviceToDevice, s));
+            src_slot[b] = jl[b];
             rolled = true;
         }
         if (rolled) {
             step_h = jl;
             BH_HIP(ctx, hipMemcpyAsync(st.step, step_h.data(), B * sizeof(int), hipMemcpyHostToDevice, s));
+            BH_HIP(ctx, hipMemcpyAsync(lock_map.p, src_slot.data(), B * sizeof(int), hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL((k_lz_restore_current<XT, B>), dim3(static_cast<unsigned>((vs + 255) / 256)), dim3(256), 0, s, V,
+                               Xc, vs, lock_map.p);
             BH_HIP(ctx, hipStreamSynchronize(s));
         }
         if (any_lock) BH_TRY(lock(new_locks, lock_slot, ans));
@@ -1680,14 +1713,29 @@ class LzEngine final : public LzEngineBase {
     // is separated when the other one is in the span.
     int rayleigh_ritz(const std::vector<int>& lanes, int nvf, double* Yp, double* HYp) {
         DevArray<double> dg, dz;
-        BH_TRY(dg.alloc(ctx, static_cast<size_t>(nvf) * 2 * nvf * B));
-        for (int i = 0; i < nvf; ++i) {
-            hipLaunchKernelGGL((k_lz_gram_row<B>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yp, HYp, D, nvf, i, part.p);
-            reduce_plain(grid_k2, 2 * nvf * B, dg.p + static_cast<size_t>(i) * 2 * nvf * B);
+        // rows i0 .. i0 + NI - 1 of [G | A] per launch, NV columns each (NV >= nvf)
+        const bool small = nvf <= 8;
+        const int NI = small ? 4 : 2, NV = small ? 8 : LZ_NVMAX;
+        const int nblk = (nvf + NI - 1) / NI;
+        const size_t per_blk = static_cast<size_t>(NI) * 2 * NV * B;
+        BH_TRY(dg.alloc(ctx, nblk * per_blk));
+        for (int q = 0; q < nblk; ++q) {
+            if (small)
+                hipLaunchKernelGGL((k_lz_gram_rows<B, 8, 4>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yp, HYp, D, nvf, q * NI, part.p);
+            else
+                hipLaunchKernelGGL((k_lz_gram_rows<B, LZ_NVMAX, 2>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yp, HYp, D, nvf, q * NI,
+                                   part.p);
+            reduce_plain(grid_k2, static_cast<int>(per_blk), dg.p + q * per_blk);
         }
-        std::vector<double> g(static_cast<size_t>(nvf) * 2 * nvf * B);
-        BH_HIP(ctx, hipMemcpyAsync(g.data(), dg.p, g.size() * 8, hipMemcpyDeviceToHost, s));
+        std::vector<double> gg(nblk * per_blk), g(static_cast<size_t>(nvf) * 2 * nvf * B);
+        BH_HIP(ctx, hipMemcpyAsync(gg.data(), dg.p, gg.size() * 8, hipMemcpyDeviceToHost, s));
         BH_HIP(ctx, hipStreamSynchronize(s));
+        for (int i = 0; i < nvf; ++i)
+            for (int w = 0; w < 2; ++w)
+                for (int q = 0; q < nvf; ++q)
+                    for (int b = 0; b < B; ++b)
+                        g[(static_cast<size_t>(i) * 2 * nvf + w * nvf + q) * B + b] =
+                            gg[(i / NI) * per_blk + ((static_cast<size_t>(i % NI) * 2 + w) * NV + q) * B + b];
         std::vector<double> Z(static_cast<size_t>(nvf) * nvf * B, 0.0);
         for (int b : lanes) {
             const int n = std::min(static_cast<int>(rh[b].s.size()), nvf);
@@ -2040,15 +2088,14 @@ class LzEngine final : public LzEngineBase {
         BH_HIP(ctx, hipMemcpyAsync(tau.data(), st.tau, tau.size() * 8, hipMemcpyDeviceToHost, s));
         BH_HIP(ctx, hipMemcpyAsync(anv.data(), st.anorm, anv.size() * 8, hipMemcpyDeviceToHost, s));
         BH_HIP(ctx, hipStreamSynchronize(s));
+        std::vector<int> dst(static_cast<size_t>(nv) * B, -1);  // Y slot of vector i of lane b
         for (int b = 0; b < B; ++b) {
             int appended = 0;
             for (int i = 0; i < static_cast<int>(new_locks[b].size()) && i < nv; ++i) {
                 const int target = lock_slot[b][i];
                 if (target < 0 && nlock[b] + appended >= LZ_LMAX) continue;
                 const int slot = target >= 0 ? target : nlock[b] + appended++;
-                BH_HIP(ctx, hipMemcpy2DAsync(Y + static_cast<int64_t>(slot) * vs + b, B * sizeof(XT),
-                                             tmp.p + static_cast<int64_t>(i) * vs + b, B * sizeof(XT), sizeof(XT),
-                                             static_cast<size_t>(D), hipMemcpyDeviceToDevice, s));
+                dst[static_cast<size_t>(i) * B + b] = slot;
                 const double th = ans[b].theta[new_locks[b][i]];
                 const double bd = ans[b].bound[new_locks[b][i]];
                 if (target < 0) {
@@ -2066,6 +2113,10 @@ class LzEngine final : public LzEngineBase {
             nlock[b] += appended;
             lmax_used = std::max(lmax_used, nlock[b]);
         }
+        // vector i of lane b goes to Y slot dst[i][b] (one interleaved column each), in one launch
+        BH_HIP(ctx, hipMemcpyAsync(lock_map.p, dst.data(), dst.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL((k_lz_put_locked<XT, B>), dim3(static_cast<unsigned>((vs + 255) / 256)), dim3(256), 0, s, tmp.p,
+                           Y, vs, nv, lock_map.p);
         const int so2[2] = {1, 0};  // orthogonalise the next two Lanczos vectors against the locked set
         BH_HIP(ctx, hipMemcpyAsync(st.ltheta, lth.data(), lth.size() * 8, hipMemcpyHostToDevice, s));
         BH_HIP(ctx, hipMemcpyAsync(st.tau, tau.data(), tau.size() * 8, hipMemcpyHostToDevice, s));
@@ -2094,7 +2145,7 @@ class LzEngine final : public LzEngineBase {
     double* r0 = nullptr;
     DevArray<XT> V_own, Y_own, Xc_own;
     DevArray<double> W_own, r0_own, st_d, part, dC, dnorm, dd;
-    DevArray<int> st_i, dnj, dno;
+    DevArray<int> st_i, dnj, dno, lock_map;
     size_t n_alpha = 0, n_beta = 0, bytes = 0;
     LzState st{};
     LzArgs<XT> a{};
