@@ -52,7 +52,9 @@ std::unique_ptr<LzEngineBase> bh_make_engine(bh_ctx* ctx, bh_ham* h, int prec, i
 // acceptance of a mixed-precision solve after the FP64 finish (else it is rerun in FP64): converged,
 // E0 within accept_rel (0: 1e-9) by its bound ||r||^2/gap, and the levels above it within a looser bound
 bool bh_lane_accepted(int status, const double* evals, const double* errest, int k, double accept_rel);
-// options of the FP64 rerun of a rejected mixed solve
+// options of the FP64 rerun of a rejected mixed solve, and of the full-reorthogonalisation rerun of an
+// FP64 rerun that is not certified either
 bh_lanczos_opts bh_fallback_opts(const bh_lanczos_opts& o);
+bh_lanczos_opts bh_fallback_full_opts(const bh_lanczos_opts& o);
 // bytes an engine of B lanes allocates for itself (use_ctx_ws = false)
 size_t bh_engine_bytes(int64_t D, int prec, int B, const bh_lanczos_opts& o);

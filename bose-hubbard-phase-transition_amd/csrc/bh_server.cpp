@@ -181,7 +181,7 @@ int ensure_engine(bh_server* S, Sector& sc, int p) {
     }
     BH_TRY(ensure_ham(S, sc));
     const int prec = p == 0 ? BH_PREC_MIXED : BH_PREC_FP64;
-    // the FP64 engine serves the reruns of rejected mixed solves: full reorthogonalisation
+    // the FP64 engine serves the reruns of rejected mixed solves
     const bh_lanczos_opts lo = p == 0 ? S->opts.lanczos : bh_fallback_opts(S->opts.lanczos);
     const int L = lanes_wanted(S, sc, p);
     const size_t need = bh_engine_bytes(sc.ham->D, prec, L, lo) +
@@ -214,6 +214,19 @@ void serve_diagonal(bh_server* S, Sector& sc, const Pending& pd) {
     int rc = bh_lanczos(S->ctx, sc.ham, 1, &J, &U, &o, &r);
     for (int i = 0; i < k; ++i) ev[i] -= pd.req.mu * pd.req.n;
     complete(S, pd, rc == BH_OK ? st : rc, ev.data(), rs.data(), k, it, uf, t);
+}
+
+void serve_full(bh_server* S, Sector& sc, const Pending& pd, int prior_iters, double t_start) {
+    // an FP64 rerun whose levels are not certified: FP64 with full reorthogonalisation, solved at once
+    const int k = S->opts.lanczos.k;
+    double J = pd.req.J, U = pd.req.U;
+    std::vector<double> ev(k), rs(k), ee(k);
+    int it = 0, uf = 0, st = 0;
+    bh_lanczos_result r{ev.data(), rs.data(), ee.data(), &it, &uf, &st};
+    bh_lanczos_opts o = bh_fallback_full_opts(S->opts.lanczos);
+    int rc = bh_lanczos(S->ctx, sc.ham, 1, &J, &U, &o, &r);
+    for (int i = 0; i < k; ++i) ev[i] -= pd.req.mu * pd.req.n;
+    complete(S, pd, rc == BH_OK ? st : rc, ev.data(), rs.data(), k, prior_iters + it, 1, t_start);
 }
 
 void worker_loop(bh_server* S) {
@@ -299,11 +312,19 @@ void worker_loop(bh_server* S) {
                 Pending pd = sc.lane_req[p][lane];
                 sc.lane_busy[p][lane] = 0;
                 const LaneResult& L = res[i];
-                const bool ok = sc.eng[p]->fp64() ||
+                // full reorthogonalisation needs no certificate; the mixed engine and the FP64 engine with
+                // selective orthogonalisation do
+                const bool fullre_eng = sc.eng[p]->fp64() && bh_lz_fullre(sc.ham->D, p == 0 ? S->opts.lanczos
+                                                                                              : bh_fallback_opts(S->opts.lanczos));
+                const bool ok = fullre_eng ||
                                 bh_lane_accepted(L.status, L.evals.data(), L.errest.data(), k, S->opts.lanczos.accept_rel);
                 if (p == 0 && !ok && S->opts.lanczos.fallback >= 0) {
                     pd.prec = BH_PREC_FP64;
                     sc.queue[1].push_back(pd);  // rerun in FP64
+                    continue;
+                }
+                if (p == 1 && !ok) {
+                    serve_full(S, sc, pd, L.iters, sc.lane_start[p][lane]);
                     continue;
                 }
                 std::vector<double> ev(k, NAN), rs(k, NAN);

@@ -168,7 +168,8 @@ typedef struct bh_lanczos_opts {
     double dedup_rel;  /* level merge tolerance relative to ||T|| (0: 1e-9) */
     double accept_rel; /* mixed: FP64-finish acceptance of E0, ||r||^2/gap <= accept_rel*|E0| (0: 1e-9) */
     uint64_t seed;     /* Park-Miller start vector seed (0 -> 1, Spectra's convention) */
-    int fallback;      /* mixed: rerun failing solves in FP64 (1, default) or report them (-1) */
+    int fallback;      /* mixed: rerun failing solves in FP64 (1, default; an FP64 rerun that is not certified
+                          either is solved again with full reorthogonalisation) or report them (-1) */
     int want_vectors;  /* 1: keep the k Ritz vectors (bh_lanczos_copy_vectors) */
     int ncv;           /* Krylov vectors kept per solve (Spectra's ncv); a solve that fills them is thick-
                           restarted with its lowest Ritz vectors (0: min(D, 512); at least 2k + 8) */
