@@ -128,6 +128,7 @@ def main():
                     help="only run one device-to-device copy kernel of BYTES bytes (calibrates HBM counters) and exit")
     ap.add_argument("--serve", type=int, default=120, metavar="N",
                     help="requests of the serving stream at 200 req/s offered load (0: skip)")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the phase-diagram sweep leg")
     ap.add_argument("--quick", action="store_true",
                     help="time the mixed batch only (no plain FP64 comparator, SpMV table or CPU leg): for profilers")
     args = ap.parse_args()
@@ -254,7 +255,7 @@ def main():
     # batched phase-diagram sweep: chain m=10 periodic, sectors n = 9, 10, 11, 32 log-spaced J/U in
     # [0.01, 0.4], 64 mu/U in [0, 3], k = 4; block width chosen per sector by the tuner
     sweep = None
-    if not args.quick:
+    if not args.quick and not args.no_sweep:
         Jsw = 0.01 * 40.0 ** (np.arange(32) / 31.0)
         musw = 3.0 * np.arange(64) / 63.0
         ctx.sweep("chain", 10, Jsw, 1.0, musw, m=10, k=4)
