@@ -35,7 +35,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(m, n, J, U, k, gpu_steps, sample_steps=80):
+def cpu_baseline(m, n, J, U, k, gpu_steps, sample_steps=300):
     """The oracle's FP64 CPU Lanczos (full reorthogonalisation, OpenMP over this host's cores) on the
     same sector: the first `sample_steps` steps are timed and the full solve is extrapolated linearly
     to the step count of the FP64 GPU solve (linear is generous to the CPU: its per-step
