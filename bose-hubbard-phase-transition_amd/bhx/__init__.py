@@ -343,10 +343,11 @@ class Hamiltonian:
         """Autotuner choice for (kind 0 = Lanczos operator pass, 1 = spmv; precision; width B):
         {"variant": v, "candidates": {variant: us}}, or None before the sector was used that way."""
         v, n = C.c_int(), C.c_int()
-        cv, cu = (C.c_int * 8)(), (C.c_double * 8)()
-        if lib().bh_ham_tuned(self.h, kind, prec, B, C.byref(v), C.byref(n), cv, cu, 8) != OK:
+        cap = 32
+        cv, cu = (C.c_int * cap)(), (C.c_double * cap)()
+        if lib().bh_ham_tuned(self.h, kind, prec, B, C.byref(v), C.byref(n), cv, cu, cap) != OK:
             return None
-        return {"variant": v.value, "candidates": {cv[i]: cu[i] for i in range(min(n.value, 8))}}
+        return {"variant": v.value, "candidates": {cv[i]: cu[i] for i in range(min(n.value, cap))}}
 
     def set_tuned(self, kind: int, prec: int, B: int, variant: int):
         """Pins the operator variant for (kind, precision, width) instead of the autotuner's choice."""

@@ -221,6 +221,7 @@ extern "C" int bh_ham_set_tuned(bh_ham* h, int kind, int prec, int B, int varian
     else if (variant == 4) ok = B >= 4 && prec == BH_PREC_MIXED;
     else if (variant == 8) ok = kind == 0 && B >= 8 && prec == BH_PREC_MIXED;
     else if (variant == 17 || variant == 33 || variant == 18 || variant == 34) ok = kind == 0 && (variant & 15) <= B && B <= 16;
+    else if (variant == 20) ok = kind == 0 && B >= 4 && B <= 32 && prec == BH_PREC_MIXED;
     else if (variant > 100 && kind == 1) {
         const int G = variant - 100;
         ok = (G & (G - 1)) == 0 && G * B <= 64;

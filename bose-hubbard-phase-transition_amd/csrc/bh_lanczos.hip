@@ -1826,6 +1826,7 @@ class LzEngine final : public LzEngineBase {
         return;                                                                                        \
     }
         if constexpr (B >= 8 && sizeof(XT) == 4) K1W(8, 4)
+        if constexpr (B >= 4 && B <= 32 && sizeof(XT) == 4) K1W(4, 8)
         if constexpr (B <= 16) {
             K1W(1, 8) K1W(1, 16)
             if constexpr (B >= 2) {
@@ -1889,6 +1890,7 @@ class LzEngine final : public LzEngineBase {
                 if (mean_row > 6.0) cand.push_back(V + 16);
                 if (mean_row > 12.0) cand.push_back(V + 32);
             }
+        if (B >= 4 && B <= 32 && sizeof(XT) == 4 && mean_row > 6.0) cand.push_back(4 + 16);
         BhTuned t;
         t.variant = k1v;
         if (bh_tune_enabled() && cand.size() > 1) {

@@ -124,8 +124,8 @@ int bh_ham_prepare(bh_ham* h, int prec);
  * autotuner chose for this sector at (kind, prec, B), once the sector has been used that way (kind 0: the
  * Lanczos operator pass at engine width B, kind 1: bh_spmv).  *variant: 0 = LDS-staged row blocks;
  * V = 1, 2, 4 (and 8, kind 0 only) = one row per B / V threads holding V vectors each (64 V / B rows per
- * wavefront), four entries of a row in flight per thread; V + 16 and V + 32 (kind 0, V <= 2, B <= 16): the
- * same with eight or sixteen entries in flight;
+ * wavefront), four entries of a row in flight per thread; V + 16 and V + 32 (kind 0, V <= 2, B <= 16), and
+ * 20 (kind 0, V = 4, FP32, 4 <= B <= 32): the same with eight or sixteen entries in flight;
  * 100 + G = G threads per (row, vector).  cand_var / cand_us (capacity cap, may be NULL) receive each
  * timed candidate and its microseconds per launch, *ncand their number (0 when the default variant was
  * kept untimed: BHX_TUNE=0, BHX_ENGINE set, or a single candidate).  BH_EINVAL when not used yet. */
@@ -135,6 +135,7 @@ int bh_ham_tuned(const bh_ham* h, int kind, int prec, int B, int* variant, int* 
  * autotuner's choice (engines already running keep theirs).  BH_EINVAL when the variant does not fit:
  * 0 needs the longest row times B within the 2048-product stage; V in {1, 2, 4, 8} needs V <= B, and
  * V = 4 and V = 8 FP32 vectors (V = 8 kind 0 only); V + 16, V + 32 need V <= 2, V <= B <= 16 (kind 0);
+ * 20 needs FP32, 4 <= B <= 32 (kind 0);
  * 100 + G (kind 1 only) needs G B <= 64. */
 int bh_ham_set_tuned(bh_ham* h, int kind, int prec, int B, int variant);
 

@@ -30,7 +30,7 @@ def ctx():
 def variants(h, kind, prec, B):
     """every variant bh_ham_set_tuned accepts for this sector and width"""
     out = []
-    for v in (0, 1, 2, 4) + ((102, 104, 108, 116) if kind == SPMV else (8, 17, 33, 18, 34)):
+    for v in (0, 1, 2, 4) + ((102, 104, 108, 116) if kind == SPMV else (8, 17, 33, 18, 34, 20)):
         try:
             h.set_tuned(kind, prec, B, v)
             out.append(v)
