@@ -31,7 +31,8 @@ def solve(h, J, prec, graphs, **kw):
 
 @pytest.mark.parametrize("m,n,prec,kw", [(10, 9, bhx.PREC_MIXED, {}),            # FP32 storage, selective orthogonalisation
                                          (10, 9, bhx.PREC_MIXED, {"ncv": 48}),   # with thick restarts
-                                         (8, 8, bhx.PREC_FP64, {})])             # small sector: full reorthogonalisation
+                                         (8, 8, bhx.PREC_FP64, {}),              # small sector: full reorthogonalisation
+                                         (12, 12, bhx.PREC_MIXED, {})])          # D B >= 4M: speculative analyses
 def test_graph_replays_equal_eager(ctx, m, n, prec, kw):
     print(f"graph m={m} n={n} prec={prec} {kw}", flush=True)
     off, sites = O.chain_neighbours(m, True)

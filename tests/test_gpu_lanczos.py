@@ -202,3 +202,20 @@ def test_small_hopping_needs_restarts(ctx, mode):
     assert np.all(np.abs(r.evals[0] - want) <= (1e-9 if prec == bhx.PREC_FP64 else MIXED_EXCITED) * scale)
     h.close()
     b.close()
+
+
+def test_speculative_analyses_full_size(ctx):
+    """A batch large enough for the host analyses to overlap speculative steps (D B >= 4M: chain m=12 n=12,
+    four lanes), with the lanes locked and rolled back to their scheduled steps: E0 of every lane within
+    1e-9 of its own FP64 solve, the levels above within 1e-5."""
+    print("speculative full size", flush=True)
+    m, off, sites, b, h = build(ctx, ("chain", 12, True), 12)
+    J = np.array([0.08, 0.1, 0.13, 0.2])
+    r = h.lanczos(J, 1.0, k=4, prec=bhx.PREC_MIXED)
+    assert np.all(r.status == 0)
+    for i, Jv in enumerate(J):
+        r64 = h.lanczos(Jv, 1.0, k=4, prec=bhx.PREC_FP64)
+        e = rel(r.evals[i], r64.evals[0])
+        assert e[0] <= 1e-9 and np.all(e <= (1e-9 if r.used_fp64[i] else MIXED_EXCITED)), (Jv, r.evals[i], r64.evals[0])
+    h.close()
+    b.close()
