@@ -2206,9 +2206,13 @@ size_t bh_engine_bytes(int64_t D, int prec, int B, const bh_lanczos_opts& o) {
     const size_t es = (prec == BH_PREC_FP64 || fr) ? 8 : 4;
     const size_t vs = static_cast<size_t>(D) * Bp;
     const int64_t rpw = std::max<int64_t>(static_cast<int64_t>(LZ_WG / Bp) * LZ_K1_TILES, (D + LZ_K1_GRID_MAX - 1) / LZ_K1_GRID_MAX);
-    const size_t part_rows = static_cast<size_t>(std::max<int64_t>({LZ_GRID_MAX, LZ_K2_GRID, (D + rpw - 1) / rpw}));
+    const size_t row_bytes = static_cast<size_t>(2 * LZ_NVMAX + 1 + 2 * LZ_LMAX) * Bp * 8;
+    // partial-sum rows: the row-lane and K2 grids, or the staged K1's (up to LZ_PART_BYTES)
+    const size_t part_rows = std::max<size_t>(
+        static_cast<size_t>(std::max<int64_t>({LZ_GRID_MAX, LZ_K2_GRID, (D + rpw - 1) / rpw})),
+        std::min<size_t>(LZ_K1_GRID_MAX, static_cast<size_t>(LZ_PART_BYTES) / row_bytes));
     return (static_cast<size_t>(slots) + LZ_LMAX + 1) * vs * es + vs * 8 + static_cast<size_t>(D) * 8 +
-           part_rows * (2 * LZ_NVMAX + 1 + 2 * LZ_LMAX) * Bp * 8 +
+           part_rows * row_bytes +
            (static_cast<size_t>(slots) + 2) * Bp * 24 + static_cast<size_t>(3 * LZ_LMAX + 1) * Bp * 8 +
            (fr ? static_cast<size_t>(slots) * Bp * 8 : 0);
 }
