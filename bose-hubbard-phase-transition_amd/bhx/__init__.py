@@ -126,6 +126,7 @@ def lib():
             "bh_ham_tuned": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                        C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int]),
             "bh_ham_set_tuned": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "bh_spdm": (C.c_int, [_vp, _vp, _vp, _vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -262,6 +263,15 @@ class Basis:
         out = np.zeros(self.D, np.float64)
         self.ctx.check(lib().bh_basis_copy_tags(self.h, _ptr(out)))
         return out
+
+    def spdm(self, phi):
+        """Single-particle density matrix <phi| b_i^+ b_j |phi> (m x m) of a real state in this basis' order."""
+        phi = np.ascontiguousarray(phi, np.float64)
+        if phi.shape != (self.D,):
+            raise BHError(EINVAL, f"spdm: phi must have {self.D} entries")
+        rho = np.zeros((self.m, self.m))
+        self.ctx.check(lib().bh_spdm(self.ctx.h, self.h, _ptr(phi), _ptr(rho)))
+        return rho
 
     def perm(self):
         a = np.zeros(self.D, np.int32)

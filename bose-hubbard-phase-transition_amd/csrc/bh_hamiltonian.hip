@@ -119,6 +119,71 @@ __global__ void __launch_bounds__(HB_BLOCK) k_ham_fill(int64_t D, int m, int n, 
     diag[q] = v;
 }
 
+// Single-particle density matrix rho_ij = <phi| b_i^+ b_j |phi> (real phi), partial sums per workgroup.
+// A workgroup is one wave; it walks tiles of 64 basis positions (tile = blockIdx.x, + gridDim.x, ...). A
+// thread unranks its state once per tile and, for every ordered pair (i, j), forms phi_q n_i phi_q (i = j)
+// or phi_q sqrt((n_i + 1) n_j) phi_t with t the position of b_i^+ b_j s_q (n_j >= 1); the wave sums each pair
+// over its 64 positions (fixed shuffle order) and lane 0 adds it to the workgroup's LDS accumulator, tile
+// after tile, so the result is reproducible.  part[(i m + j) * gridDim.x + blockIdx.x] receives the sums.
+template <int NW, int BITS>
+__global__ void __launch_bounds__(HB_BLOCK) k_spdm(int64_t D, int m, int n, const uint32_t* __restrict__ g_glob, int gsize,
+                                                    const int32_t* __restrict__ lex_of_pos,
+                                                    const int32_t* __restrict__ pos_of_lex, const double* __restrict__ phi,
+                                                    double* __restrict__ part) {
+    extern __shared__ uint32_t lds[];
+    uint32_t* g = lds;                                     // gsize words
+    double* acc = reinterpret_cast<double*>(lds + ((gsize + 1) & ~1));  // m * m sums (8-byte aligned)
+    for (int a = threadIdx.x; a < gsize; a += blockDim.x) g[a] = g_glob[a];
+    for (int a = threadIdx.x; a < m * m; a += blockDim.x) acc[a] = 0.0;
+    __syncthreads();
+    const int64_t ntile = (D + HB_BLOCK - 1) / HB_BLOCK;
+    for (int64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        const int64_t q = tile * HB_BLOCK + threadIdx.x;
+        const bool valid = q < D;
+        PState<NW, BITS> s;
+        s.clear();
+        if (valid) bh_unrank(lex_of_pos[q], m, n, g, s);
+        const double pq = valid ? phi[q] : 0.0;
+        for (int i = 0; i < m; ++i) {
+            const int ni = s.get(i);
+            for (int j = 0; j < m; ++j) {
+                double c = 0.0;
+                if (valid) {
+                    if (i == j) {
+                        c = pq * pq * ni;
+                    } else {
+                        const int nj = s.get(j);
+                        if (nj >= 1) {
+                            PState<NW, BITS> t = s;
+                            pstate_add(t, i, 1);
+                            pstate_add(t, j, -1);
+                            const int64_t tl = pstate_rank(t, m, n, g);
+                            c = pq * phi[pos_of_lex[tl]] * std::sqrt(static_cast<double>((ni + 1) * nj));
+                        }
+                    }
+                }
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o, 64);
+                if (threadIdx.x == 0) acc[i * m + j] += c;
+            }
+        }
+    }
+    __syncthreads();
+    for (int a = threadIdx.x; a < m * m; a += blockDim.x) part[static_cast<int64_t>(a) * gridDim.x + blockIdx.x] = acc[a];
+}
+
+// out[p] = sum over the nb workgroup partials of value p, in order (one workgroup per value)
+__global__ void __launch_bounds__(256) k_spdm_sum(const double* __restrict__ part, int nb, double* __restrict__ out) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (int w = threadIdx.x; w < nb; w += 256) s += part[static_cast<int64_t>(blockIdx.x) * nb + w];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
 __global__ void k_ham_expand64(int64_t nnz, const uint32_t* __restrict__ words, int cb, const double* __restrict__ table,
                                int32_t* __restrict__ col, double* __restrict__ val) {
     const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -493,6 +558,37 @@ int bh_ham_copy_csr(const bh_ham* h, int64_t* row_ptr, int32_t* col, double* val
         BH_HIP(ctx, hipStreamSynchronize(s));
         for (size_t i = 0; i < d.size(); ++i) diag[i] = d[i];
     }
+    return BH_OK;
+}
+
+// Single-particle density matrix of one state (Analysis::SPDM / braket, src/analysis.cpp:497-512, 562-594).
+int bh_spdm(bh_ctx* ctx, const bh_basis* b, const double* phi, double* rho) {
+    if (!ctx || !b || !phi || !rho) return BH_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    BH_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const int m = b->m, n = b->n;
+    const int64_t D = b->D;
+    if (m < 1) return bh_set_error(ctx, BH_EINVAL, "spdm: empty lattice");
+    const int gsize = (m > 1 ? m - 1 : 1) * (n + 1);
+    const size_t lds_bytes = static_cast<size_t>((gsize + 1) & ~1) * 4 + static_cast<size_t>(m) * m * 8;
+    if (lds_bytes > 64 * 1024) return bh_set_error(ctx, BH_ERANGE, "spdm: rank table too large for LDS");
+    const int64_t ntile = (D + HB_BLOCK - 1) / HB_BLOCK;
+    const int nb = static_cast<int>(std::min<int64_t>(std::max<int64_t>(ntile, 1), 2048));
+    DevArray<double> dphi, part, out;
+    BH_TRY(dphi.alloc(ctx, static_cast<size_t>(D)));
+    BH_TRY(part.alloc(ctx, static_cast<size_t>(m) * m * nb));
+    BH_TRY(out.alloc(ctx, static_cast<size_t>(m) * m));
+    BH_HIP(ctx, hipMemcpyAsync(dphi.p, phi, static_cast<size_t>(D) * 8, hipMemcpyHostToDevice, s));
+#define LAUNCH_SPDM(NW_, BITS_)                                                                                   \
+    hipLaunchKernelGGL((k_spdm<NW_, BITS_>), dim3(nb), dim3(HB_BLOCK), lds_bytes, s, D, m, n, b->rank_tab, gsize, \
+                       b->lex_of_pos, b->pos_of_lex, dphi.p, part.p)
+    BH_LAYOUT_DISPATCH(b->nw, b->bits, LAUNCH_SPDM);
+#undef LAUNCH_SPDM
+    hipLaunchKernelGGL(k_spdm_sum, dim3(m * m), dim3(256), 0, s, part.p, nb, out.p);
+    BH_HIP(ctx, hipGetLastError());
+    BH_HIP(ctx, hipMemcpyAsync(rho, out.p, static_cast<size_t>(m) * m * 8, hipMemcpyDeviceToHost, s));
+    BH_HIP(ctx, hipStreamSynchronize(s));
     return BH_OK;
 }
 

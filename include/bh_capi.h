@@ -101,6 +101,14 @@ int bh_basis_copy_tags(const bh_basis* b, double* tags);
 /* lex_of_pos[k]: lexicographic rank of the state at position k; pos_of_lex: the inverse */
 int bh_basis_copy_perm(const bh_basis* b, int32_t* lex_of_pos, int32_t* pos_of_lex);
 
+/* Replaces Analysis::SPDM and Analysis::braket (include/analysis.hpp:56-57, src/analysis.cpp:497-512,
+ * 562-594): the single-particle density matrix rho[i*m + j] = <phi| b_i^+ b_j |phi> of a real state phi
+ * (D doubles in the basis's position order, e.g. a Ritz vector of bh_lanczos_copy_vectors for a Hamiltonian
+ * built on this basis).  Amplitude sqrt((n_i + 1) n_j) of the hop and exact target positions; the reference's
+ * amplitude taken after the hop (src/analysis.cpp:586-588), its tolerance tag search and its 1/nb_eigen
+ * scale (:511) are not reproduced.  The trace is n <phi|phi>. */
+int bh_spdm(bh_ctx* ctx, const bh_basis* b, const double* phi, double* rho);
+
 /* ---- Hamiltonian ------------------------------------------------------------------------------- */
 /* Replaces BH::fixed_bosons_hamiltonian(nbr, basis, tags, m, n, J, U, mu) (include/hamiltonian.hpp:87,
  * src/hamiltonian.cpp:170-256) for the J and U terms: two-pass count / exclusive scan / fill CSR

@@ -38,6 +38,10 @@
  *                            rule (GenEigsBase.h:141-158); plain Lanczos with full
  *                            reorthogonalisation instead of the implicitly restarted Arnoldi
  *   orc_gap_ratios           src/analysis.cpp:433-454
+ *   orc_spdm                 src/analysis.cpp:497-512, 562-594 (Analysis::SPDM / braket): <phi| b_i^+ b_j
+ *                            |phi> with the intended amplitude sqrt((n_i+1) n_j) and exact key search (the
+ *                            reference multiplies by the occupations after the hop, :586-588, finds the
+ *                            target with its tolerance search and divides by nb_eigen, :511)
  */
 #include <math.h>
 #include <stdint.h>
@@ -629,4 +633,37 @@ double orc_gap_ratio(int nb, const double *ev_in, double *ratios) {
     }
     free(ev);
     return sum / (double)(nb - 2);
+}
+
+/*
+ * Single-particle density matrix rho[i*m + j] = <phi| b_i^+ b_j |phi> of a real state phi given in the order
+ * of `states` (D states of m sites).  Returns ORC_OK, or ORC_ENOMEM / ORC_ERANGE (a target not in the basis).
+ */
+int orc_spdm(int m, int64_t D, const uint8_t *states, const double *phi, double *rho) {
+    if (m < 1 || m > 32 || D < 1) return ORC_EINVAL;
+    keyed_t *keys = make_keys(states, m, D);
+    if (!keys) return ORC_ENOMEM;
+    for (int a = 0; a < m * m; ++a) rho[a] = 0.0;
+    uint8_t t[64];
+    for (int64_t k = 0; k < D; ++k) {
+        const uint8_t *s = states + (size_t)k * m;
+        const double pk = phi[k];
+        for (int i = 0; i < m; ++i) {
+            rho[i * m + i] += pk * pk * (double)s[i];
+            for (int j = 0; j < m; ++j) {
+                if (j == i || s[j] < 1) continue;
+                memcpy(t, s, (size_t)m);
+                t[i] += 1;
+                t[j] -= 1;
+                const int64_t c = find_key(keys, D, state_key(t, m));
+                if (c < 0) {
+                    free(keys);
+                    return ORC_ERANGE;
+                }
+                rho[i * m + j] += pk * phi[c] * sqrt((double)(s[i] + 1) * (double)s[j]);
+            }
+        }
+    }
+    free(keys);
+    return ORC_OK;
 }

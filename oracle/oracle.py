@@ -67,6 +67,8 @@ def lib():
                                   C.POINTER(C.c_int)]
         L.orc_gap_ratio.restype = C.c_double
         L.orc_gap_ratio.argtypes = [C.c_int, _f64p, C.c_void_p]
+        L.orc_spdm.restype = C.c_int
+        L.orc_spdm.argtypes = [C.c_int, C.c_int64, _u8p, _f64p, _f64p]
         _LIB = L
     return _LIB
 
@@ -228,6 +230,29 @@ def lanczos(h: CSR, J: float, U: float, k: int, tol: float = 1e-10, max_iter: in
     rc = lib().orc_lanczos(h.D, h.row_ptr, _c32(h.col), _c64(h.val), h.diag, J, U, k, tol, max_iter, seed,
                            dedup_rel, ev, bd, C.byref(it))
     return rc, ev, bd, it.value
+
+
+def spdm(states, phi):
+    """<phi| b_i^+ b_j |phi> (m x m) for a real state phi in the order of `states` (D x m occupations)."""
+    states = np.ascontiguousarray(states, np.uint8)
+    D, m = states.shape
+    rho = np.zeros(m * m)
+    rc = lib().orc_spdm(m, D, states.reshape(-1), np.ascontiguousarray(phi, np.float64), rho)
+    if rc != 0:
+        raise ValueError(f"orc_spdm failed rc={rc}")
+    return rho.reshape(m, m)
+
+
+def condensate_fraction(rho) -> float:
+    """largest eigenvalue of the (real part of the) density matrix over its trace, as src/analysis.cpp:330-333"""
+    w = np.linalg.eigvalsh(0.5 * (rho + rho.T))
+    return float(abs(w[np.argmax(np.abs(w))]) / np.trace(rho))
+
+
+def coherence(rho) -> float:
+    """(sum_ij rho_ij rho_ji - sum_i rho_ii^2) / sum_ij rho_ij rho_ji, as src/analysis.cpp:541-556"""
+    total = float(np.sum(rho * rho.T))
+    return (total - float(np.sum(np.diag(rho) ** 2))) / total
 
 
 def gap_ratio(ev) -> float:

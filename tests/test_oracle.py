@@ -248,3 +248,53 @@ def test_tridiag_ql():
     w, z = O.tridiag_eig(d, e, vectors=True)
     assert np.allclose(w, np.linalg.eigvalsh(T), atol=1e-12)
     assert np.allclose(T @ z, z * w, atol=1e-10)
+
+
+def brute_hop_operator(states, i, j):
+    """b_i^+ b_j as a dense matrix on the basis `states` (integer occupations, explicit ladder amplitudes)"""
+    idx = {tuple(int(x) for x in s): k for k, s in enumerate(states)}
+    D = len(states)
+    op = np.zeros((D, D))
+    for k, s in enumerate(states):
+        t = [int(x) for x in s]
+        if t[j] == 0:
+            continue
+        amp = math.sqrt(t[j])
+        t[j] -= 1
+        amp *= math.sqrt(t[i] + 1)
+        t[i] += 1
+        op[idx[tuple(t)], k] += amp
+    return op
+
+
+@pytest.mark.parametrize("m,n", [(3, 2), (4, 3), (5, 2)])
+def test_spdm_matches_ladder_operators(m, n):
+    states, _, _ = O.basis(m, n)
+    rng = np.random.default_rng(m * 10 + n)
+    phi = rng.standard_normal(states.shape[0])
+    phi /= np.linalg.norm(phi)
+    rho = O.spdm(states, phi)
+    want = np.array([[phi @ brute_hop_operator(states, i, j) @ phi for j in range(m)] for i in range(m)])
+    assert np.allclose(rho, want, atol=1e-13)
+    assert np.trace(rho) == pytest.approx(n)  # sum_i <n_i> = n for a normalised state
+
+
+def test_spdm_closed_forms():
+    # U = 0: the ground state of the periodic chain puts every boson in the k = 0 mode, rho_ij = n / m:
+    # condensate fraction 1, coherence 1 - 1/m
+    m, n = 6, 4
+    off, sites = O.chain_neighbours(m, True)
+    states, _, _ = O.basis(m, n)
+    h = O.hamiltonian(m, n, off, sites, states=states)
+    _, v = np.linalg.eigh(h.dense(1.0, 0.0))
+    rho = O.spdm(states, v[:, 0])
+    assert np.allclose(rho, n / m, atol=1e-10)
+    assert O.condensate_fraction(rho) == pytest.approx(1.0, abs=1e-10)
+    assert O.coherence(rho) == pytest.approx(1.0 - 1.0 / m, abs=1e-10)
+    # J = 0 at unit filling: the Mott state |1,...,1>, rho = identity, coherence 0, condensate fraction 1/m
+    m = n = 5
+    states, _, _ = O.basis(m, n)
+    phi = np.array([1.0 if np.all(s == 1) else 0.0 for s in states])
+    rho = O.spdm(states, phi)
+    assert np.allclose(rho, np.eye(m)) and O.coherence(rho) == 0.0
+    assert O.condensate_fraction(rho) == pytest.approx(1.0 / m)
