@@ -282,6 +282,86 @@ std::vector<double> gap_ratios(const std::vector<std::complex<double>>& eigenval
     return r;
 }
 
+namespace {
+// rho = <phi| b_i^+ b_j |phi> on the device (m x m, row-major)
+std::vector<double> spdm_device(const bh_basis* b, int m, const double* phi) {
+    std::vector<double> rho(static_cast<size_t>(m) * m);
+    bhx::check(bh_spdm(bhx::context(), b, phi, rho.data()), "spdm");
+    return rho;
+}
+double coherence_of(const std::vector<double>& rho, int m) {
+    double all = 0.0, diag = 0.0;
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) {
+            const double v = rho[static_cast<size_t>(i) * m + j] * rho[static_cast<size_t>(j) * m + i];
+            all += v;
+            if (i == j) diag += v;
+        }
+    return (all - diag) / all;
+}
+double condensate_fraction_of(const std::vector<double>& rho, int m) {
+    std::vector<double> A(static_cast<size_t>(m) * m), w(m), V(static_cast<size_t>(m) * m);
+    double tr = 0.0;
+    for (int i = 0; i < m; ++i) {
+        tr += rho[static_cast<size_t>(i) * m + i];
+        for (int j = 0; j < m; ++j)
+            A[static_cast<size_t>(i) * m + j] = 0.5 * (rho[static_cast<size_t>(i) * m + j] + rho[static_cast<size_t>(j) * m + i]);
+    }
+    bhtri::jacobi_eig(m, A.data(), w.data(), V.data());
+    double big = 0.0;
+    for (double x : w)
+        if (std::abs(x) > std::abs(big)) big = x;
+    return std::abs(big / tr);
+}
+}  // namespace
+
+bhx::MatrixXcd SPDM(const bhx::MatrixXd& basis, const std::vector<double>& tags, bhx::MatrixXcd& eigenvectors) {
+    (void)tags;  // positions are exact here; the reference needs the tags for its search
+    const int m = static_cast<int>(basis.rows());
+    const int64_t D = basis.cols();
+    if (m < 1 || D < 1 || eigenvectors.rows() != D || eigenvectors.cols() < 1)
+        throw std::invalid_argument("SPDM: basis and eigenvectors do not match");
+    int n = 0;
+    for (int i = 0; i < m; ++i) n += static_cast<int>(std::lround(basis(i, 0)));
+    bh_basis* b = nullptr;
+    bhx::check(bh_basis_build(bhx::context(), m, n, BH_ORDER_REFERENCE, &b), "basis");
+    std::vector<double> phi(static_cast<size_t>(D));
+    for (int64_t q = 0; q < D; ++q) phi[static_cast<size_t>(q)] = eigenvectors(q, 0).real();
+    std::vector<double> rho;
+    try {
+        rho = spdm_device(b, m, phi.data());
+    } catch (...) {
+        bh_basis_destroy(b);
+        throw;
+    }
+    bh_basis_destroy(b);
+    bhx::MatrixXcd out(m, m);
+    const double scale = 1.0 / static_cast<double>(eigenvectors.cols());
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) out(i, j) = rho[static_cast<size_t>(i) * m + j] * scale;
+    return out;
+}
+
+double coherence(const bhx::MatrixXcd& spdm) {
+    const int m = static_cast<int>(spdm.rows());
+    double all = 0.0, diag = 0.0;
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) {
+            const double v = std::real(spdm(i, j) * spdm(j, i));
+            all += v;
+            if (i == j) diag += v;
+        }
+    return (all - diag) / all;
+}
+
+double condensate_fraction(const bhx::MatrixXcd& spdm) {
+    const int m = static_cast<int>(spdm.rows());
+    std::vector<double> rho(static_cast<size_t>(m) * m);
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) rho[static_cast<size_t>(i) * m + j] = spdm(i, j).real();
+    return condensate_fraction_of(rho, m);
+}
+
 void exact_parameters(int m, int n, double J, double U, double mu, double s, double r, std::string fixed_param,
                       const std::string& out) {
     const double eps = std::numeric_limits<double>::epsilon();
@@ -368,9 +448,43 @@ void exact_parameters(int m, int n, double J, double U, double mu, double s, dou
         if (var > 1e-8 * mean || k == 16) break;
         nb_eigen += 5;
     }
+    // condensate fraction and coherence from the ground state of each distinct (J, U) pair (-mu N leaves the
+    // eigenvectors unchanged); ground states in batches of 64, one vector each
+    std::map<std::pair<double, double>, std::pair<double, double>> obs;
+    {
+        std::vector<std::pair<double, double>> keys;
+        for (auto& kv : lev) keys.push_back(kv.first);
+        const int64_t D = sec->basis ? bh_basis_dim(sec->basis) : 0;
+        for (size_t q0 = 0; q0 < keys.size(); q0 += 64) {
+            const int B = static_cast<int>(std::min<size_t>(64, keys.size() - q0));
+            std::vector<double> Jb(B), Ub(B), ev(B);
+            std::vector<int> st(B);
+            for (int b = 0; b < B; ++b) {
+                Jb[b] = keys[q0 + b].first;
+                Ub[b] = keys[q0 + b].second;
+            }
+            bh_lanczos_opts o;
+            bh_lanczos_default_opts(&o);
+            o.k = 1;
+            o.prec = bhx::settings().prec;
+            o.want_vectors = 1;
+            bh_lanczos_result res{ev.data(), nullptr, nullptr, nullptr, nullptr, st.data()};
+            const int rc = bh_lanczos(bhx::context(), sec->ham, B, Jb.data(), Ub.data(), &o, &res);
+            if (rc != BH_OK && rc != BH_ENOCONV) throw std::runtime_error("Eigenvalue computation failed.");
+            std::vector<double> vec(static_cast<size_t>(B) * D);
+            bhx::check(bh_lanczos_copy_vectors(bhx::context(), B, 1, vec.data()), "ground states");
+            for (int b = 0; b < B; ++b) {
+                const auto rho = spdm_device(sec->basis, m, vec.data() + static_cast<size_t>(b) * D);
+                obs[keys[q0 + b]] = {condensate_fraction_of(rho, m), coherence_of(rho, m)};
+            }
+        }
+    }
     std::ofstream f(out);
     f << fixed_param << " " << fixed_value << std::endl;
-    for (int idx = 0; idx < npt; ++idx) f << p1v[idx] << " " << p2v[idx] << " " << gr[idx] << " nan nan" << std::endl;
+    for (int idx = 0; idx < npt; ++idx) {
+        const auto& ob = obs[{H[idx].J, H[idx].U}];
+        f << p1v[idx] << " " << p2v[idx] << " " << gr[idx] << " " << ob.first << " " << ob.second << std::endl;
+    }
 }
 
 }  // namespace Analysis

@@ -136,10 +136,24 @@ namespace Analysis {
 // larger gap is 0 (src/analysis.cpp:433-454)
 std::vector<double> gap_ratios(const std::vector<std::complex<double>>& eigenvalues, int nb_eigen);
 
+// Single-particle density matrix of the ground state (first column of `eigenvectors`, real part; basis and
+// tags as BH::fixed_set_basis returns them), computed on the device: rho_ij = <phi0| b_i^+ b_j |phi0> with the
+// hop amplitude sqrt((n_i+1) n_j), divided by the number of columns as the reference does
+// (src/analysis.cpp:497-512; its braket, :562-594, takes the amplitude after the hop and finds the target with
+// a tolerance search -- not reproduced).  m x m.
+bhx::MatrixXcd SPDM(const bhx::MatrixXd& basis, const std::vector<double>& tags, bhx::MatrixXcd& eigenvectors);
+
+// (sum_ij rho_ij rho_ji - sum_i rho_ii^2) / sum_ij rho_ij rho_ji (src/analysis.cpp:541-556)
+double coherence(const bhx::MatrixXcd& spdm);
+
+// |largest-magnitude eigenvalue of Re rho| / trace rho, as the reference computes it inline
+// (src/analysis.cpp:330-333)
+double condensate_fraction(const bhx::MatrixXcd& spdm);
+
 // The exact sweep of the reference CLI (src/analysis.cpp:207-387): periodic chain of m sites, n bosons,
 // a grid over two parameters with the third fixed, mean gap ratio per grid point, written to `out`
 // (default phase.txt) as "fixed_param fixed_value" then "p1 p2 gap_ratio condensate_fraction coherence"
-// per point.  The single-particle density matrix is not computed here: the last two columns are nan.
+// per point (condensate fraction and coherence from the ground state's single-particle density matrix).
 void exact_parameters(int m, int n, double J, double U, double mu, double s, double r, std::string fixed_param,
                       const std::string& out = "phase.txt");
 

@@ -1,8 +1,9 @@
 // bh_host_example.cpp -- the reference's call sequence through the C++ host surface, as a caller writes it:
 //   Neighbours -> BH::fixed_set_basis -> BH::fixed_bosons_hamiltonian (one term each) -> H = JH*J + UH*U
-//   + uH*mu -> Op::IRLM_eigen (and Op::exact_eigen for small sectors).
-// Prints one JSON object: dimension, the first tags, the CSR size of H, the Lanczos levels and, when the
-// sector is small, the lowest exact eigenvalues.
+//   + uH*mu -> Op::IRLM_eigen (and Op::exact_eigen for small sectors) -> Analysis::SPDM, condensate fraction,
+//   coherence.
+// Prints one JSON object: dimension, the first tags, the CSR size of H, the Lanczos levels, the ground
+// state's density-matrix observables and, when the sector is small, the lowest exact eigenvalues.
 //   bhx-host-example M N J U MU [K]
 #include <cstdio>
 #include <cstdlib>
@@ -40,7 +41,14 @@ int main(int argc, char** argv) {
         for (int i = 0; i < m; ++i) std::printf("%s%g", i ? ", " : "", basis(i, 0));
         std::printf("], \"lanczos\": [");
         for (size_t i = 0; i < ev.size(); ++i) std::printf("%s%.17g", i ? ", " : "", ev[i].real());
-        std::printf("], \"vec_rows\": %lld, \"exact\": [", static_cast<long long>(vecs.rows()));
+        const bhx::MatrixXcd rho = Analysis::SPDM(basis, tags, vecs);
+        std::printf("], \"condensate_fraction\": %.17g, \"coherence\": %.17g, \"spdm_trace\": %.17g",
+                    Analysis::condensate_fraction(rho), Analysis::coherence(rho), [&] {
+                        double t = 0.0;
+                        for (int i = 0; i < m; ++i) t += rho(i, i).real();
+                        return t;
+                    }());
+        std::printf(", \"vec_rows\": %lld, \"exact\": [", static_cast<long long>(vecs.rows()));
         if (H.rows() <= 1500) {
             bhx::MatrixXd ex;
             const auto w = Op::exact_eigen(H, ex);

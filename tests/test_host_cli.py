@@ -69,6 +69,12 @@ def test_host_api_matches_oracle(m, n, J, U, mu):
     assert np.allclose(out["lanczos"], lv, rtol=1e-9, atol=1e-9)
     assert np.allclose(out["exact"], ev[:len(out["exact"])], rtol=1e-9, atol=1e-9)
     assert out["vec_rows"] == h.D
+    # ground-state density matrix (Analysis::SPDM divides by the number of eigenvectors, as the reference)
+    w, v = np.linalg.eigh(h.dense(J, U))
+    rho = O.spdm(st, v[:, 0])
+    assert out["spdm_trace"] == pytest.approx(n / 4, rel=1e-9)
+    assert out["condensate_fraction"] == pytest.approx(O.condensate_fraction(rho), abs=1e-6)
+    assert out["coherence"] == pytest.approx(O.coherence(rho), abs=1e-6)
 
 
 def ref_gap_ratio(levels):
@@ -100,9 +106,9 @@ def test_cli_phase_file(tmp_path, fixed):
     assert len(rows) == n1 * n2
     off, sites = O.chain_neighbours(m, True)
     h = O.hamiltonian(m, n, off, sites)
+    st, _, _ = O.basis(m, n, O.ORDER_REFERENCE)
     for row in rows:
         p1, p2, g = row[0], row[1], row[2]
-        assert np.isnan(row[3]) and np.isnan(row[4])
         # H = H_fixed + H1 p1 + H2 p2 with the reference's wiring (src/analysis.cpp:247-256)
         if fixed == "J":
             Jv, Uv, muv = J, p1, p2
@@ -110,5 +116,10 @@ def test_cli_phase_file(tmp_path, fixed):
             Jv, Uv, muv = p1, U, p2
         else:
             Jv, Uv, muv = p1, p2, mu
-        lv = O.distinct_levels(np.linalg.eigvalsh(h.dense(Jv, Uv)) - muv * n, 16)
+        w, v = np.linalg.eigh(h.dense(Jv, Uv))
+        lv = O.distinct_levels(w - muv * n, 16)
         assert g == pytest.approx(ref_gap_ratio(lv), abs=1e-6)
+        # condensate fraction and coherence of the ground state (src/analysis.cpp:327-336)
+        rho = O.spdm(st, v[:, 0])
+        assert row[3] == pytest.approx(O.condensate_fraction(rho), abs=1e-6)
+        assert row[4] == pytest.approx(O.coherence(rho), abs=1e-6)
