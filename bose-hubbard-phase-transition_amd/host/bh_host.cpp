@@ -133,9 +133,21 @@ std::vector<std::vector<int>> Neighbours::getNeighbours() const { return neighbo
 
 namespace BH {
 
-int64_t SparseMatrix::rows() const { return sec ? bh_ham_dim(sec->ham) : 0; }
+int64_t SparseMatrix::rows() const {
+    if (!blocks.empty()) {
+        int64_t d = 0;
+        for (const auto& b : blocks) d += b.rows();
+        return d;
+    }
+    return sec ? bh_ham_dim(sec->ham) : 0;
+}
 
 int64_t SparseMatrix::nonZeros() const {
+    if (!blocks.empty()) {
+        int64_t z = 0;
+        for (const auto& b : blocks) z += b.nonZeros();
+        return z;
+    }
     if (!sec) return 0;
     int64_t z = 0;
     if (J != 0.0) z += bh_ham_nnz(sec->ham);
@@ -143,7 +155,23 @@ int64_t SparseMatrix::nonZeros() const {
     return z;
 }
 
+SparseMatrix SparseMatrix::operator*(double f) const {
+    if (blocks.empty()) return SparseMatrix(sec, J * f, U * f, Mu * f);
+    std::vector<SparseMatrix> b;
+    for (const auto& x : blocks) b.push_back(x * f);
+    return SparseMatrix(std::move(b));
+}
+
 SparseMatrix SparseMatrix::operator+(const SparseMatrix& o) const {
+    if (!blocks.empty() || !o.blocks.empty()) {
+        if (blocks.empty() && !sec) return o;
+        if (o.blocks.empty() && !o.sec) return *this;
+        if (blocks.size() != o.blocks.size())
+            throw std::invalid_argument("block-diagonal matrices of different block structure cannot be added");
+        std::vector<SparseMatrix> b;
+        for (size_t i = 0; i < blocks.size(); ++i) b.push_back(blocks[i] + o.blocks[i]);
+        return SparseMatrix(std::move(b));
+    }
     if (!sec) return o;
     if (!o.sec) return *this;
     if (sec != o.sec) throw std::invalid_argument("sparse matrices of different sectors cannot be added");
@@ -151,6 +179,26 @@ SparseMatrix SparseMatrix::operator+(const SparseMatrix& o) const {
 }
 
 void SparseMatrix::to_csr(std::vector<int64_t>& row_ptr, std::vector<int32_t>& col, std::vector<double>& val) const {
+    if (!blocks.empty()) {
+        row_ptr.assign(1, 0);
+        col.clear();
+        val.clear();
+        int64_t off = 0;
+        for (const auto& b : blocks) {
+            std::vector<int64_t> rp;
+            std::vector<int32_t> c;
+            std::vector<double> v;
+            b.to_csr(rp, c, v);
+            const int64_t base = static_cast<int64_t>(col.size());
+            for (size_t q = 0; q < c.size(); ++q) {
+                col.push_back(static_cast<int32_t>(c[q] + off));
+                val.push_back(v[q]);
+            }
+            for (size_t r = 1; r < rp.size(); ++r) row_ptr.push_back(rp[r] + base);
+            off += b.rows();
+        }
+        return;
+    }
     const int64_t D = rows();
     const int64_t nt = sec ? bh_ham_nnz(sec->ham) : 0;
     std::vector<int64_t> rp(static_cast<size_t>(D) + 1, 0);
@@ -197,6 +245,48 @@ std::pair<std::vector<double>, bhx::MatrixXd> fixed_set_basis(int m, int n) {
     return {tags, basis};
 }
 
+std::pair<std::vector<double>, bhx::MatrixXd> max_set_basis(int m, int n) {
+    std::vector<double> tags;
+    std::vector<std::vector<double>> cols;
+    for (int N = 1; N <= n; ++N) {
+        auto [t, b] = fixed_set_basis(m, N);
+        for (int64_t k = 0; k < b.cols(); ++k) {
+            tags.push_back(t[static_cast<size_t>(k)]);
+            std::vector<double> c(m);
+            for (int i = 0; i < m; ++i) c[i] = b(i, k);
+            cols.push_back(std::move(c));
+        }
+    }
+    // the reference's sort: argsort by tag, then its cycle-swap loop (which realises the inverse permutation)
+    std::vector<int> idx(tags.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    std::sort(idx.begin(), idx.end(), [&tags](int a, int b) { return tags[a] < tags[b]; });
+    for (int i = 0; i < static_cast<int>(idx.size()); ++i) {
+        while (idx[i] != i) {
+            const int j = idx[i];
+            std::swap(tags[i], tags[j]);
+            std::swap(cols[i], cols[j]);
+            std::swap(idx[i], idx[j]);
+        }
+    }
+    bhx::MatrixXd basis(m, static_cast<int64_t>(cols.size()));
+    for (size_t k = 0; k < cols.size(); ++k)
+        for (int i = 0; i < m; ++i) basis(i, static_cast<int64_t>(k)) = cols[k][i];
+    return {tags, basis};
+}
+
+SparseMatrix max_bosons_hamiltonian(const std::vector<std::vector<int>>& neighbours, int m, int n_min, int n_max, double J,
+                                    double U, double mu) {
+    if (n_min < 0) n_min = 0;
+    if (n_max < n_min) n_max = n_min;
+    std::vector<SparseMatrix> b;
+    for (int N = n_min; N <= n_max; ++N) {
+        auto [t, basis] = fixed_set_basis(m, N);
+        b.push_back(fixed_bosons_hamiltonian(neighbours, basis, t, m, N, J, U, mu));
+    }
+    return SparseMatrix(std::move(b));
+}
+
 SparseMatrix fixed_bosons_hamiltonian(const std::vector<std::vector<int>>& neighbours, const bhx::MatrixXd& basis,
                                       const std::vector<double>& tags, int m, int n, double J, double U, double mu) {
     auto s = bhx::sector(neighbours, m, n);
@@ -218,6 +308,53 @@ SparseMatrix fixed_bosons_hamiltonian(const std::vector<std::vector<int>>& neigh
 namespace Op {
 
 std::vector<std::complex<double>> IRLM_eigen(const BH::SparseMatrix& O, int nb_eigen, bhx::MatrixXcd& eigenvectors) {
+    if (!O.blocks.empty()) {
+        // block by block: the lowest nb_eigen levels of each block (all of a smaller block), merged
+        struct Lv {
+            double e;
+            size_t blk;
+            int64_t col;
+        };
+        std::vector<Lv> all;
+        std::vector<bhx::MatrixXcd> bv(O.blocks.size());
+        std::vector<int64_t> off(O.blocks.size() + 1, 0);
+        for (size_t q = 0; q < O.blocks.size(); ++q) {
+            const auto& blk = O.blocks[q];
+            off[q + 1] = off[q] + blk.rows();
+            const int kb = static_cast<int>(std::min<int64_t>(std::min(nb_eigen, 16), blk.rows()));
+            if (kb < 1) continue;
+            std::vector<std::complex<double>> e;
+            if (blk.rows() <= 1500) {
+                // small blocks (down to the one-state vacuum) by the dense host solve, reduced to distinct
+                // levels as the Lanczos path returns them (values within 1e-9 of the spectrum's scale merged)
+                bhx::MatrixXd V;
+                const auto w = exact_eigen(blk, V);
+                const double scale = std::max({std::abs(w.front()), std::abs(w.back()), 1e-300});
+                std::vector<int> pick;
+                for (int i = 0; i < static_cast<int>(w.size()) && static_cast<int>(pick.size()) < kb; ++i)
+                    if (pick.empty() || std::abs(w[i] - w[pick.back()]) > 1e-9 * scale) pick.push_back(i);
+                bv[q].resize(blk.rows(), static_cast<int64_t>(pick.size()));
+                for (size_t i = 0; i < pick.size(); ++i) {
+                    e.emplace_back(w[pick[i]], 0.0);
+                    for (int64_t r = 0; r < blk.rows(); ++r) bv[q](r, static_cast<int64_t>(i)) = V(r, pick[i]);
+                }
+            } else {
+                e = IRLM_eigen(blk, kb, bv[q]);
+            }
+            for (size_t i = 0; i < e.size(); ++i) all.push_back({e[i].real(), q, static_cast<int64_t>(i)});
+        }
+        std::stable_sort(all.begin(), all.end(), [](const Lv& a, const Lv& b) { return a.e < b.e; });
+        const int k = static_cast<int>(std::min<size_t>(static_cast<size_t>(std::min(nb_eigen, 16)), all.size()));
+        if (k < 1) throw std::runtime_error("Eigenvalue computation failed.");
+        eigenvectors.resize(O.rows(), k);
+        std::vector<std::complex<double>> out;
+        for (int i = 0; i < k; ++i) {
+            out.emplace_back(all[i].e, 0.0);
+            const auto& V = bv[all[i].blk];
+            for (int64_t r = 0; r < V.rows(); ++r) eigenvectors(off[all[i].blk] + r, i) = V(r, all[i].col);
+        }
+        return out;
+    }
     if (!O.sec) throw std::invalid_argument("empty operator");
     if (nb_eigen < 1) throw std::invalid_argument("nb_eigen must be positive");
     const int k = std::min(nb_eigen, 16);

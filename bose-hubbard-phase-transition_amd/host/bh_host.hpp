@@ -90,25 +90,40 @@ class Neighbours {
 
 namespace BH {
 
-// Sparse operator cJ T + cU V + cMu N on one sector (D x D, reference basis order)
+// Sparse operator cJ T + cU V + cMu N on one sector (D x D, reference basis order), or a block-diagonal
+// composition of such operators over consecutive boson numbers (max_bosons_hamiltonian: `blocks` set, rows of
+// block b after those of blocks 0..b-1)
 class SparseMatrix {
    public:
     SparseMatrix() = default;
     SparseMatrix(std::shared_ptr<bhx::Sector> s, double cJ, double cU, double cMu) : sec(std::move(s)), J(cJ), U(cU), Mu(cMu) {}
+    explicit SparseMatrix(std::vector<SparseMatrix> b) : blocks(std::move(b)) {}
     int64_t rows() const;
     int64_t cols() const { return rows(); }
     int64_t nonZeros() const;  // stored entries of the materialised matrix (off-diagonal plus diagonal)
     SparseMatrix operator+(const SparseMatrix& o) const;
-    SparseMatrix operator*(double f) const { return SparseMatrix(sec, J * f, U * f, Mu * f); }
+    SparseMatrix operator*(double f) const;
     // CSR materialisation on the host (row_ptr D+1, col, val), rows in the reference basis order
     void to_csr(std::vector<int64_t>& row_ptr, std::vector<int32_t>& col, std::vector<double>& val) const;
     std::shared_ptr<bhx::Sector> sec;
     double J = 0.0, U = 0.0, Mu = 0.0;
+    std::vector<SparseMatrix> blocks;
 };
 inline SparseMatrix operator*(double f, const SparseMatrix& a) { return a * f; }
 
 // (tags, basis): tags[k] = sum_i n_i ln p_i and column k of the m x D basis, in the reference order
 std::pair<std::vector<double>, bhx::MatrixXd> fixed_set_basis(int m, int n);
+
+// Tags and states of every boson number 1 .. n (src/hamiltonian.cpp:151-166): the fixed-N bases one after
+// the other, then the reference's sort_basis (src/hamiltonian.cpp:108-123: argsort of the tags, applied by
+// its cycle-swap loop) over the concatenation.
+std::pair<std::vector<double>, bhx::MatrixXd> max_set_basis(int m, int n);
+
+// Block-diagonal Hamiltonian over the boson numbers n_min .. n_max (src/hamiltonian.cpp:259-287): block N is
+// fixed_bosons_hamiltonian(neighbours, ..., m, N, J, U, mu) with its one-term selection, in the fixed-N
+// reference order; n_min < 0 is taken as 0 and n_max < n_min as n_min, as the reference does.
+SparseMatrix max_bosons_hamiltonian(const std::vector<std::vector<int>>& neighbours, int m, int n_min, int n_max, double J,
+                                    double U, double mu);
 
 // The reference's term selection (src/hamiltonian.cpp:242-255): J != 0 gives the hopping term alone,
 // else U != 0 the interaction term alone, else mu != 0 the chemical term alone, else a message on stderr
@@ -122,7 +137,8 @@ namespace Op {
 
 // Lowest nb_eigen eigenvalues (distinct levels, ascending; at most 16) and eigenvectors (D x nb_eigen)
 // of O by the device Lanczos solver; std::runtime_error("Eigenvalue computation failed.") when it does
-// not converge (src/operator.cpp:26-29).
+// not converge (src/operator.cpp:26-29).  A block-diagonal O is solved block by block and the lowest
+// nb_eigen values over all blocks are returned (levels of different blocks are different states).
 std::vector<std::complex<double>> IRLM_eigen(const BH::SparseMatrix& O, int nb_eigen, bhx::MatrixXcd& eigenvectors);
 
 // All eigenvalues (ascending) and eigenvectors of O by a dense host solve (D <= 1500).

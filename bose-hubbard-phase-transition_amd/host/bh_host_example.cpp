@@ -5,13 +5,66 @@
 // Prints one JSON object: dimension, the first tags, the CSR size of H, the Lanczos levels, the ground
 // state's density-matrix observables and, when the sector is small, the lowest exact eigenvalues.
 //   bhx-host-example M N J U MU [K]
+// With --max, the variable-boson-number calls instead: BH::max_set_basis(M, NMAX) (every tag and state) and
+// Op::IRLM_eigen of BH::max_bosons_hamiltonian(neighbours, M, NMIN, NMAX, J, U, MU) (its size and levels).
+//   bhx-host-example --max M NMIN NMAX J U MU [K]
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
 
 #include "bh_host.hpp"
 
+static int run_max(int argc, char** argv) {
+    if (argc < 8) {
+        std::fprintf(stderr, "usage: bhx-host-example --max M NMIN NMAX J U MU [K]\n");
+        return 1;
+    }
+    const int m = std::atoi(argv[2]), nmin = std::atoi(argv[3]), nmax = std::atoi(argv[4]);
+    const double J = std::atof(argv[5]), U = std::atof(argv[6]), mu = std::atof(argv[7]);
+    const int k = argc > 8 ? std::atoi(argv[8]) : 4;
+    try {
+        Neighbours nb(m);
+        nb.chain_neighbours();
+        auto [tags, basis] = BH::max_set_basis(m, nmax);
+        std::printf("{\"basis_D\": %lld, \"tags\": [", static_cast<long long>(basis.cols()));
+        for (size_t i = 0; i < tags.size(); ++i) std::printf("%s%.17g", i ? ", " : "", tags[i]);
+        std::printf("], \"states\": [");
+        for (int64_t c = 0; c < basis.cols(); ++c)
+            for (int i = 0; i < m; ++i) std::printf("%s%g", c || i ? ", " : "", basis(i, c));
+        const BH::SparseMatrix H = BH::max_bosons_hamiltonian(nb.getNeighbours(), m, nmin, nmax, J, U, mu);
+        std::vector<int64_t> rp;
+        std::vector<int32_t> col;
+        std::vector<double> val;
+        H.to_csr(rp, col, val);
+        bhx::MatrixXcd vecs;
+        const auto ev = Op::IRLM_eigen(H, k, vecs);
+        std::printf("], \"D\": %lld, \"blocks\": %zu, \"nnz\": %lld, \"lanczos\": [", static_cast<long long>(H.rows()),
+                    H.blocks.size(), static_cast<long long>(rp.back()));
+        for (size_t i = 0; i < ev.size(); ++i) std::printf("%s%.17g", i ? ", " : "", ev[i].real());
+        // residual norms |H v - e v| of the returned eigenvectors, through the materialised CSR
+        std::printf("], \"residual\": [");
+        for (size_t q = 0; q < ev.size(); ++q) {
+            double r2 = 0.0, v2 = 0.0;
+            for (int64_t r = 0; r + 1 < static_cast<int64_t>(rp.size()); ++r) {
+                double hv = 0.0;
+                for (int64_t e = rp[r]; e < rp[r + 1]; ++e) hv += val[e] * vecs(col[e], q).real();
+                const double d = hv - ev[q].real() * vecs(r, q).real();
+                r2 += d * d;
+                v2 += vecs(r, q).real() * vecs(r, q).real();
+            }
+            std::printf("%s%.6g", q ? ", " : "", std::sqrt(r2 / v2));
+        }
+        std::printf("]}\n");
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 3;
+    }
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc > 1 && std::string(argv[1]) == "--max") return run_max(argc, argv);
     if (argc < 6) {
         std::fprintf(stderr, "usage: bhx-host-example M N J U MU [K]\n");
         return 1;

@@ -91,6 +91,8 @@ def ref():
         R.ref_square_neighbours.argtypes = [C.c_int, C.c_int, _i32p, _i32p]
         R.ref_fixed_set_basis.restype = C.c_int
         R.ref_fixed_set_basis.argtypes = [C.c_int, C.c_int, _f64p, _f64p]
+        R.ref_max_set_basis.restype = C.c_int
+        R.ref_max_set_basis.argtypes = [C.c_int, C.c_int, _f64p, _f64p]
         R.ref_diagonal_term.restype = C.c_int
         R.ref_diagonal_term.argtypes = [C.c_int, C.c_int, C.c_int, _f64p]
         R.ref_irlm_eigen.restype = C.c_int
@@ -152,6 +154,32 @@ def basis(m: int, n: int, order: int = ORDER_REFERENCE):
     if rc != 0:
         raise ValueError(f"orc_basis failed rc={rc}")
     return st, tags, lex
+
+
+_PRIMES = (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71, 73, 79, 83, 89, 97, 101, 103,
+           107, 109, 113, 127, 131)
+
+
+def max_basis(m: int, n: int):
+    """(states[D,m] uint8, tags[D]) of every boson number 1..n as BH::max_set_basis returns them
+    (src/hamiltonian.cpp:151-166): the fixed-N bases in the reference order, one after the other, then
+    sort_basis (src/hamiltonian.cpp:108-123) over the concatenation.  As in the fixed-N case, its cycle-swap
+    loop places entry p of the concatenation at position sortperm[p], sortperm being the argsort of the tags
+    (equal to the order of the exact keys prod p_i^n_i, which are sorted here)."""
+    parts = [basis(m, N, ORDER_REFERENCE) for N in range(1, n + 1)]
+    st = np.concatenate([q[0] for q in parts]) if parts else np.zeros((0, m), np.uint8)
+    tags = np.concatenate([q[1] for q in parts]) if parts else np.zeros(0)
+    keys = []
+    for r in st:
+        k = 1
+        for i, v in enumerate(r):
+            k *= _PRIMES[i] ** int(v)
+        keys.append(k)
+    perm = np.array(sorted(range(len(keys)), key=keys.__getitem__), dtype=np.int64)
+    out_st, out_tags = np.empty_like(st), np.empty_like(tags)
+    out_st[perm] = st
+    out_tags[perm] = tags
+    return out_st, out_tags
 
 
 class CSR:

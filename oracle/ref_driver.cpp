@@ -66,6 +66,19 @@ int ref_square_neighbours(int m, int closed, int* off, int* sites) {
     }
 }
 
+// BH::max_set_basis (src/hamiltonian.cpp:151-166): tags (D) and states (D x m) of every boson number 1..n;
+// returns D
+int ref_max_set_basis(int m, int n, double* tags, double* states) {
+    auto tb = BH::max_set_basis(m, n);
+    const Eigen::VectorXd& t = tb.first;
+    const Eigen::MatrixXd& b = tb.second;
+    for (Eigen::Index k = 0; k < b.cols(); ++k) {
+        tags[k] = t[k];
+        for (int i = 0; i < m; ++i) states[k * m + i] = b(i, k);
+    }
+    return static_cast<int>(b.cols());
+}
+
 // BH::fixed_set_basis (src/hamiltonian.cpp:143-149): tags (D) and states (D x m, row k = column k)
 int ref_fixed_set_basis(int m, int n, double* tags, double* states) {
     auto tb = BH::fixed_set_basis(m, n);

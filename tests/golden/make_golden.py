@@ -11,6 +11,8 @@ computes deterministically is recorded (its hopping matrix is not: see the oracl
                           (:214-232)
   neighbours.json         Neighbours::chain_neighbours (src/neighbours.cpp:21-34), open and periodic,
                           m = 2..9, and whether square_neighbours throws / what it returns (:40-71)
+  maxbasis_m{m}_n{n}.npz  BH::max_set_basis (src/hamiltonian.cpp:151-166): states[D, m] and tags[D] of every
+                          boson number 1..n after the reference's sort
   spectra_m8_n8.npz       Op::IRLM_eigen (src/operator.cpp:22-33, Spectra GenEigs nev=10) run on the
                           intended operator J T + U V of the chain m=8, n=8 at (J, U) = (1, 1) and (0.1, 1)
 
@@ -29,6 +31,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import oracle as O  # noqa: E402
 
 BASES = [(3, 4), (5, 5), (6, 6), (8, 8)]
+MAX_BASES = [(3, 3), (4, 3), (5, 2)]
 
 
 def main():
@@ -47,6 +50,13 @@ def main():
         assert R.ref_diagonal_term(m, n, 1, dU) == D
         assert R.ref_diagonal_term(m, n, 2, dmu) == D
         np.savez_compressed(os.path.join(HERE, f"diag_m{m}_n{n}.npz"), interaction=dU, chemical=dmu)
+    for m, n in MAX_BASES:
+        D = sum(O.dimension(m, k) for k in range(1, n + 1))
+        tags = np.zeros(D)
+        st = np.zeros(D * m)
+        assert R.ref_max_set_basis(m, n, tags, st) == D
+        np.savez_compressed(os.path.join(HERE, f"maxbasis_m{m}_n{n}.npz"), states=st.reshape(D, m).astype(np.uint8),
+                            tags=tags)
     nb = {"chain": {}, "square": {}}
     for m in range(2, 10):
         for closed in (0, 1):

@@ -77,6 +77,51 @@ def test_host_api_matches_oracle(m, n, J, U, mu):
     assert out["coherence"] == pytest.approx(O.coherence(rho), abs=1e-6)
 
 
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("m,nmin,nmax,J,U,mu,k", [
+    (4, 0, 3, 0.5, 1.0, 0.3, 6),    # J != 0: every block holds the hopping term alone
+    (5, 1, 2, 0.0, 1.0, 0.5, 5),    # U alone: the distinct values of U V per block
+    (3, 2, 3, 0.0, 0.0, 0.7, 4),    # mu alone: -mu N, one level per block
+    (7, 5, 7, 1.0, 0.0, 0.0, 8),    # block N = 7 (D = 1716) by the device Lanczos solver, the rest dense
+    (5, -2, 1, 0.2, 0.0, 0.0, 3),   # n_min < 0 taken as 0
+])
+def test_host_max_bosons(m, nmin, nmax, J, U, mu, k):
+    """BH::max_set_basis against the reference build's fixture and brute force, and Op::IRLM_eigen of
+    BH::max_bosons_hamiltonian (src/hamiltonian.cpp:259-287) against the union of the blocks' dense spectra
+    with the reference's one-term selection (src/hamiltonian.cpp:242-255)"""
+    print(f"host max m={m} n={nmin}..{nmax} J={J} U={U} mu={mu}", flush=True)
+    p = run([EXAMPLE, "--max", str(m), str(nmin), str(nmax), str(J), str(U), str(mu), str(k)])
+    assert p.returncode == 0, p.stderr
+    out = json.loads(p.stdout)
+    st, tags = O.max_basis(m, nmax)
+    fx = os.path.join(GOLDEN, f"maxbasis_m{m}_n{nmax}.npz")
+    if os.path.exists(fx):
+        z = np.load(fx)
+        st, tags = z["states"], z["tags"]
+    assert out["basis_D"] == len(tags) and np.array_equal(out["tags"], tags)
+    assert np.array_equal(np.array(out["states"]).reshape(-1, m), st)
+    off, sites = O.chain_neighbours(m, True)
+    lo = max(nmin, 0)
+    want, D = [], 0
+    for N in range(lo, max(nmax, lo) + 1):
+        h = O.hamiltonian(m, N, off, sites)
+        D += h.D
+        if J != 0.0:
+            ev = np.linalg.eigvalsh(h.dense(J, 0.0))
+        elif U != 0.0:
+            ev = np.linalg.eigvalsh(h.dense(0.0, U))
+        else:
+            ev = np.full(h.D, -mu * N)
+        want += list(O.distinct_levels(ev, k))
+    want = np.sort(want)[:k]
+    assert out["D"] == D and out["blocks"] == max(nmax, lo) - lo + 1
+    assert np.allclose(out["lanczos"], want, rtol=1e-9, atol=1e-9)
+    assert np.all(np.array(out["residual"]) <= 1e-6 * np.maximum(1.0, np.abs(want)))
+
+
 def ref_gap_ratio(levels):
     """The reference's mean gap ratio (src/analysis.cpp:317-318, 433-454) over the given levels."""
     e = np.sort(levels)

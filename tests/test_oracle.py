@@ -2,12 +2,14 @@
 (oracle/_ref, when built), brute force and closed forms.  No GPU."""
 import itertools
 import math
+import os
 
 import numpy as np
 import pytest
 
 import oracle as O
 
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 PRIMES = [2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71, 73, 79, 83, 89, 97]
 
 
@@ -64,6 +66,38 @@ def test_basis_matches_reference_build(m, n):
     rt = np.zeros(D)
     rs = np.zeros(D * m)
     assert R.ref_fixed_set_basis(m, n, rt, rs) == D
+    assert np.array_equal(rs.reshape(D, m).astype(np.uint8), st)
+    assert np.array_equal(rt, tags)
+
+
+@pytest.mark.parametrize("m,n", [(3, 3), (4, 3), (5, 2)])
+def test_max_basis_matches_reference_fixture(m, n):
+    """variable boson number: BH::max_set_basis as the reference build wrote it (tests/golden/maxbasis_*)"""
+    z = np.load(os.path.join(GOLDEN, f"maxbasis_m{m}_n{n}.npz"))
+    st, tags = O.max_basis(m, n)
+    assert np.array_equal(st, z["states"]) and np.array_equal(tags, z["tags"])
+    # brute force: each fixed-N basis in the reference order (lex rank p at position sortperm[p]), the
+    # bases concatenated for N = 1..n, and the same placement once more over the concatenation
+    def placed(seq):
+        out = [None] * len(seq)
+        for p, q in enumerate(sorted(range(len(seq)), key=lambda p: key(seq[p]))):
+            out[q] = seq[p]
+        return out
+
+    concat = [s for N in range(1, n + 1) for s in placed(brute_states(m, N))]
+    assert [tuple(r) for r in st] == placed(concat)
+    assert not np.all(np.diff(tags) > 0)  # the placement is the inverse of the sort: not ascending
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build oracle/_ref not present")
+@pytest.mark.parametrize("m,n", [(2, 3), (3, 3), (4, 4), (6, 3)])
+def test_max_basis_matches_reference_build(m, n):
+    R = O.ref()
+    st, tags = O.max_basis(m, n)
+    D = st.shape[0]
+    rt = np.zeros(D)
+    rs = np.zeros(D * m)
+    assert R.ref_max_set_basis(m, n, rt, rs) == D
     assert np.array_equal(rs.reshape(D, m).astype(np.uint8), st)
     assert np.array_equal(rt, tags)
 
