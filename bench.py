@@ -48,10 +48,67 @@ def cpu_baseline(m, n, J, U, k, gpu_steps, sample_steps=300):
     rc, ev, bd, it = O.lanczos(h, J, U, k, max_iter=sample_steps)
     dt = time.perf_counter() - t0
     per_solve = dt / it * gpu_steps
-    return {"value": 1.0 / per_solve, "unit": "eigensolves/s", "cores": os.cpu_count(), "kind": "port",
+    return {"value": 1.0 / per_solve, "unit": "eigensolves/s", "cores": omp_threads(), "kind": "port",
             "sample": f"first {it} steps of the oracle's FP64 Lanczos (full reorthogonalisation, OpenMP CSR) on the "
                       f"m=12 n=12 sector took {dt:.2f} s; extrapolated linearly to the {gpu_steps} steps of the FP64 "
                       f"GPU solve"}
+
+
+def omp_threads():
+    """threads the oracle's OpenMP loops use on this host"""
+    try:
+        return int(os.environ.get("OMP_NUM_THREADS", ""))
+    except ValueError:
+        return os.cpu_count()
+
+
+def mf_grid(n):
+    """the mean-field grid of the reference CLI (src/analysis.cpp:60-79): mu over [0, 4) in steps 4/n, J over
+    [0, 0.25) in steps 0.25/n, accumulated in double; mu outer"""
+    def rng(lo, hi, d):
+        out, x = [], lo
+        while x < hi:
+            out.append(x)
+            x += d
+        return out
+    mus, Js = rng(0.0, 4.0, 4.0 / n), rng(0.0, 0.25, 0.25 / n)
+    return np.repeat(mus, len(Js)), np.tile(Js, len(mus))
+
+
+# the report's mean-field phase diagram: 10^6 points at precision 1e-5 in 20 min 12 s on the authors' PC
+MF_REFERENCE_S = 20 * 60 + 12
+
+
+def mean_field_leg(ctx, cpu_sample, n=1000, precision=5):
+    """The reference CLI's mean-field phase diagram (-t mean -i 1000 -e 5): 10^6 self-consistent solves in one
+    launch; start values uniform in [0, 1) (seeded).  cpu_sample > 0 also times the oracle's scalar restatement
+    of the same solves on that many grid points (one host thread)."""
+    mu, J = mf_grid(n)
+    psi0 = np.random.default_rng(0).uniform(0.0, 1.0, mu.size)
+    eps = 10.0 ** -precision
+    ctx.mean_field(mu[:4096], J[:4096], psi0[:4096], eps=eps)
+    ctx.set_timing(True)
+    ctx.kernel_stats("scmf", reset=True)
+    t0 = time.perf_counter()
+    r = ctx.mean_field(mu, J, psi0, eps=eps)
+    wall = time.perf_counter() - t0
+    ctx.set_timing(False)
+    cnt, kms = ctx.kernel_stats("scmf")
+    out = {"points": int(mu.size), "precision": precision, "kernel_ms": kms, "wall_s_with_copies": wall,
+           "points_per_s": mu.size / wall, "converged": int(np.sum(r["status"] == 0)),
+           "mean_iters": float(r["iters"].mean()), "max_iters": int(r["iters"].max()), "max_p": int(r["p"].max()),
+           "reference_published_s": MF_REFERENCE_S, "speedup_vs_published": MF_REFERENCE_S / wall}
+    if cpu_sample > 0:
+        import oracle as O
+
+        idx = np.random.default_rng(1).choice(mu.size, cpu_sample, replace=False)
+        t0 = time.perf_counter()
+        for k in idx:
+            O.scmf(mu[k], J[k], 1, psi0[k], eps)
+        dt = time.perf_counter() - t0
+        out["cpu_oracle"] = {"points_per_s": cpu_sample / dt, "cores": 1, "kind": "port",
+                             "sample": f"{cpu_sample} random points of the grid in {dt:.2f} s, one thread"}
+    return out
 
 
 PROFILE = os.path.join(ROOT, "profiles", "k1_lanczos_mixed_b16_chain12.txt")
@@ -129,6 +186,7 @@ def main():
     ap.add_argument("--serve", type=int, default=120, metavar="N",
                     help="requests of the serving stream at 200 req/s offered load (0: skip)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the phase-diagram sweep leg")
+    ap.add_argument("--no-mean-field", action="store_true", help="skip the mean-field phase-diagram leg")
     ap.add_argument("--quick", action="store_true",
                     help="time the mixed batch only (no plain FP64 comparator, SpMV table or CPU leg): for profilers")
     args = ap.parse_args()
@@ -266,6 +324,11 @@ def main():
                  "fp64_reruns": int(rsw["used_fp64"].sum()), "all_ok": bool(np.all(rsw["status"] == 0))}
         log("sweep chain m=10 n=9/10/11:", json.dumps(sweep))
 
+    mf = None
+    if not args.quick and not args.no_mean_field:
+        mf = mean_field_leg(ctx, 0 if args.no_cpu else 20000)
+        log("mean field 1000x1000:", json.dumps(mf))
+
     serve = None
     if args.serve > 0 and not args.quick:
         serve = serving(ctx, args.serve, 200.0)
@@ -297,6 +360,7 @@ def main():
         "spmv": spmv,
         "spmv_square4x4_n10": spmv3 or None,
         "sweep_chain10": sweep,
+        "mean_field_1000x1000": mf,
         "serving": serve,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile()},

@@ -127,6 +127,8 @@ def lib():
                                        C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int]),
             "bh_ham_set_tuned": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
             "bh_spdm": (C.c_int, [_vp, _vp, _vp, _vp]),
+            "bh_mean_field": (C.c_int, [_vp, C.c_int64, _vp, _vp, _vp, C.c_int, C.c_double, C.c_int, C.c_int, _vp, _vp,
+                                        _vp, _vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -244,6 +246,23 @@ class Context:
         out = SweepOut(*[_ptr(res[f]).value for f in ("E", "gap_ratio", "density", "compressibility", "mu_plus",
                                                       "mu_minus", "status", "used_fp64", "width")])
         self.check(lib().bh_sweep(self.h, C.byref(sp), C.byref(o), C.byref(out)))
+        return res
+
+
+    def mean_field(self, mu, J, psi0, q: int = 1, eps: float = 1e-6, max_outer: int = 100000, pmax: int = 32):
+        """Self-consistent mean-field order parameter at the points (mu[k], J[k]) from psi0[k] (see
+        bh_mean_field in bh_capi.h).  Returns a dict of psi, status (0 converged, 1 iteration cap, 2 p cap),
+        iters (self-consistency iterations) and p (last truncation)."""
+        mu = np.ascontiguousarray(np.atleast_1d(mu), np.float64)
+        J = np.ascontiguousarray(np.atleast_1d(J), np.float64)
+        psi0 = np.ascontiguousarray(np.atleast_1d(psi0), np.float64)
+        if not (mu.shape == J.shape == psi0.shape) or mu.ndim != 1:
+            raise ValueError("mu, J and psi0 must be 1-D arrays of one length")
+        n = mu.size
+        res = {"psi": np.zeros(n), "status": np.zeros(n, np.int32), "iters": np.zeros(n, np.int32),
+               "p": np.zeros(n, np.int32)}
+        self.check(lib().bh_mean_field(self.h, n, _ptr(mu), _ptr(J), _ptr(psi0), q, eps, max_outer, pmax,
+                                       _ptr(res["psi"]), _ptr(res["status"]), _ptr(res["iters"]), _ptr(res["p"])))
         return res
 
 

@@ -287,6 +287,21 @@ SparseMatrix max_bosons_hamiltonian(const std::vector<std::vector<int>>& neighbo
     return SparseMatrix(std::move(b));
 }
 
+void h_MF(double psi, int p, double mu, double J, int q, bhx::MatrixXd& h) {
+    if (p < 1) throw std::invalid_argument("h_MF: need p >= 1");
+    const int64_t N = 2 * static_cast<int64_t>(p) + 1;
+    if (h.rows() < N || h.cols() < N) h.resize(N, N);
+    for (int64_t i = 0; i < N; ++i) {
+        const double x = static_cast<double>(i);
+        h(i, i) = -mu * x + 0.5 * x * static_cast<double>(i - 1) + q * J * psi * psi;
+    }
+    for (int64_t i = 0; i + 1 < N; ++i) {
+        const double e = -q * J * psi * std::sqrt(static_cast<double>(i + 1));
+        h(i + 1, i) = e;
+        h(i, i + 1) = e;
+    }
+}
+
 SparseMatrix fixed_bosons_hamiltonian(const std::vector<std::vector<int>>& neighbours, const bhx::MatrixXd& basis,
                                       const std::vector<double>& tags, int m, int n, double J, double U, double mu) {
     auto s = bhx::sector(neighbours, m, n);
@@ -404,6 +419,65 @@ std::vector<double> exact_eigen(const BH::SparseMatrix& O, bhx::MatrixXd& eigenv
 // ---- Analysis ----------------------------------------------------------------------------------------------
 
 namespace Analysis {
+
+double SF_density(const std::vector<double>& phi0, int p) {
+    if (static_cast<int64_t>(phi0.size()) < 2 * static_cast<int64_t>(p) + 1)
+        throw std::invalid_argument("SF_density: phi0 shorter than 2p+1");
+    double a = 0.0;
+    for (int i = 1; i < 2 * p + 1; ++i) a += std::sqrt(static_cast<double>(i)) * phi0[i] * phi0[i - 1];
+    return a;
+}
+
+namespace {
+constexpr int MF_MAX_OUTER = 100000;
+constexpr int MF_PMAX = 32;
+}  // namespace
+
+double SCMF(double mu, double J, int q, double psi0, int precision) {
+    const double eps = std::pow(10.0, -precision);
+    double psi = 0.0;
+    int32_t st = 0;
+    bhx::check(bh_mean_field(bhx::context(), 1, &mu, &J, &psi0, q, eps, MF_MAX_OUTER, MF_PMAX, &psi, &st, nullptr,
+                             nullptr),
+               "SCMF");
+    if (st != 0) throw std::runtime_error("SCMF: no convergence within the iteration caps");
+    return psi;
+}
+
+int64_t mean_field_parameters(int n, int precision, const std::string& out, uint64_t seed) {
+    if (n < 1) throw std::invalid_argument("mean_field_parameters: need n >= 1");
+    const double J_min = 0.0, J_max = 0.25, mu_min = 0.0, mu_max = 4.0;
+    const double dJ = (J_max - J_min) / n, dmu = (mu_max - mu_min) / n;
+    std::vector<double> mus, Js;
+    for (double x = mu_min; x < mu_max; x += dmu) mus.push_back(x);
+    for (double x = J_min; x < J_max; x += dJ) Js.push_back(x);
+    const size_t npts = mus.size() * Js.size();
+    std::vector<double> mu(npts), J(npts), psi0(npts), psi(npts);
+    std::vector<int32_t> st(npts);
+    for (size_t a = 0, k = 0; a < mus.size(); ++a)
+        for (size_t b = 0; b < Js.size(); ++b, ++k) {
+            mu[k] = mus[a];
+            J[k] = Js[b];
+            // SplitMix64: output k of the stream seeded with `seed`, top 53 bits
+            uint64_t z = seed + (static_cast<uint64_t>(k) + 1) * 0x9E3779B97F4A7C15ull;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            z ^= z >> 31;
+            psi0[k] = static_cast<double>(z >> 11) * (1.0 / 9007199254740992.0);
+        }
+    const double eps = std::pow(10.0, -precision);
+    bhx::check(bh_mean_field(bhx::context(), static_cast<int64_t>(npts), mu.data(), J.data(), psi0.data(), 1, eps,
+                             MF_MAX_OUTER, MF_PMAX, psi.data(), st.data(), nullptr, nullptr),
+               "mean_field_parameters");
+    std::ofstream f(out);
+    if (!f) throw std::runtime_error("cannot write " + out);
+    int64_t bad = 0;
+    for (size_t k = 0; k < npts; ++k) {
+        f << mu[k] << " " << J[k] << " " << psi[k] << "\n";
+        bad += st[k] != 0;
+    }
+    return bad;
+}
 
 std::vector<double> gap_ratios(const std::vector<std::complex<double>>& eigenvalues, int nb_eigen) {
     const int ne = std::min<int>(nb_eigen, static_cast<int>(eigenvalues.size()));

@@ -125,6 +125,11 @@ std::pair<std::vector<double>, bhx::MatrixXd> max_set_basis(int m, int n);
 SparseMatrix max_bosons_hamiltonian(const std::vector<std::vector<int>>& neighbours, int m, int n_min, int n_max, double J,
                                     double U, double mu);
 
+// The mean-field single-site Hamiltonian (src/hamiltonian.cpp:290-309), U = 1: fills the tridiagonal of the
+// leading (2p+1) x (2p+1) block of h, d_i = -mu i + i(i-1)/2 + q J psi^2 and h(i, i+1) = h(i+1, i) =
+// -q J psi sqrt(i+1); entries outside it are left as they are.  h is resized (zero-filled) when smaller.
+void h_MF(double psi, int p, double mu, double J, int q, bhx::MatrixXd& h);
+
 // The reference's term selection (src/hamiltonian.cpp:242-255): J != 0 gives the hopping term alone,
 // else U != 0 the interaction term alone, else mu != 0 the chemical term alone, else a message on stderr
 // and the zero matrix.  basis and tags must be the pair fixed_set_basis(m, n) returned.
@@ -165,6 +170,23 @@ double coherence(const bhx::MatrixXcd& spdm);
 // |largest-magnitude eigenvalue of Re rho| / trace rho, as the reference computes it inline
 // (src/analysis.cpp:330-333)
 double condensate_fraction(const bhx::MatrixXcd& spdm);
+
+// <phi0| a |phi0> = sum_{i=1}^{2p} sqrt(i) phi0_i phi0_{i-1} over the occupations 0..2p (src/analysis.cpp:95-105)
+double SF_density(const std::vector<double>& phi0, int p);
+
+// Self-consistent mean-field order parameter psi at (mu, J), U = 1, q neighbours, from the ansatz psi0, both
+// stopping thresholds 10^-precision (src/analysis.cpp:111-178), on the device (bh_mean_field).  Throws
+// std::runtime_error when the point does not converge within the caps (100000 self-consistency iterations,
+// p <= 32); the reference loops without a cap.
+double SCMF(double mu, double J, int q, double psi0, int precision);
+
+// The mean-field phase diagram of the reference CLI (src/analysis.cpp:58-89): mu over [0, 4) in steps 4/n and
+// J over [0, 0.25) in steps 0.25/n (accumulated in double as the reference's range loop does), q = 1, one
+// batch of SCMF solves on the device, written to `out` (default mean_field.txt) as "mu J psi" per point, mu
+// outer.  The reference draws each psi0 uniformly in [0, 1) from a nondeterministic seed; here psi0 of the
+// k-th point (file order) is output k of the SplitMix64 stream seeded with `seed`.  Returns the number of
+// points that did not converge within the caps (their last iterate is written).
+int64_t mean_field_parameters(int n, int precision, const std::string& out = "mean_field.txt", uint64_t seed = 0);
 
 // The exact sweep of the reference CLI (src/analysis.cpp:207-387): periodic chain of m sites, n bosons,
 // a grid over two parameters with the third fixed, mean gap ratio per grid point, written to `out`

@@ -1,12 +1,15 @@
 // bh_phase.cpp -- command-line driver with the reference's options (src/main.cpp:100-177).
 //
 //   bhx-phase -m SITES -n BOSONS -J HOPPING -U INTERACTION -u POTENTIAL -r RANGE -s STEP -f J|U|u -t exact
+//   bhx-phase -t mean -i STEPS -e PRECISION
 //
-// runs Analysis::exact_parameters and writes phase.txt.  Validation follows the reference: the
-// calculation type must be exact or mean, s < r, and the fixed parameter J, U or u, with exit code 1
-// otherwise.  -t mean (the mean-field path) is not part of this build and exits with code 2.  -i and -e
-// are accepted and ignored, as the exact path of the reference ignores them.  Extensions: --out FILE,
-// --prec fp64|mixed, --reference-conventions (the reference's doubled hopping and sum n(n+1) interaction).
+// -t exact runs Analysis::exact_parameters and writes phase.txt; -t mean runs
+// Analysis::mean_field_parameters (an STEPS x STEPS grid, convergence threshold 10^-PRECISION) and writes
+// mean_field.txt (src/main.cpp:179-201).  Validation follows the reference: the calculation type must be
+// exact or mean, s < r, and the fixed parameter J, U or u, with exit code 1 otherwise; -i and -e are read
+// as integers (the reference stores them in ints) and default to 100 and 6 (the reference leaves them
+// uninitialised).  Extensions: --out FILE, --prec fp64|mixed, --reference-conventions (the reference's
+// doubled hopping and sum n(n+1) interaction), --seed N (start values of the mean-field grid).
 #include <getopt.h>
 
 #include <cstdio>
@@ -26,10 +29,11 @@ static void usage() {
                  "  -r, --range X            range of the varying parameters\n"
                  "  -s, --step X             step of the varying parameters (s < r)\n"
                  "  -f, --fixed J|U|u        parameter held fixed\n"
-                 "  -t, --type exact|mean    calculation type (exact only in this build)\n"
-                 "  -i, --iterations N       accepted, unused by the exact path\n"
-                 "  -e, --epsilon X          accepted, unused by the exact path\n"
-                 "      --out FILE           output file (default phase.txt)\n"
+                 "  -t, --type exact|mean    calculation type\n"
+                 "  -i, --iterations N       mean field: N x N grid of (mu, J) (default 100)\n"
+                 "  -e, --epsilon P          mean field: convergence threshold 10^-P (default 6)\n"
+                 "      --out FILE           output file (default phase.txt, mean_field.txt)\n"
+                 "      --seed N             mean field: seed of the start values (default 0)\n"
                  "      --prec fp64|mixed    eigensolver precision (default mixed)\n"
                  "      --reference-conventions  doubled hopping and sum n(n+1) interaction\n"
                  "  -h, --help               this text\n";
@@ -38,8 +42,10 @@ static void usage() {
 int main(int argc, char* argv[]) {
     int m = 0, n = 0;
     double J = 0, U = 0, mu = 0, s = 0, r = 0;
-    std::string fixed_param, calc_type, out = "phase.txt";
-    enum { OPT_OUT = 1000, OPT_PREC, OPT_REFCONV };
+    int iters = 100, precision = 6;
+    unsigned long long seed = 0;
+    std::string fixed_param, calc_type, out;
+    enum { OPT_OUT = 1000, OPT_PREC, OPT_REFCONV, OPT_SEED };
     const char* const short_opts = "m:n:J:U:u:r:s:f:t:i:e:h";
     const option long_opts[] = {{"sites", required_argument, nullptr, 'm'},
                                 {"bosons", required_argument, nullptr, 'n'},
@@ -55,6 +61,7 @@ int main(int argc, char* argv[]) {
                                 {"out", required_argument, nullptr, OPT_OUT},
                                 {"prec", required_argument, nullptr, OPT_PREC},
                                 {"reference-conventions", no_argument, nullptr, OPT_REFCONV},
+                                {"seed", required_argument, nullptr, OPT_SEED},
                                 {"help", no_argument, nullptr, 'h'},
                                 {nullptr, no_argument, nullptr, 0}};
     try {
@@ -71,8 +78,9 @@ int main(int argc, char* argv[]) {
                 case 's': s = std::stod(optarg); break;
                 case 'f': fixed_param = optarg; break;
                 case 't': calc_type = optarg; break;
-                case 'i':
-                case 'e': break;
+                case 'i': iters = static_cast<int>(std::stod(optarg)); break;
+                case 'e': precision = static_cast<int>(std::stod(optarg)); break;
+                case OPT_SEED: seed = std::stoull(optarg); break;
                 case OPT_OUT: out = optarg; break;
                 case OPT_PREC:
                     if (std::strcmp(optarg, "fp64") == 0) bhx::settings().prec = BH_PREC_FP64;
@@ -96,9 +104,23 @@ int main(int argc, char* argv[]) {
         return 1;
     }
     if (calc_type == "mean") {
-        std::cerr << "Error: the mean-field calculation is not part of this build." << std::endl;
-        return 2;
+        if (iters < 1) {
+            std::cerr << "Error: the number of iterations must be positive." << std::endl;
+            return 1;
+        }
+        if (out.empty()) out = "mean_field.txt";
+        try {
+            const int64_t bad = Analysis::mean_field_parameters(iters, precision, out, seed);
+            if (bad > 0)
+                std::cerr << "Warning: " << bad << " points did not converge within the iteration caps." << std::endl;
+        } catch (const std::exception& e) {
+            std::cerr << "Error: " << e.what() << std::endl;
+            return 3;
+        }
+        std::cout << "wrote " << out << std::endl;
+        return 0;
     }
+    if (out.empty()) out = "phase.txt";
     if (s >= r) {
         std::cerr << "Error: s must be smaller than r." << std::endl;
         return 1;

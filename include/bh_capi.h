@@ -109,6 +109,21 @@ int bh_basis_copy_perm(const bh_basis* b, int32_t* lex_of_pos, int32_t* pos_of_l
  * scale (:511) are not reproduced.  The trace is n <phi|phi>. */
 int bh_spdm(bh_ctx* ctx, const bh_basis* b, const double* phi, double* rho);
 
+/* Replaces Analysis::SCMF with Analysis::SF_density and BH::h_MF (include/analysis.hpp:13-16,
+ * src/analysis.cpp:95-178, include/hamiltonian.hpp:103-110, src/hamiltonian.cpp:290-309): the self-consistent
+ * mean-field order parameter psi = <a> at npts independent points (mu[k], J[k]), U = 1, q neighbours per site,
+ * starting from psi0[k].  Each point repeats: ground state of h_MF on 2p+1 occupations for p = 1, 2, ... until
+ * the ground energy changes by at most eps; psi <- <a> of that state; until psi changes by at most eps (the
+ * reference's eps = 10^-precision).  Arrays are host memory of npts entries.  Outputs: psi (required);
+ * status (0 converged, 1 max_outer self-consistency iterations reached without convergence, psi then being
+ * the last iterate, 2 p would exceed pmax), outer iterations run and the last p (each may be NULL).  The
+ * reference has no iteration cap and a 1000 x 1000 buffer (p <= 499); here 2 <= pmax <= 32.  A ground state
+ * that is degenerate (psi = 0 or J = 0 with tied diagonal entries) is taken as the first lowest occupation
+ * state; the reference's Spectra solve returns an unspecified vector of the eigenspace there. */
+int bh_mean_field(bh_ctx* ctx, int64_t npts, const double* mu, const double* J, const double* psi0, int q,
+                  double eps, int max_outer, int pmax, double* psi, int32_t* status, int32_t* iters,
+                  int32_t* p_last);
+
 /* ---- Hamiltonian ------------------------------------------------------------------------------- */
 /* Replaces BH::fixed_bosons_hamiltonian(nbr, basis, tags, m, n, J, U, mu) (include/hamiltonian.hpp:87,
  * src/hamiltonian.cpp:170-256) for the J and U terms: two-pass count / exclusive scan / fill CSR

@@ -38,6 +38,13 @@
  *                            rule (GenEigsBase.h:141-158); plain Lanczos with full
  *                            reorthogonalisation instead of the implicitly restarted Arnoldi
  *   orc_gap_ratios           src/analysis.cpp:433-454
+ *   orc_mf_ground            src/hamiltonian.cpp:290-309 (BH::h_MF) and the ground state the reference asks
+ *                            of Spectra's SymEigsSolver (src/analysis.cpp:128-133), with SF_density
+ *                            (:95-105) of it: lowest eigenvalue by a bracketed Newton iteration on the
+ *                            LDL^T pivots, <a> by Hellmann-Feynman from the same pivots
+ *   orc_scmf                 src/analysis.cpp:95-178 (Analysis::SCMF, SF_density): the self-consistent
+ *                            loop with the reference's two stopping rules; a cap on the outer iterations
+ *                            and on p (the reference has none; status 1 / 2 when reached)
  *   orc_spdm                 src/analysis.cpp:497-512, 562-594 (Analysis::SPDM / braket): <phi| b_i^+ b_j
  *                            |phi> with the intended amplitude sqrt((n_i+1) n_j) and exact key search (the
  *                            reference multiplies by the occupations after the hop, :586-588, finds the
@@ -666,4 +673,183 @@ int orc_spdm(int m, int64_t D, const uint8_t *states, const double *phi, double 
     }
     free(keys);
     return ORC_OK;
+}
+
+/* ---- mean-field single-site problem ---------------------------------------------------------- */
+
+/*
+ * h_MF on the 2p+1 occupations 0..2p (src/hamiltonian.cpp:290-309), U = 1:
+ *   diagonal  d_i = -mu i + i(i-1)/2 + q J psi^2
+ *   coupling  e_i = h(i, i+1) = -q J psi sqrt(i+1),  i = 0 .. 2p-1
+ * sq[i] = sqrt(i) (correctly rounded); every expression keeps one operand order, and the file is built
+ * without floating-point contraction, so each value is one fixed sequence of IEEE operations.
+ */
+static double mf_diag(int i, double mu, double w) { return -mu * (double)i + 0.5 * (double)i * (double)(i - 1) + w; }
+
+/*
+ * LDL^T pivots u_i of T - x I and their x-derivatives; returns 1 when every pivot is positive (x below the
+ * lowest eigenvalue) with *S = sum_i u_i'/u_i = d/dx ln det(T - x I) = -sum_k 1/(lambda_k - x), else 0.
+ * The squared couplings are taken as e_{i-1}^2 = c^2 i (c2 = c^2).
+ *   u_0 = d_0 - x, u_i = (d_i - x) - e_{i-1}^2 / u_{i-1};  u_0' = -1, u_i' = -1 + e_{i-1}^2 u_{i-1}' / u_{i-1}^2
+ */
+static int mf_pivots(int N, double mu, double w, double c2, double x, double *S) {
+    double u = mf_diag(0, mu, w) - x, du = -1.0;
+    if (!(u > 0.0)) return 0;
+    double r = 1.0 / u;
+    double s = du * r;
+    for (int i = 1; i < N; ++i) {
+        const double ee = c2 * (double)i;
+        du = -1.0 + ee * du * r * r;
+        u = (mf_diag(i, mu, w) - x) - ee * r;
+        if (!(u > 0.0)) return 0;
+        r = 1.0 / u;
+        s = s + du * r;
+    }
+    *S = s;
+    return 1;
+}
+
+/*
+ * Logarithmic derivatives of det(T - x I) in x and in the coupling scale c (diagonal held fixed), from the
+ * pivots and their derivatives; 1 when every pivot is positive, else 0.  d(e_{i-1}^2)/dc = 2 c i (tc = 2c).
+ *   u_i'(c) = -(2 c i) / u_{i-1} + e_{i-1}^2 u_{i-1}'(c) / u_{i-1}^2,  u_0'(c) = 0
+ */
+static int mf_logdet_deriv(int N, double mu, double w, double c2, double tc, double x, double *Sx, double *Sc) {
+    double u = mf_diag(0, mu, w) - x, dux = -1.0, duc = 0.0;
+    if (!(u > 0.0)) return 0;
+    double r = 1.0 / u;
+    double sx = dux * r, sc = 0.0;
+    for (int i = 1; i < N; ++i) {
+        const double ee = c2 * (double)i;
+        const double dee = tc * (double)i;
+        dux = -1.0 + ee * dux * r * r;
+        duc = -(dee * r) + ee * duc * r * r;
+        u = (mf_diag(i, mu, w) - x) - ee * r;
+        if (!(u > 0.0)) return 0;
+        r = 1.0 / u;
+        sx = sx + dux * r;
+        sc = sc + duc * r;
+    }
+    *Sx = sx;
+    *Sc = sc;
+    return 1;
+}
+
+/*
+ * Ground state of h_MF(psi, p): returns its energy and *sf = <phi|a|phi> (what SF_density computes from the
+ * eigenvector, src/analysis.cpp:95-105).
+ *   energy: Newton's iteration on ln det(T - x I) from below (x <- x - 1/S).  Every root is real, so from
+ *           below the lowest one each exact step stays below it and the iterates rise monotonically to it
+ *           (quadratically once close).  The start is `hint` when its pivots are all positive (the caller
+ *           passes the previous p's eigenvalue less 1e-3 (|lambda| + 1): more occupations only lower it), else
+ *           the Gershgorin lower bound; hi = min_i d_i, or the rejected hint, lies at or above the root.  An
+ *           iterate whose pivots are not all positive, or a step reaching hi, can only come from rounding at
+ *           the root: the iterate (resp. hi) is returned; so is the last iterate when a step no longer moves x.
+ *   <a>   : Hellmann-Feynman.  The couplings are c sqrt(i+1), so d lambda/dc = 2 sum_i sqrt(i+1) phi_i phi_{i+1}
+ *           = 2 <a>, and d lambda/dc = -(d ln det/dc) / (d ln det/dx) as x -> lambda.  The ratio f(x) is
+ *           evaluated at lambda - delta and lambda - 2 delta, delta = 1e-9 (|lambda| + 1), and extrapolated
+ *           linearly to x = lambda (2 f(lambda - delta) - f(lambda - 2 delta)), which leaves an O(delta^2)
+ *           error; no eigenvector is formed.
+ *   psi = 0 or J = 0 (diagonal h): the first smallest diagonal entry and <a> = 0.
+ */
+double orc_mf_ground(double psi, int p, double mu, double J, int q, const double *sq, double hint, double *sf) {
+    const int N = 2 * p + 1;
+    const double c = (double)(-q) * J * psi;
+    const double w = (double)q * J * psi * psi;
+    double lo = 0.0, hi = 0.0;
+    for (int i = 0; i < N; ++i) {
+        const double d = mf_diag(i, mu, w);
+        const double a0 = i > 0 ? fabs(c * sq[i]) : 0.0;
+        const double a1 = i < N - 1 ? fabs(c * sq[i + 1]) : 0.0;
+        const double g = d - a0 - a1;
+        if (i == 0 || g < lo) lo = g;
+        if (i == 0 || d < hi) hi = d;
+    }
+    *sf = 0.0;
+    if (c == 0.0) return hi;
+    const double c2 = c * c, tc = 2.0 * c;
+    double S = 0.0;
+    /* a valid lower end: the Gershgorin bound, moved down if rounding put it at the eigenvalue */
+    for (int t = 0; t < 64 && !mf_pivots(N, mu, w, c2, lo, &S); ++t) lo = lo - (fabs(lo) + 1.0);
+    double xk = lo;
+    if (hint > lo && hint < hi) {
+        if (mf_pivots(N, mu, w, c2, hint, &S))
+            xk = hint;
+        else
+            hi = hint;
+    }
+    double lam = xk;
+    for (int it = 0; it < 200; ++it) {
+        if (!mf_pivots(N, mu, w, c2, xk, &S)) break; /* a Newton iterate at the root within rounding */
+        const double xn = xk - 1.0 / S;
+        if (!(xn < hi)) { /* the root lies at hi within rounding */
+            lam = hi;
+            break;
+        }
+        if (!(xn > xk)) break; /* no progress */
+        xk = xn;
+        lam = xk;
+    }
+    double delta = 1e-9 * (fabs(lam) + 1.0);
+    for (int t = 0; t < 8; ++t) {
+        double sx1, sc1, sx2, sc2;
+        if (mf_logdet_deriv(N, mu, w, c2, tc, lam - delta, &sx1, &sc1) &&
+            mf_logdet_deriv(N, mu, w, c2, tc, lam - 2.0 * delta, &sx2, &sc2)) {
+            const double f1 = -0.5 * sc1 / sx1;
+            const double f2 = -0.5 * sc2 / sx2;
+            *sf = 2.0 * f1 - f2;
+            break;
+        }
+        delta = 2.0 * delta;
+    }
+    return lam;
+}
+
+/* the start hint for p + 1 after the eigenvalue e of p */
+static double mf_hint(double e) { return e - 1e-3 * (fabs(e) + 1.0); }
+
+/*
+ * Analysis::SCMF (src/analysis.cpp:111-178) at one (mu, J) point, U = 1:
+ *   repeat: p = 1, E0 = ground(psi, 1); repeat p += 1, E = ground(psi, p) until |E0 - E| <= eps (E0 <- E);
+ *           psi' = <a> of the last ground state; stop when |psi' - psi| <= eps (psi <- psi')
+ * Outputs psi, the outer iterations run and the last p.  Returns 0 (converged), 1 (max_outer outer iterations
+ * without convergence: psi is the last iterate), 2 (p would exceed pmax), -ORC_EINVAL or -ORC_ENOMEM.
+ */
+int orc_scmf(double mu, double J, int q, double psi0, double eps, int max_outer, int pmax, double *psi_out,
+             int *iters_out, int *p_out) {
+    if (pmax < 2 || pmax > 4096 || max_outer < 1) return -ORC_EINVAL;
+    const int NMAX = 2 * pmax + 1;
+    double *sq = (double *)malloc(sizeof(double) * (size_t)(NMAX + 1));
+    if (!sq) return -ORC_ENOMEM;
+    for (int i = 0; i <= NMAX; ++i) sq[i] = sqrt((double)i);
+    double psi = psi0, sf = 0.0;
+    int status = 1, outer = 0, p = 1;
+    while (outer < max_outer) {
+        ++outer;
+        p = 1;
+        double e0 = orc_mf_ground(psi, p, mu, J, q, sq, -INFINITY, &sf);
+        double tmp;
+        do {
+            if (p + 1 > pmax) {
+                status = 2;
+                goto done;
+            }
+            ++p;
+            const double e = orc_mf_ground(psi, p, mu, J, q, sq, mf_hint(e0), &sf);
+            tmp = fabs(e0 - e);
+            e0 = e;
+        } while (tmp > eps);
+        tmp = fabs(sf - psi);
+        psi = sf;
+        if (tmp <= eps) {
+            status = 0;
+            break;
+        }
+    }
+done:
+    *psi_out = psi;
+    if (iters_out) *iters_out = outer;
+    if (p_out) *p_out = p;
+    free(sq);
+    return status;
 }

@@ -69,6 +69,12 @@ def lib():
         L.orc_gap_ratio.argtypes = [C.c_int, _f64p, C.c_void_p]
         L.orc_spdm.restype = C.c_int
         L.orc_spdm.argtypes = [C.c_int, C.c_int64, _u8p, _f64p, _f64p]
+        L.orc_mf_ground.restype = C.c_double
+        L.orc_mf_ground.argtypes = [C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, _f64p, C.c_double,
+                                    C.POINTER(C.c_double)]
+        L.orc_scmf.restype = C.c_int
+        L.orc_scmf.argtypes = [C.c_double, C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                               C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         _LIB = L
     return _LIB
 
@@ -93,6 +99,12 @@ def ref():
         R.ref_fixed_set_basis.argtypes = [C.c_int, C.c_int, _f64p, _f64p]
         R.ref_max_set_basis.restype = C.c_int
         R.ref_max_set_basis.argtypes = [C.c_int, C.c_int, _f64p, _f64p]
+        R.ref_scmf.restype = C.c_double
+        R.ref_scmf.argtypes = [C.c_double, C.c_double, C.c_int, C.c_double, C.c_int]
+        R.ref_h_mf.restype = None
+        R.ref_h_mf.argtypes = [C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, _f64p]
+        R.ref_sf_density.restype = C.c_double
+        R.ref_sf_density.argtypes = [_f64p, C.c_int]
         R.ref_diagonal_term.restype = C.c_int
         R.ref_diagonal_term.argtypes = [C.c_int, C.c_int, C.c_int, _f64p]
         R.ref_irlm_eigen.restype = C.c_int
@@ -300,3 +312,40 @@ def distinct_levels(ev, k: int, rel: float = 1e-9):
         if len(out) == k:
             break
     return np.array(out)
+
+
+# ---- mean-field single-site problem -----------------------------------------------------------
+
+MF_CONVERGED, MF_MAX_OUTER, MF_MAX_P = 0, 1, 2
+
+
+def mf_ground(psi: float, p: int, mu: float, J: float, q: int = 1, hint: float = -np.inf):
+    """(E0, <a>): ground energy of h_MF(psi, p) (src/hamiltonian.cpp:290-309) and SF_density of its ground
+    state (src/analysis.cpp:95-105); hint: a start for the eigenvalue iteration (used when below it)."""
+    N = 2 * p + 1
+    sq = np.sqrt(np.arange(N + 1, dtype=np.float64))
+    sf = C.c_double()
+    e = lib().orc_mf_ground(psi, p, mu, J, q, sq, hint, C.byref(sf))
+    return e, sf.value
+
+
+def scmf(mu: float, J: float, q: int = 1, psi0: float = 0.5, eps: float = 1e-6, max_outer: int = 100000,
+         pmax: int = 32):
+    """(status, psi, outer iterations, last p) of Analysis::SCMF (src/analysis.cpp:111-178) at one point."""
+    psi, it, p = C.c_double(), C.c_int(), C.c_int()
+    st = lib().orc_scmf(mu, J, q, psi0, eps, max_outer, pmax, C.byref(psi), C.byref(it), C.byref(p))
+    if st < 0:
+        raise ValueError(f"orc_scmf failed rc={st}")
+    return st, psi.value, it.value, p.value
+
+
+def mf_psi0(seed: int, idx):
+    """Start values of the mean-field grid: output idx of the SplitMix64 stream seeded with seed
+    (z = seed + (idx + 1) * 0x9E3779B97F4A7C15, then its two xor-shift-multiply rounds), top 53 bits as a
+    uniform value in [0, 1)."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (np.asarray(idx, np.uint64) + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)

@@ -34,7 +34,7 @@ def test_host_library_loads():
     (["-m", "4", "-n", "4", "-J", "1", "-r", "0.5", "-s", "1", "-f", "J", "-t", "exact"], 1, "s must be smaller than r"),
     (["-m", "4", "-n", "4", "-J", "1", "-r", "1", "-s", "0.5", "-f", "X", "-t", "exact"], 1,
      "fixed parameter must be J, U or u"),
-    (["-m", "4", "-n", "4", "-t", "mean"], 2, "mean-field"),
+    (["-t", "mean", "-i", "0"], 1, "number of iterations must be positive"),
     (["-m", "x"], 1, "invalid numeric argument"),
 ])
 def test_cli_validation(args, code, msg):
@@ -120,6 +120,35 @@ def test_host_max_bosons(m, nmin, nmax, J, U, mu, k):
     assert out["D"] == D and out["blocks"] == max(nmax, lo) - lo + 1
     assert np.allclose(out["lanczos"], want, rtol=1e-9, atol=1e-9)
     assert np.all(np.array(out["residual"]) <= 1e-6 * np.maximum(1.0, np.abs(want)))
+
+
+def mf_grid(n):
+    """the reference's grid (src/analysis.cpp:60-79): mu over [0, 4), J over [0, 0.25), steps accumulated"""
+    def rng(lo, hi, d):
+        out, x = [], lo
+        while x < hi:
+            out.append(x)
+            x += d
+        return out
+    mus, Js = rng(0.0, 4.0, 4.0 / n), rng(0.0, 0.25, 0.25 / n)
+    return np.repeat(mus, len(Js)), np.tile(Js, len(mus))
+
+
+@pytest.mark.gpu
+def test_cli_mean_field_file(tmp_path):
+    """-t mean writes mean_field.txt as the reference does ("mu J psi", default stream formatting, mu outer);
+    psi of every point equals the oracle's from the same start value"""
+    n, prec, seed = 24, 6, 3
+    print(f"cli mean field n={n}", flush=True)
+    p = run([CLI, "-t", "mean", "-i", str(n), "-e", str(prec), "--seed", str(seed)], cwd=tmp_path)
+    assert p.returncode == 0, p.stderr
+    lines = (tmp_path / "mean_field.txt").read_text().splitlines()
+    mu, J = mf_grid(n)
+    assert len(lines) == mu.size
+    psi0 = O.mf_psi0(seed, np.arange(mu.size))
+    for k, ln in enumerate(lines):
+        _, psi, _, _ = O.scmf(mu[k], J[k], 1, psi0[k], 10.0 ** -prec)
+        assert ln == f"{mu[k]:g} {J[k]:g} {psi:g}", (k, ln)
 
 
 def ref_gap_ratio(levels):

@@ -332,3 +332,102 @@ def test_spdm_closed_forms():
     rho = O.spdm(states, phi)
     assert np.allclose(rho, np.eye(m)) and O.coherence(rho) == 0.0
     assert O.condensate_fraction(rho) == pytest.approx(1.0 / m)
+
+
+# ---- mean-field single-site problem (Analysis::SCMF) --------------------------------------------------------
+
+def mott_lobe_jc(mu):
+    """Mean-field Mott-lobe boundary for U = 1 (Fisher et al. 1989; van Oosten et al. 2001; the report's
+    eq. 10): in lobe g (g - 1 < mu < g), qJ_c = (g - mu)(mu - g + 1) / (mu + 1)"""
+    g = np.floor(mu) + 1.0
+    return (g - mu) * (mu - g + 1.0) / (mu + 1.0)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build oracle/_ref not present")
+@pytest.mark.parametrize("p", [1, 2, 5, 12])
+def test_mf_ground_state_of_reference_h_mf(p):
+    """the oracle's ground state of h_MF against LAPACK on the matrix BH::h_MF fills"""
+    R = O.ref()
+    rng = np.random.default_rng(p)
+    for _ in range(20):
+        psi, mu, J = rng.uniform(0, 2), rng.uniform(0, 4), rng.uniform(0, 0.3)
+        N = 2 * p + 1
+        h = np.zeros(N * N)
+        R.ref_h_mf(psi, p, mu, J, 1, h)
+        h = h.reshape(N, N)
+        w, v = np.linalg.eigh(h)
+        e, sf = O.mf_ground(psi, p, mu, J, 1)
+        assert e == pytest.approx(w[0], rel=1e-13, abs=1e-13)
+        # <a> (Hellmann-Feynman) against the reference's SF_density of LAPACK's ground state
+        want = R.ref_sf_density(np.ascontiguousarray(v[:, 0]), p)
+        assert sf == pytest.approx(want, rel=1e-11, abs=1e-13)
+        # a start below the eigenvalue, and one above it (rejected), give the same pair
+        for hint in (w[0] - 1e-3 * (abs(w[0]) + 1), w[0] + 0.5, w[0]):
+            eh, sfh = O.mf_ground(psi, p, mu, J, 1, hint)
+            assert eh == pytest.approx(w[0], rel=1e-13, abs=1e-13)
+            assert sfh == pytest.approx(want, rel=1e-11, abs=1e-13)
+
+
+def test_mf_ground_state_closed_forms():
+    # psi = 0: h is diagonal, the ground state the lowest -mu n + n(n-1)/2, <a> = 0
+    for mu in (0.3, 1.5, 2.7):
+        e, sf = O.mf_ground(0.0, 4, mu, 0.1)
+        nstar = int(np.floor(mu)) + 1 if mu > 0 else 0
+        assert e == pytest.approx(-mu * nstar + nstar * (nstar - 1) / 2, abs=1e-15) and sf == 0.0
+    # mu = -1/2, weak coupling c = -qJ psi from |0> to |1> (gap 1/2): second order E0 = qJ psi^2 - c^2 / (1/2)
+    # and first order <a> = phi_0 phi_1 = -c / (1/2)
+    c = 1e-3
+    e, sf = O.mf_ground(1.0, 6, -0.5, c)
+    assert e == pytest.approx(c - c * c / 0.5, rel=1e-6)
+    assert sf == pytest.approx(c / 0.5, rel=1e-5)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build oracle/_ref not present")
+@pytest.mark.parametrize("precision", [5, 6, 8])
+def test_scmf_matches_reference_build(precision):
+    """psi of Analysis::SCMF (Spectra ground states) against the oracle at random points; J > 0 so that the
+    ground state is not degenerate"""
+    R = O.ref()
+    rng = np.random.default_rng(precision)
+    for _ in range(25):
+        mu, J, psi0 = rng.uniform(0, 4), rng.uniform(0.002, 0.25), rng.uniform(0.01, 1)
+        st, psi, it, p = O.scmf(mu, J, 1, psi0, 10.0 ** -precision)
+        assert st == O.MF_CONVERGED
+        assert R.ref_scmf(mu, J, 1, psi0, precision) == pytest.approx(psi, abs=1e-8), (mu, J, psi0)
+
+
+def test_scmf_mott_lobes():
+    """psi vanishes inside the mean-field Mott lobes and not outside them (closed-form boundary)"""
+    eps = 1e-6
+    rng = np.random.default_rng(11)
+    inside = outside = 0
+    for _ in range(300):
+        mu, J = rng.uniform(0.05, 3.95), rng.uniform(0.0, 0.25)
+        if abs(mu - round(mu)) < 0.05:
+            continue
+        jc = mott_lobe_jc(mu)
+        st, psi, it, p = O.scmf(mu, J, 1, rng.uniform(0.05, 1.0), eps)
+        assert st == O.MF_CONVERGED
+        if J < 0.8 * jc:
+            inside += 1
+            assert abs(psi) <= 10 * eps, (mu, J, psi)
+        elif J > 1.25 * jc:
+            outside += 1
+            assert psi > 1e-3, (mu, J, psi)
+    assert inside > 20 and outside > 20
+
+
+def test_scmf_atomic_limit_and_caps():
+    # J = 0: h is diagonal, psi = 0 after one self-consistency step for a non-integer mu
+    st, psi, it, p = O.scmf(1.5, 0.0, 1, 0.7, 1e-6)
+    assert (st, psi, it) == (O.MF_CONVERGED, 0.0, 2)
+    # caps: one outer iteration, or p <= 2, are reported
+    assert O.scmf(0.5, 0.2, 1, 0.3, 1e-10, max_outer=1)[0] == O.MF_MAX_OUTER
+    assert O.scmf(0.5, 0.2, 1, 0.3, 1e-12, pmax=2)[0] == O.MF_MAX_P
+
+
+def test_mf_start_values_splitmix64():
+    # SplitMix64 with seed 0: first outputs 0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4 (published test vector)
+    z = O.mf_psi0(0, [0, 1])
+    assert int(z[0] * 2.0 ** 53) == 0xE220A8397B1DCDAF >> 11
+    assert int(z[1] * 2.0 ** 53) == 0x6E789E6AA1B965F4 >> 11
