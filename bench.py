@@ -187,6 +187,8 @@ def main():
                     help="requests of the serving stream at 200 req/s offered load (0: skip)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the phase-diagram sweep leg")
     ap.add_argument("--no-mean-field", action="store_true", help="skip the mean-field phase-diagram leg")
+    ap.add_argument("--mean-field-only", action="store_true",
+                    help="run only the mean-field leg (1000 x 1000 points, precision 1e-5) and print its JSON: for profilers")
     ap.add_argument("--quick", action="store_true",
                     help="time the mixed batch only (no plain FP64 comparator, SpMV table or CPU leg): for profilers")
     args = ap.parse_args()
@@ -204,6 +206,12 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         log(f"copied {args.copy_calib} bytes")
+        return
+
+    if args.mean_field_only:
+        ctx = bhx.Context(0)
+        print(json.dumps(mean_field_leg(ctx, 0 if args.no_cpu else 20000)))
+        ctx.close()
         return
 
     m, n, J0, U, mu, k = 12, 12, 0.1, 1.0, 0.5, 4
