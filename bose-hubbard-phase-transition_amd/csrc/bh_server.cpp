@@ -25,6 +25,8 @@
 namespace {
 
 using Clock = std::chrono::steady_clock;
+constexpr int64_t SRV_SMALL_D = 1 << 15;  // sectors driven to completion within a pass (see worker_loop)
+constexpr double SRV_SMALL_MS = 5.0;
 using SectorKey = std::tuple<int, int, int, int, int>;  // kind, Lx, Ly, periodic, n
 
 struct Pending {
@@ -48,6 +50,7 @@ struct Sector {
     double last_use = 0.0;
     size_t ham_bytes = 0, eng_bytes[2] = {0, 0};
     bool never_fits = false;
+    bool inflight[2] = {false, false};       // a chunk of the engine's steps is on its stream
 };
 
 }  // namespace
@@ -229,9 +232,88 @@ void serve_full(bh_server* S, Sector& sc, const Pending& pd, int prior_iters, do
     complete(S, pd, rc == BH_OK ? st : rc, ev.data(), rs.data(), k, prior_iters + it, 1, t_start);
 }
 
+// Fills the free lanes of a sector's engine for precision p from its queue (the engine is idle).
+void admit_queued(bh_server* S, Sector& sc, int p) {
+    const int k = S->opts.lanczos.k;
+    while (!sc.queue[p].empty()) {
+        Pending pd = sc.queue[p].front();
+        int rc = ensure_ham(S, sc);
+        if (rc == BH_OK && pd.req.J == 0.0) {
+            sc.queue[p].pop_front();
+            serve_diagonal(S, sc, pd);
+            continue;
+        }
+        if (rc == BH_OK) rc = ensure_engine(S, sc, p);
+        if (rc != BH_OK) {
+            if (rc == BH_EBUDGET && !sc.never_fits) break;  // retry when memory frees up
+            sc.queue[p].pop_front();
+            complete(S, pd, rc, nullptr, nullptr, k, 0, p, S->now());
+            continue;
+        }
+        int lane = -1;
+        for (int l = 0; l < static_cast<int>(sc.lane_busy[p].size()); ++l)
+            if (!sc.lane_busy[p][l]) {
+                lane = l;
+                break;
+            }
+        if (lane < 0) break;
+        sc.queue[p].pop_front();
+        if (sc.eng[p]->admit(lane, pd.req.J, pd.req.U) != BH_OK) {
+            complete(S, pd, BH_EHIP, nullptr, nullptr, k, 0, p, S->now());
+            continue;
+        }
+        sc.lane_busy[p][lane] = 1;
+        sc.lane_req[p][lane] = pd;
+        sc.lane_start[p][lane] = S->now();
+        sc.last_use = S->now();
+    }
+}
+
+// Analyses a sector engine whose chunk has completed: retires converged lanes, completes their requests or
+// queues their reruns (FP64, then FP64 with full reorthogonalisation).
+void harvest(bh_server* S, Sector& sc, int p) {
+    const int k = S->opts.lanczos.k;
+    std::vector<int> fin;
+    if (sc.eng[p]->check(fin) != BH_OK || fin.empty()) return;
+    std::vector<LaneResult> res;
+    if (sc.eng[p]->finish(fin, res, false) != BH_OK) return;
+    for (size_t i = 0; i < fin.size(); ++i) {
+        const int lane = fin[i];
+        Pending pd = sc.lane_req[p][lane];
+        sc.lane_busy[p][lane] = 0;
+        const LaneResult& L = res[i];
+        // full reorthogonalisation needs no certificate; the mixed engine and the FP64 engine with selective
+        // orthogonalisation do
+        const bool fullre_eng =
+            sc.eng[p]->fp64() && bh_lz_fullre(sc.ham->D, p == 0 ? S->opts.lanczos : bh_fallback_opts(S->opts.lanczos));
+        const bool ok = fullre_eng ||
+                        bh_lane_accepted(L.status, L.evals.data(), L.errest.data(), k, S->opts.lanczos.accept_rel);
+        if (p == 0 && !ok && S->opts.lanczos.fallback >= 0) {
+            pd.prec = BH_PREC_FP64;
+            sc.queue[1].push_back(pd);  // rerun in FP64
+            continue;
+        }
+        if (p == 1 && !ok) {
+            serve_full(S, sc, pd, L.iters, sc.lane_start[p][lane]);
+            continue;
+        }
+        std::vector<double> ev(k, NAN), rs(k, NAN);
+        for (int q = 0; q < k && q < static_cast<int>(L.evals.size()); ++q) {
+            ev[q] = L.evals[q] - pd.req.mu * pd.req.n;
+            rs[q] = L.resid[q];
+        }
+        complete(S, pd, L.status, ev.data(), rs.data(), k, L.iters, sc.eng[p]->fp64() ? 1 : 0, sc.lane_start[p][lane]);
+    }
+    sc.last_use = S->now();
+}
+
+// The engines run independently: each pass of the loop visits every sector engine; one whose chunk is still
+// on its stream is skipped, one whose stream is idle is analysed (finished lanes retired), refilled from its
+// queue and given its next chunk at once.  A small sector thus cycles through its steps while a large one's
+// chunk is running, and no engine waits for another's.  Each engine still runs the same sequence of chunks
+// and analyses, so a request's result does not depend on the other requests.
 void worker_loop(bh_server* S) {
     (void)hipSetDevice(S->ctx->device);
-    const int k = S->opts.lanczos.k;
     for (;;) {
         {
             std::unique_lock<std::mutex> lk(S->mu);
@@ -254,88 +336,35 @@ void worker_loop(bh_server* S) {
                 sp->queue[pd.prec == BH_PREC_FP64 ? 1 : 0].push_back(pd);
             }
         }
-        // admission: fill free lanes of every sector from its queues
+        bool progressed = false;
         for (auto it = S->sectors.begin(); it != S->sectors.end(); ++it) {
             Sector& sc = *it->second;
             for (int p = 0; p < 2; ++p) {
-                while (!sc.queue[p].empty()) {
-                    Pending pd = sc.queue[p].front();
-                    int rc = ensure_ham(S, sc);
-                    if (rc == BH_OK && pd.req.J == 0.0) {
-                        sc.queue[p].pop_front();
-                        serve_diagonal(S, sc, pd);
-                        continue;
-                    }
-                    if (rc == BH_OK) rc = ensure_engine(S, sc, p);
-                    if (rc != BH_OK) {
-                        if (rc == BH_EBUDGET && !sc.never_fits) break;  // retry when memory frees up
-                        sc.queue[p].pop_front();
-                        complete(S, pd, rc, nullptr, nullptr, k, 0, p, S->now());
-                        continue;
-                    }
-                    int lane = -1;
-                    for (int l = 0; l < static_cast<int>(sc.lane_busy[p].size()); ++l)
-                        if (!sc.lane_busy[p][l]) {
-                            lane = l;
+                // a small sector's chunks take well under a millisecond: it is driven chunk after chunk for up
+                // to SRV_SMALL_MS instead of one chunk per pass, so that it does not wait for the host work of
+                // every other sector (CSR builds, tuning, finishes) between its chunks
+                const bool small = sc.ham && sc.ham->D <= SRV_SMALL_D;
+                const auto t0 = Clock::now();
+                for (;;) {
+                    if (sc.inflight[p]) {
+                        if (small)
+                            (void)hipStreamSynchronize(sc.stream[p]);
+                        else if (hipStreamQuery(sc.stream[p]) == hipErrorNotReady)
                             break;
-                        }
-                    if (lane < 0) break;
-                    sc.queue[p].pop_front();
-                    if (sc.eng[p]->admit(lane, pd.req.J, pd.req.U) != BH_OK) {
-                        complete(S, pd, BH_EHIP, nullptr, nullptr, k, 0, p, S->now());
-                        continue;
+                        sc.inflight[p] = false;
+                        progressed = true;
+                        harvest(S, sc, p);
                     }
-                    sc.lane_busy[p][lane] = 1;
-                    sc.lane_req[p][lane] = pd;
-                    sc.lane_start[p][lane] = S->now();
-                    sc.last_use = S->now();
+                    admit_queued(S, sc, p);
+                    if (!(sc.eng[p] && sc.eng[p]->active() > 0 && sc.eng[p]->run(sc.eng[p]->suggest_chunk()) == BH_OK))
+                        break;
+                    sc.inflight[p] = true;
+                    progressed = true;
+                    if (!small || std::chrono::duration<double, std::milli>(Clock::now() - t0).count() > SRV_SMALL_MS) break;
                 }
             }
         }
-        // one chunk for every engine with active lanes (streams overlap on the device), then the checks
-        std::vector<std::pair<Sector*, int>> running;
-        for (auto& kv : S->sectors)
-            for (int p = 0; p < 2; ++p)
-                if (kv.second->eng[p] && kv.second->eng[p]->active() > 0) {
-                    if (kv.second->eng[p]->run(kv.second->eng[p]->suggest_chunk()) == BH_OK)
-                        running.push_back({kv.second.get(), p});
-                }
-        for (auto& rp : running) {
-            Sector& sc = *rp.first;
-            const int p = rp.second;
-            std::vector<int> fin;
-            if (sc.eng[p]->check(fin) != BH_OK || fin.empty()) continue;
-            std::vector<LaneResult> res;
-            if (sc.eng[p]->finish(fin, res, false) != BH_OK) continue;
-            for (size_t i = 0; i < fin.size(); ++i) {
-                const int lane = fin[i];
-                Pending pd = sc.lane_req[p][lane];
-                sc.lane_busy[p][lane] = 0;
-                const LaneResult& L = res[i];
-                // full reorthogonalisation needs no certificate; the mixed engine and the FP64 engine with
-                // selective orthogonalisation do
-                const bool fullre_eng = sc.eng[p]->fp64() && bh_lz_fullre(sc.ham->D, p == 0 ? S->opts.lanczos
-                                                                                              : bh_fallback_opts(S->opts.lanczos));
-                const bool ok = fullre_eng ||
-                                bh_lane_accepted(L.status, L.evals.data(), L.errest.data(), k, S->opts.lanczos.accept_rel);
-                if (p == 0 && !ok && S->opts.lanczos.fallback >= 0) {
-                    pd.prec = BH_PREC_FP64;
-                    sc.queue[1].push_back(pd);  // rerun in FP64
-                    continue;
-                }
-                if (p == 1 && !ok) {
-                    serve_full(S, sc, pd, L.iters, sc.lane_start[p][lane]);
-                    continue;
-                }
-                std::vector<double> ev(k, NAN), rs(k, NAN);
-                for (int q = 0; q < k && q < static_cast<int>(L.evals.size()); ++q) {
-                    ev[q] = L.evals[q] - pd.req.mu * pd.req.n;
-                    rs[q] = L.resid[q];
-                }
-                complete(S, pd, L.status, ev.data(), rs.data(), k, L.iters, sc.eng[p]->fp64() ? 1 : 0, sc.lane_start[p][lane]);
-            }
-            sc.last_use = S->now();
-        }
+        if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
 }
 
