@@ -194,7 +194,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported. Solves are independent: a "
-              "multi-GPU run shards batches or requests across devices (see DESIGN.md).", file=sys.stderr)
+              "multi-GPU run shards batches or requests across devices.", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_calib:
