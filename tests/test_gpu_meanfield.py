@@ -50,6 +50,21 @@ def test_scmf_grid_bitwise(ctx, n, prec):
     assert np.array_equal(r["psi"], psi)
 
 
+@pytest.mark.parametrize("q", [2, 4, 6])
+def test_scmf_coordination_bitwise(ctx, q):
+    """q neighbours per site (square and cubic lattices): the lobes shrink to qJ_c; still bit for bit"""
+    print(f"scmf q={q}", flush=True)
+    rng = np.random.default_rng(q)
+    mu = rng.uniform(0, 4, 600)
+    J = rng.uniform(0, 0.25 / q, 600)
+    psi0 = rng.uniform(0, 1, 600)
+    r = ctx.mean_field(mu, J, psi0, q=q, eps=1e-7)
+    out = np.array([O.scmf(mu[k], J[k], q, psi0[k], 1e-7) for k in range(mu.size)], dtype=object)
+    assert np.array_equal(r["status"], out[:, 0].astype(np.int32))
+    assert np.array_equal(r["psi"], out[:, 1].astype(np.float64))
+    assert np.array_equal(r["iters"], out[:, 2].astype(np.int32))
+
+
 def test_scmf_caps_and_edge_points(ctx):
     print("scmf caps", flush=True)
     rng = np.random.default_rng(2)

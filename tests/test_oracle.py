@@ -383,17 +383,17 @@ def test_mf_ground_state_closed_forms():
 
 
 @pytest.mark.skipif(not O.ref_available(), reason="reference build oracle/_ref not present")
-@pytest.mark.parametrize("precision", [5, 6, 8])
-def test_scmf_matches_reference_build(precision):
+@pytest.mark.parametrize("precision,q", [(5, 1), (6, 1), (8, 1), (6, 4)])
+def test_scmf_matches_reference_build(precision, q):
     """psi of Analysis::SCMF (Spectra ground states) against the oracle at random points; J > 0 so that the
     ground state is not degenerate"""
     R = O.ref()
-    rng = np.random.default_rng(precision)
+    rng = np.random.default_rng(precision + 10 * q)
     for _ in range(25):
-        mu, J, psi0 = rng.uniform(0, 4), rng.uniform(0.002, 0.25), rng.uniform(0.01, 1)
-        st, psi, it, p = O.scmf(mu, J, 1, psi0, 10.0 ** -precision)
+        mu, J, psi0 = rng.uniform(0, 4), rng.uniform(0.002, 0.25) / q, rng.uniform(0.01, 1)
+        st, psi, it, p = O.scmf(mu, J, q, psi0, 10.0 ** -precision)
         assert st == O.MF_CONVERGED
-        assert R.ref_scmf(mu, J, 1, psi0, precision) == pytest.approx(psi, abs=1e-8), (mu, J, psi0)
+        assert R.ref_scmf(mu, J, q, psi0, precision) == pytest.approx(psi, abs=1e-8), (mu, J, psi0)
 
 
 def test_scmf_mott_lobes():
