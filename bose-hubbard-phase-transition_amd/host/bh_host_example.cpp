@@ -8,10 +8,15 @@
 // With --max, the variable-boson-number calls instead: BH::max_set_basis(M, NMAX) (every tag and state) and
 // Op::IRLM_eigen of BH::max_bosons_hamiltonian(neighbours, M, NMIN, NMAX, J, U, MU) (its size and levels).
 //   bhx-host-example --max M NMIN NMAX J U MU [K]
+// With --hmf, BH::h_MF(psi, p, mu, J, q) as a dense (2p+1)^2 matrix and Analysis::SF_density of the vector
+// v_i = 1 / (i + 1) (host only, no device); with --scmf, Analysis::SCMF(mu, J, q, psi0, precision).
+//   bhx-host-example --hmf PSI P MU J Q
+//   bhx-host-example --scmf MU J Q PSI0 PRECISION
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
+#include <vector>
 
 #include "bh_host.hpp"
 
@@ -63,8 +68,50 @@ static int run_max(int argc, char** argv) {
     return 0;
 }
 
+static int run_hmf(int argc, char** argv) {
+    if (argc < 7) {
+        std::fprintf(stderr, "usage: bhx-host-example --hmf PSI P MU J Q\n");
+        return 1;
+    }
+    const double psi = std::atof(argv[2]), mu = std::atof(argv[4]), J = std::atof(argv[5]);
+    const int p = std::atoi(argv[3]), q = std::atoi(argv[6]);
+    try {
+        bhx::MatrixXd h;
+        BH::h_MF(psi, p, mu, J, q, h);
+        const int N = 2 * p + 1;
+        std::vector<double> v(N);
+        for (int i = 0; i < N; ++i) v[i] = 1.0 / (i + 1);
+        std::printf("{\"h\": [");
+        for (int i = 0; i < N; ++i)
+            for (int c = 0; c < N; ++c) std::printf("%s%.17g", i || c ? ", " : "", h(i, c));
+        std::printf("], \"sf_density\": %.17g}\n", Analysis::SF_density(v, p));
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 3;
+    }
+    return 0;
+}
+
+static int run_scmf(int argc, char** argv) {
+    if (argc < 7) {
+        std::fprintf(stderr, "usage: bhx-host-example --scmf MU J Q PSI0 PRECISION\n");
+        return 1;
+    }
+    try {
+        const double psi = Analysis::SCMF(std::atof(argv[2]), std::atof(argv[3]), std::atoi(argv[4]), std::atof(argv[5]),
+                                          std::atoi(argv[6]));
+        std::printf("{\"psi\": %.17g}\n", psi);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 3;
+    }
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc > 1 && std::string(argv[1]) == "--max") return run_max(argc, argv);
+    if (argc > 1 && std::string(argv[1]) == "--hmf") return run_hmf(argc, argv);
+    if (argc > 1 && std::string(argv[1]) == "--scmf") return run_scmf(argc, argv);
     if (argc < 6) {
         std::fprintf(stderr, "usage: bhx-host-example M N J U MU [K]\n");
         return 1;

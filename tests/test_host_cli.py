@@ -80,6 +80,31 @@ def test_host_api_matches_oracle(m, n, J, U, mu):
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
+@pytest.mark.skipif(not O.ref_available(), reason="reference build oracle/_ref not present")
+@pytest.mark.parametrize("psi,p,mu,J,q", [(0.3, 1, 0.5, 0.1, 1), (1.2, 4, 2.7, 0.05, 4), (0.0, 3, 1.5, 0.2, 2)])
+def test_host_h_mf_and_sf_density_match_reference(psi, p, mu, J, q):
+    """BH::h_MF and Analysis::SF_density of the host layer against the reference's own (host only)"""
+    out = run([EXAMPLE, "--hmf", repr(psi), str(p), repr(mu), repr(J), str(q)])
+    assert out.returncode == 0, out.stderr
+    got = json.loads(out.stdout)
+    N = 2 * p + 1
+    want = np.zeros(N * N)
+    O.ref().ref_h_mf(psi, p, mu, J, q, want)
+    assert np.array_equal(np.array(got["h"]), want)
+    v = 1.0 / np.arange(1, N + 1)
+    assert got["sf_density"] == O.ref().ref_sf_density(np.ascontiguousarray(v), p)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mu,J,q,psi0,prec", [(0.5, 0.2, 1, 0.3, 6), (2.5, 0.02, 4, 0.8, 8), (1.5, 0.0, 1, 0.5, 5)])
+def test_host_scmf(mu, J, q, psi0, prec):
+    """Analysis::SCMF of the host layer (one point on the device) against the oracle"""
+    out = run([EXAMPLE, "--scmf", repr(mu), repr(J), str(q), repr(psi0), str(prec)])
+    assert out.returncode == 0, out.stderr
+    st, psi, _, _ = O.scmf(mu, J, q, psi0, 10.0 ** -prec)
+    assert st == 0 and json.loads(out.stdout)["psi"] == psi
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("m,nmin,nmax,J,U,mu,k", [
     (4, 0, 3, 0.5, 1.0, 0.3, 6),    # J != 0: every block holds the hopping term alone
