@@ -105,6 +105,10 @@ def ref():
         R.ref_h_mf.argtypes = [C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, _f64p]
         R.ref_sf_density.restype = C.c_double
         R.ref_sf_density.argtypes = [_f64p, C.c_int]
+        R.ref_gap_ratios.restype = None
+        R.ref_gap_ratios.argtypes = [_f64p, C.c_int, _f64p]
+        R.ref_coherence.restype = C.c_double
+        R.ref_coherence.argtypes = [_f64p, C.c_int]
         R.ref_diagonal_term.restype = C.c_int
         R.ref_diagonal_term.argtypes = [C.c_int, C.c_int, C.c_int, _f64p]
         R.ref_irlm_eigen.restype = C.c_int

@@ -266,6 +266,29 @@ def test_gap_ratio():
     assert O.gap_ratio(ev[::-1].copy()) == pytest.approx(r)  # sorted internally
 
 
+@pytest.mark.skipif(not O.ref_available(), reason="reference build oracle/_ref not present")
+def test_gap_ratios_and_coherence_match_reference_build():
+    """Analysis::gap_ratios and Analysis::coherence as the reference computes them"""
+    R = O.ref()
+    rng = np.random.default_rng(5)
+    for n in (3, 4, 7, 10, 20):
+        for deg in (False, True):
+            ev = rng.standard_normal(n)
+            if deg:
+                ev[1] = ev[0]                      # a zero gap
+                ev[-1] = ev[-2]
+            want = np.zeros(n - 2)
+            R.ref_gap_ratios(np.ascontiguousarray(ev), n, want)
+            got = np.zeros(n - 2)
+            mean = O.lib().orc_gap_ratio(n, np.ascontiguousarray(ev), got.ctypes.data_as(O.C.c_void_p))
+            assert np.array_equal(got, want)
+            assert mean == pytest.approx(float(np.mean(want)), rel=1e-15, abs=1e-300)
+    for m in (2, 5, 12):
+        a = rng.standard_normal((m, m))
+        rho = a @ a.T
+        assert O.coherence(rho) == pytest.approx(R.ref_coherence(np.ascontiguousarray(rho).reshape(-1), m), rel=1e-14)
+
+
 def test_start_vector_park_miller():
     r = O.start_vector(3, 0)
     # seed 0 -> 1; x1 = 16807, x2 = 16807^2 mod (2^31-1), x3 = 16807^3 mod (2^31-1)
