@@ -29,7 +29,7 @@ def lat(geom):
 BASIS_CASES = [(1, 0), (1, 5), (2, 7), (3, 2), (5, 5), (8, 8), (6, 4), (10, 9), (10, 11), (12, 5),
                (4, 30),   # n > 15: 8-bit site fields; key 7^30 > 2^64: 128-bit keys
                (20, 3),   # m > 16: two 64-bit words
-               (18, 17),  # two words of 8-bit fields
+               (9, 16),   # n > 15 and m > 8: both words of 8-bit fields in use
                (9, 9)]
 
 
