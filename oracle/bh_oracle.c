@@ -769,14 +769,21 @@ double orc_mf_ground(double psi, int p, double mu, double J, int q, const double
     if (c == 0.0) return hi;
     const double c2 = c * c, tc = 2.0 * c;
     double S = 0.0;
-    /* a valid lower end: the Gershgorin bound, moved down if rounding put it at the eigenvalue */
-    for (int t = 0; t < 64 && !mf_pivots(N, mu, w, c2, lo, &S); ++t) lo = lo - (fabs(lo) + 1.0);
+    /* the start: the hint when its pivots are all positive, else the Gershgorin bound (moved down if rounding
+       put it at the eigenvalue) */
     double xk = lo;
+    int have_start = 0;
     if (hint > lo && hint < hi) {
-        if (mf_pivots(N, mu, w, c2, hint, &S))
+        if (mf_pivots(N, mu, w, c2, hint, &S)) {
             xk = hint;
-        else
+            have_start = 1;
+        } else {
             hi = hint;
+        }
+    }
+    if (!have_start) {
+        for (int t = 0; t < 64 && !mf_pivots(N, mu, w, c2, lo, &S); ++t) lo = lo - (fabs(lo) + 1.0);
+        xk = lo;
     }
     double lam = xk;
     for (int it = 0; it < 200; ++it) {
