@@ -43,7 +43,7 @@ class LanczosOpts(C.Structure):
     _fields_ = [("k", C.c_int), ("prec", C.c_int), ("max_iter", C.c_int), ("chunk", C.c_int),
                 ("tol", C.c_double), ("dedup_rel", C.c_double), ("accept_rel", C.c_double),
                 ("seed", C.c_uint64), ("fallback", C.c_int), ("want_vectors", C.c_int), ("ncv", C.c_int),
-                ("fullre_d", C.c_int)]
+                ("fullre_d", C.c_int), ("distinct", C.c_int)]
 
 
 class SweepSpec(C.Structure):
@@ -144,6 +144,9 @@ def _bind_optional(L):
         "bh_lanczos_default_opts": (None, [C.POINTER(LanczosOpts)]),
         "bh_lanczos": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, C.POINTER(LanczosOpts), C.POINTER(LanczosResult)]),
         "bh_lanczos_copy_vectors": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
+        "bh_lanczos_tridiag": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_int, C.c_uint64, _vp, _vp,
+                                         C.POINTER(C.c_int), _vp]),
+        "bh_dense_eigen": (C.c_int, [_vp, C.c_int64, _vp, _vp, _vp, _vp, _vp]),
         "bh_tridiag_eigvals": (C.c_int, [C.c_int, _vp, _vp, _vp]),
         "bh_sweep": (C.c_int, [_vp, C.POINTER(SweepSpec), C.POINTER(LanczosOpts), C.POINTER(SweepOut)]),
         "bh_server_create": (C.c_int, [_vp, C.POINTER(ServerOpts), C.POINTER(_vp)]),
@@ -248,6 +251,18 @@ class Context:
         self.check(lib().bh_sweep(self.h, C.byref(sp), C.byref(o), C.byref(out)))
         return res
 
+
+    def dense_eigen(self, row_ptr, col, val, vectors: bool = False):
+        """All eigenvalues (and the column eigenvectors) of a symmetric CSR matrix by the device dense solve
+        (bh_dense_eigen)."""
+        rp = np.ascontiguousarray(row_ptr, np.int64)
+        n = rp.size - 1
+        col = np.ascontiguousarray(col, np.int32) if len(col) else np.zeros(1, np.int32)
+        val = np.ascontiguousarray(val, np.float64) if len(val) else np.zeros(1, np.float64)
+        w = np.zeros(n)
+        V = np.zeros((n, n)) if vectors else None
+        self.check(lib().bh_dense_eigen(self.h, n, _ptr(rp), _ptr(col), _ptr(val), _ptr(w), _ptr(V) if vectors else None))
+        return (w, V.T.copy()) if vectors else w  # V holds the eigenvectors as columns in column-major order
 
     def mean_field(self, mu, J, psi0, q: int = 1, eps: float = 1e-6, max_outer: int = 100000, pmax: int = 32):
         """Self-consistent mean-field order parameter at the points (mu[k], J[k]) from psi0[k] (see
@@ -410,6 +425,18 @@ class Hamiltonian:
         v = np.zeros((B, k, self.D))
         self.ctx.check(lib().bh_lanczos_copy_vectors(self.ctx.h, B, k, _ptr(v)))
         return v
+
+    def tridiag(self, J: float, U: float, nb_iter: int, seed: int = 0, basis: bool = False):
+        """(alpha, beta, basis or None): nb_iter FP64 Lanczos steps with full reorthogonalisation
+        (bh_lanczos_tridiag), cut where beta < 1e-6; beta[0] = 0, beta[t] couples t-1 and t."""
+        n = min(nb_iter, self.D)
+        a, b = np.zeros(n), np.zeros(n)
+        steps = C.c_int()
+        V = np.zeros((n, self.D)) if basis else None
+        self.ctx.check(lib().bh_lanczos_tridiag(self.ctx.h, self.h, float(J), float(U), nb_iter, seed, _ptr(a), _ptr(b),
+                                                C.byref(steps), _ptr(V) if basis else None))
+        s = steps.value
+        return a[:s], b[:s], (V[:s] if basis else None)
 
 
 def device_name(device: int = 0) -> str:

@@ -41,6 +41,9 @@ class LzEngineBase {
     virtual int suggest_chunk() const = 0;  // steps until the next useful analysis
     virtual size_t device_bytes() const = 0;
     virtual bool fp64() const = 0;  // Krylov vectors stored in FP64
+    // lane 0's Lanczos coefficients after `steps` steps without analyses (alpha[t], t < steps; beta[t] couples
+    // t - 1 and t, t = 1 .. steps) and, when basis is given, its normalised vectors v_t as [steps][D]
+    virtual int export_tridiag(int steps, double* alpha, double* beta, double* basis) = 0;
 };
 
 // sectors with D at most opts.fullre_d (0: 16384, < 0: none) run FP64 with full reorthogonalisation

@@ -41,8 +41,14 @@
 
 constexpr int LZ_WG = 512;         // threads per workgroup of the Lanczos kernels (8 waves)
 constexpr int LZ_GRID_MAX = 1024;  // workgroups that write partial sums
-constexpr int LZ_LMAX = 16;        // locked Ritz vectors per solve
-constexpr int LZ_NVMAX = 16;       // Ritz vectors formed per pass
+constexpr int LZ_LMAX = 16;        // locked (and frozen) vectors per lane with selective orthogonalisation
+constexpr int LZ_NVMAX = 16;       // vectors per register-tiled launch (locks formed at once, Gram and combine tiles)
+constexpr int LZ_KMAX = 64;        // wanted eigenvalues per solve
+constexpr int LZ_CMAX = 2 * LZ_KMAX + 8;  // candidates of the FP64 finish (frozen vectors + the last pass's)
+constexpr int LZ_SO_KMAX = 6;      // multiplicities: larger k runs with full reorthogonalisation (frozen + locked
+                                   // vectors of a selectively orthogonalised lane share its LZ_LMAX slots)
+constexpr int LZ_PASS_MAX = 64;    // deflation passes per solve
+constexpr int LZ_GRAM_VALS = 2 * 2 * LZ_NVMAX;  // partial sums per lane of one Gram-tile launch (NI 2 x NV 16, or 4 x 8)
 constexpr int LZ_RED_WG = 1024;    // reduction workgroup: 16 waves, one value per wave
 constexpr int LZ_GRAPH_STEPS = 8;  // Lanczos steps per captured HIP graph
 constexpr int LZ_K2_GRID = 1024;   // K2 workgroups at most (rows of partial sums for R2; 512 and 2048 measured slower)
@@ -425,31 +431,35 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_b(LzArgs<XT> a) {
 }
 
 // Full reorthogonalisation (small sectors), classical Gram-Schmidt pass over the whole basis: workgroup t
-// computes dd[t][b] = scale_t <V[t][.][b], v_{j+1}[.][b]> for the lanes whose step j has t <= j.
+// computes dd[t][b] = scale_t <V[t][.][b], v_{j+1}[.][b]> for the lanes whose step j has t <= j; workgroups
+// t >= maxit do the same for the lane's frozen vectors Y[t - maxit] (t - maxit < nlock[b]; unit norm), so that
+// a deflation pass stays orthogonal to the eigenvectors the earlier passes froze.
 template <typename XT, int B>
 __global__ void __launch_bounds__(LZ_WG) k_lz_fr_dots(LzArgs<XT> a, double* __restrict__ dd) {
     __shared__ double lds[LZ_WG / 64 * B];
     const int t = blockIdx.x;
     const int b = threadIdx.x % B;
     const int j = a.st.step[b];
-    const bool act = a.st.active[b] != 0 && j < a.maxit && t <= j;
+    const bool frozen = t >= a.maxit;
+    const bool act = a.st.active[b] != 0 && j < a.maxit && (frozen ? t - a.maxit < a.st.nlock[b] : t <= j);
     double acc = 0.0;
     if (act) {
-        const XT* vt = a.V + static_cast<int64_t>(t) * a.vs;
+        const XT* vt = frozen ? a.Y + static_cast<int64_t>(t - a.maxit) * a.vs : a.V + static_cast<int64_t>(t) * a.vs;
         const XT* vn = a.V + static_cast<int64_t>(j + 1) * a.vs;
         for (int64_t r = threadIdx.x / B; r < a.D; r += LZ_WG / B) acc += to_d(vt[r * B + b]) * to_d(vn[r * B + b]);
-        acc *= a.st.scale[t * B + b];
+        if (!frozen) acc *= a.st.scale[t * B + b];
     }
     lz_block_put<B>(acc, 0, lds, dd, B);
 }
 
-// v_{j+1} -= sum_{t <= j} dd[t] scale_t V[t]; the last pass also refreshes the current vector and writes the
-// partials of ||v_{j+1}||^2 (as K2 does)
+// v_{j+1} -= sum_{t <= j} dd[t] scale_t V[t] + sum_{l < nlock} dd[maxit + l] Y[l]; the last pass also refreshes
+// the current vector and writes the partials of ||v_{j+1}||^2 (as K2 does)
 template <typename XT, int B>
 __global__ void __launch_bounds__(LZ_WG) k_lz_fr_axpy(LzArgs<XT> a, const double* __restrict__ dd, int last) {
     __shared__ double lds[LZ_WG / 64 * B];
     const int b = threadIdx.x % B;
     const int j = a.st.step[b];
+    const int nfz = a.st.nlock[b];
     const bool act = a.st.active[b] != 0 && j < a.maxit;
     constexpr int ROWS = LZ_WG / B;
     const int64_t ntile = (a.D + ROWS - 1) / ROWS;
@@ -464,6 +474,7 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_fr_axpy(LzArgs<XT> a, const double
             const int64_t idx = r * B + b;
             double w = to_d(vn[idx]);
             for (int t = 0; t <= j; ++t) w -= dd[t * B + b] * a.st.scale[t * B + b] * to_d(a.V[static_cast<int64_t>(t) * a.vs + idx]);
+            for (int l = 0; l < nfz; ++l) w -= dd[(a.maxit + l) * B + b] * to_d(a.Y[static_cast<int64_t>(l) * a.vs + idx]);
             const XT wr = static_cast<XT>(w);
             vn[idx] = wr;
             if (last) {
@@ -877,6 +888,154 @@ __global__ void k_lz_scale(YT* __restrict__ Y, int64_t vs, int nv, const double*
     Y[idx] = static_cast<YT>(to_d(Y[idx]) * f[i * B + b]);
 }
 
+// out-of-place form of k_lz_combine for more than LZ_NVMAX inputs: T[c][r][b] = sum_{i < nin} X[i][r][b] Z[(i nout + c) B + b]
+template <int B>
+__global__ void __launch_bounds__(256) k_lz_combine_wide(const double* __restrict__ X, double* __restrict__ T, int64_t vs,
+                                                         const double* __restrict__ Z, int nin, int nout) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= vs) return;
+    const int b = static_cast<int>(idx % B);
+    for (int c = 0; c < nout; ++c) {
+        double acc = 0.0;
+        for (int i = 0; i < nin; ++i) acc += X[static_cast<int64_t>(i) * vs + idx] * Z[(static_cast<int64_t>(i) * nout + c) * B + b];
+        T[static_cast<int64_t>(c) * vs + idx] = acc;
+    }
+}
+
+// Gram-matrix block of the finish for more than LZ_NVMAX candidates: rows i0 .. i0 + NI - 1 and columns
+// q0 .. q0 + NV - 1 of G = Y^T Y and A = Y^T H Y per lane, as per-workgroup partials of the values
+// ((i - i0) 2 + w) NV + (q - q0) (w = 0: G, 1: A)
+template <int B, int NV, int NI>
+__global__ void __launch_bounds__(LZ_WG) k_lz_gram_blk(const double* __restrict__ Y, const double* __restrict__ HY, int64_t D,
+                                                       int nv, int i0, int q0, double* part) {
+    __shared__ double lds[LZ_PUT_LDS];
+    const int b = threadIdx.x % B;
+    constexpr int ROWS = LZ_WG / B;
+    const int64_t vs = D * B;
+    const int64_t ntile = (D + ROWS - 1) / ROWS;
+    const int64_t per = (ntile + gridDim.x - 1) / gridDim.x;
+    const int64_t t0 = min(static_cast<int64_t>(blockIdx.x) * per, ntile), t1 = min(t0 + per, ntile);
+    double g[NI * 2 * NV];
+#pragma unroll
+    for (int q = 0; q < NI * 2 * NV; ++q) g[q] = 0.0;
+    for (int64_t t = t0; t < t1; ++t) {
+        const int64_t r = t * ROWS + threadIdx.x / B;
+        if (r >= D) continue;
+        const int64_t idx = r * B + b;
+        double yi[NI];
+#pragma unroll
+        for (int ii = 0; ii < NI; ++ii) yi[ii] = i0 + ii < nv ? Y[static_cast<int64_t>(i0 + ii) * vs + idx] : 0.0;
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+            if (q0 + q < nv) {
+                const double yq = Y[static_cast<int64_t>(q0 + q) * vs + idx], hq = HY[static_cast<int64_t>(q0 + q) * vs + idx];
+#pragma unroll
+                for (int ii = 0; ii < NI; ++ii) {
+                    g[(ii * 2) * NV + q] += yi[ii] * yq;
+                    g[(ii * 2 + 1) * NV + q] += yi[ii] * hq;
+                }
+            }
+    }
+    lz_block_put_multi<B, NI * 2 * NV>(g, lds, part, NI * 2 * NV * B);
+}
+
+// ---- deflation passes ---------------------------------------------------------------------------------------
+// A lane that has converged its pass freezes the levels it found (their Ritz vectors go to Y after the vectors
+// frozen before) and restarts from a fresh start vector orthogonal to every frozen vector; the kernels below
+// form that vector and keep frozen sets orthonormal.  cnt[b] < 0 leaves lane b untouched.
+
+// Z[r][b] = x_{off + r + 1} / (2^31 - 1) - 0.5 for lane b, where x_i = 16807^i x_0 mod (2^31 - 1): the Park-Miller
+// sequence of the start vector continued past its first `off` values (jumped to by modular powers)
+template <int B>
+__global__ void __launch_bounds__(256) k_lz_pm_fill(double* __restrict__ Z, int64_t D, int b, uint64_t x0, uint64_t off) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= D) return;
+    const uint64_t p = 2147483647ull;
+    uint64_t e = off + static_cast<uint64_t>(r) + 1, base = 16807ull, pw = 1;
+    while (e) {
+        if (e & 1) pw = (pw * base) % p;
+        base = (base * base) % p;
+        e >>= 1;
+    }
+    const uint64_t x = (pw * x0) % p;
+    Z[r * B + b] = static_cast<double>(x) / static_cast<double>(p) - 0.5;
+}
+
+// partials of <Y_l, Z> over Y's slots l < cnt[b] (values l B + b of rows of width lmax B)
+template <typename XT, int B>
+__global__ void __launch_bounds__(LZ_WG) k_lz_ydots(const XT* __restrict__ Y, const double* __restrict__ Z, int64_t D,
+                                                    const int* __restrict__ cnt, int lmax, double* part) {
+    __shared__ double lds[LZ_WG / 64 * B];
+    const int b = threadIdx.x % B;
+    constexpr int ROWS = LZ_WG / B;
+    const int64_t vs = D * B;
+    const int64_t ntile = (D + ROWS - 1) / ROWS;
+    const int64_t per = (ntile + gridDim.x - 1) / gridDim.x;
+    const int64_t t0 = min(static_cast<int64_t>(blockIdx.x) * per, ntile), t1 = min(t0 + per, ntile);
+    const int nl = cnt[b];
+    for (int l = 0; l < lmax; ++l) {
+        double s = 0.0;
+        if (l < nl) {
+            for (int64_t t = t0; t < t1; ++t) {
+                const int64_t r = t * ROWS + threadIdx.x / B;
+                if (r < D) s += to_d(Y[static_cast<int64_t>(l) * vs + r * B + b]) * Z[r * B + b];
+            }
+        }
+        lz_block_put<B>(s, l, lds, part, lmax * B);
+    }
+}
+
+// Z -= sum_{l < cnt[b]} c[l B + b] Y_l
+template <typename XT, int B>
+__global__ void __launch_bounds__(256) k_lz_yaxpy(const XT* __restrict__ Y, double* __restrict__ Z, int64_t vs,
+                                                  const int* __restrict__ cnt, const double* __restrict__ c) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= vs) return;
+    const int b = static_cast<int>(idx % B);
+    const int nl = cnt[b];
+    if (nl <= 0) return;
+    double z = Z[idx];
+    for (int l = 0; l < nl; ++l) z -= c[l * B + b] * to_d(Y[static_cast<int64_t>(l) * vs + idx]);
+    Z[idx] = z;
+}
+
+// base[slot[b]][r][b] = Z[r][b] f[b] in storage precision (and the lane's current vector when xc is given)
+template <typename XT, int B>
+__global__ void __launch_bounds__(256) k_lz_defl_store(const double* __restrict__ Z, XT* base, int64_t vs,
+                                                       const int* __restrict__ slot, const double* __restrict__ f, XT* xc) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= vs) return;
+    const int b = static_cast<int>(idx % B);
+    const int s = slot[b];
+    if (s < 0) return;
+    const XT x = static_cast<XT>(Z[idx] * f[b]);
+    base[static_cast<int64_t>(s) * vs + idx] = x;
+    if (xc) xc[idx] = x;
+}
+
+// Y[s][r][b] = Y[src[s B + b]][r][b] for s < ns (src -1: slot untouched; src[s] >= s and ascending per lane, so the
+// moves in ascending s never read a slot an earlier move wrote)
+template <typename XT, int B>
+__global__ void __launch_bounds__(256) k_lz_compact(XT* Y, int64_t vs, const int* __restrict__ src, int ns) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= vs) return;
+    const int b = static_cast<int>(idx % B);
+    for (int q = 0; q < ns; ++q) {
+        const int a = src[q * B + b];
+        if (a >= 0 && a != q) Y[static_cast<int64_t>(q) * vs + idx] = Y[static_cast<int64_t>(a) * vs + idx];
+    }
+}
+
+// Yf[i][r][b] = Y[i][r][b] for i < nf[b] (the frozen vectors among the finish's candidates, in FP64)
+template <typename XT, int B>
+__global__ void __launch_bounds__(256) k_lz_copy_frozen(const XT* __restrict__ Y, double* __restrict__ Yf, int64_t vs,
+                                                        const int* __restrict__ nf, int nfmax) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= vs) return;
+    const int n = nf[idx % B];
+    for (int i = 0; i < nfmax && i < n; ++i) Yf[static_cast<int64_t>(i) * vs + idx] = to_d(Y[static_cast<int64_t>(i) * vs + idx]);
+}
+
 // ---- host driver: the Lanczos engine --------------------------------------------------------------------
 //
 // An engine holds B lanes over one Hamiltonian.  A lane runs one solve (J_b, U_b); lanes are admitted
@@ -945,6 +1104,11 @@ struct RunHost {
     // log(bound / threshold) at the previous analysis, and the lane's total steps then (-1: none)
     double prev_gap = 0.0;
     int prev_tot = -1;
+    // deflation passes (eigenvalues with multiplicity): passes completed before the current one, and the
+    // vectors they froze (Y slots 0 .. nfz - 1 of the lane) with their Ritz values
+    int pass = 0;
+    int nfz = 0;
+    std::vector<double> fz_theta;
 };
 
 struct Analysis {
@@ -994,7 +1158,14 @@ static int lz_slots(int64_t D, const bh_lanczos_opts& o) {
 // basis (j D B words per step) is cheap.
 bool bh_lz_fullre(int64_t D, const bh_lanczos_opts& o) {
     const int64_t lim = o.fullre_d > 0 ? o.fullre_d : (o.fullre_d < 0 ? 0 : 16384);
-    return D <= lim;
+    return D <= lim || (!o.distinct && o.k > LZ_SO_KMAX);
+}
+
+// Y slots of a lane: locked and frozen vectors under selective orthogonalisation; frozen vectors only (up to
+// 2k + 8: a pass freezes only values among the lowest k) under full reorthogonalisation
+static int lz_ymax(int64_t D, const bh_lanczos_opts& o) {
+    if (!bh_lz_fullre(D, o)) return LZ_LMAX;
+    return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(D, std::max(LZ_LMAX, 2 * o.k + 8))));
 }
 
 template <typename XT, int B>
@@ -1014,6 +1185,8 @@ class LzEngine final : public LzEngineBase {
         slots = lz_slots(D, o);
         cap = slots - 1;
         fullre = bh_lz_fullre(D, o);
+        mult = !o.distinct;
+        ymax = lz_ymax(D, o);
         {
             const char* e = std::getenv("BHX_GRAPH");
             use_graphs = !(e && std::strcmp(e, "0") == 0);
@@ -1023,7 +1196,7 @@ class LzEngine final : public LzEngineBase {
         dedup = o.dedup_rel > 0 ? o.dedup_rel : 1e-9;
         eps_store = MIXED ? 5.96e-8 : 1.11e-16;
         eps23 = std::pow(2.220446049250313e-16, 2.0 / 3.0);
-        const size_t nV = static_cast<size_t>(slots) * vs, nY = static_cast<size_t>(LZ_LMAX) * vs;
+        const size_t nV = static_cast<size_t>(slots) * vs, nY = static_cast<size_t>(ymax) * vs;
         if (ws) {
             void* q = nullptr;
             BH_TRY(bh_ws_get(ctx, 0, nV * sizeof(XT), &q));
@@ -1055,7 +1228,7 @@ class LzEngine final : public LzEngineBase {
         const size_t nd = n_alpha + 2 * n_beta + 2 * B + n_sums + n_tau + n_lth + B + 3 * n_lth;
         BH_TRY(dalloc(ctx, st_d, nd));
         BH_TRY(dalloc(ctx, st_i, static_cast<size_t>(3 * B + 2)));
-        const int nvmax = std::max(static_cast<int>(n_sums), 2 * LZ_NVMAX * B);
+        const int nvmax = std::max({static_cast<int>(n_sums), 2 * LZ_NVMAX * B, ymax * B});
         // K1 on the row-lane engine: small row tiles (LZ_K1_TILES row groups of LZ_WG / B rows), many
         // workgroups in flight; its partial sums need a row per workgroup
         rpw = std::max<int64_t>(static_cast<int64_t>(LZ_WG / B) * LZ_K1_TILES, (D + LZ_K1_GRID_MAX - 1) / LZ_K1_GRID_MAX);
@@ -1066,12 +1239,20 @@ class LzEngine final : public LzEngineBase {
         if (staged_ok) BH_TRY(bh_ham_rowblocks(h, B, &rblk, &nrblk));
         const int64_t rb_cap = std::max<int64_t>(grid_rl, LZ_PART_BYTES / (static_cast<int64_t>(nvmax) * 8));
         grid_rb = static_cast<int>(std::min<int64_t>({std::max<int64_t>(nrblk, 1), LZ_K1_GRID_MAX, rb_cap}));
-        BH_TRY(dalloc(ctx, part, static_cast<size_t>(std::max({LZ_GRID_MAX, LZ_K2_GRID, grid_rl, grid_rb})) * nvmax));
-        BH_TRY(dalloc(ctx, dnorm, static_cast<size_t>(2 * LZ_NVMAX * B)));
+        // partial-sum rows: K1 and R2 use up to max(grid) rows of nvmax values; the launches over grid_k2
+        // workgroups of the finish and the deflation write up to LZ_GRAM_VALS * B values per row (Gram tiles)
+        {
+            const int64_t ntb = (D + LZ_WG / B - 1) / (LZ_WG / B);
+            const size_t g2 = static_cast<size_t>(std::min<int64_t>(ntb, LZ_K2_GRID));
+            const size_t n1 = static_cast<size_t>(std::max({LZ_GRID_MAX, LZ_K2_GRID, grid_rl, grid_rb})) * nvmax;
+            const size_t n2 = g2 * static_cast<size_t>(std::max(LZ_GRAM_VALS, ymax)) * B;
+            BH_TRY(dalloc(ctx, part, std::max(n1, n2)));
+        }
+        BH_TRY(dalloc(ctx, dnorm, static_cast<size_t>(2 * std::max(LZ_CMAX, ymax) * B)));
         BH_TRY(dalloc(ctx, dnj, B));
-        BH_TRY(dalloc(ctx, lock_map, static_cast<size_t>(LZ_NVMAX) * B));
+        BH_TRY(dalloc(ctx, lock_map, static_cast<size_t>(std::max(LZ_CMAX, ymax)) * B));
         BH_TRY(dalloc(ctx, dno, B));
-        if (fullre) BH_TRY(dalloc(ctx, dd, static_cast<size_t>(slots) * B));
+        if (fullre) BH_TRY(dalloc(ctx, dd, static_cast<size_t>(slots + ymax) * B));
         st.alpha = st_d.p;
         st.beta = st.alpha + n_alpha;
         st.scale = st.beta + n_beta;
@@ -1152,6 +1333,28 @@ class LzEngine final : public LzEngineBase {
     }
 
     int lanes() const override { return B; }
+    int export_tridiag(int steps, double* alpha, double* beta, double* basis) override {
+        if (steps < 1 || steps > cap) return bh_set_error(ctx, BH_EINVAL, "tridiag: steps beyond the Krylov slots");
+        std::vector<double> ha2(static_cast<size_t>(steps) * B), hb2(static_cast<size_t>(steps + 1) * B),
+            hs2(static_cast<size_t>(steps) * B);
+        BH_HIP(ctx, hipMemcpyAsync(ha2.data(), st.alpha, ha2.size() * 8, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipMemcpyAsync(hb2.data(), st.beta, hb2.size() * 8, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipMemcpyAsync(hs2.data(), st.scale, hs2.size() * 8, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipStreamSynchronize(s));
+        for (int t = 0; t < steps; ++t) alpha[t] = ha2[static_cast<size_t>(t) * B];
+        beta[0] = 0.0;
+        for (int t = 1; t <= steps; ++t) beta[t] = hb2[static_cast<size_t>(t) * B];
+        if (basis) {
+            std::vector<XT> col(static_cast<size_t>(vs));
+            for (int t = 0; t < steps; ++t) {
+                BH_HIP(ctx, hipMemcpyAsync(col.data(), V + static_cast<int64_t>(t) * vs, vs * sizeof(XT), hipMemcpyDeviceToHost, s));
+                BH_HIP(ctx, hipStreamSynchronize(s));
+                const double sc = hs2[static_cast<size_t>(t) * B];
+                for (int64_t r = 0; r < D; ++r) basis[static_cast<size_t>(t) * D + r] = static_cast<double>(col[r * B]) * sc;
+            }
+        }
+        return BH_OK;
+    }
     int active() const override {
         int c = 0;
         for (int b = 0; b < B; ++b) c += act[b];
@@ -1207,7 +1410,11 @@ class LzEngine final : public LzEngineBase {
         return BH_OK;
     }
 
+    // Work may still be queued on the engine's stream (the speculative steps a check enqueues before it
+    // retires lanes): the stream is drained before the graph executables and the buffers those launches
+    // use are released.
     ~LzEngine() override {
+        (void)hipStreamSynchronize(s);
         for (auto& g : graphs)
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
     }
@@ -1271,7 +1478,7 @@ class LzEngine final : public LzEngineBase {
         prof.checks += 1;
         std::vector<std::vector<int>> new_locks(B), lock_slot(B);  // Ritz index, Y slot (-1: append)
         std::vector<Analysis> ans(B);
-        std::vector<int> done_now(B, 0), restart_now(B, 0);
+        std::vector<int> done_now(B, 0), restart_now(B, 0), deflate_now(B, 0);
         // analysis of lane b at step j (total steps tj); true when it retires or restarts the lane
         auto analyse_at = [&](int b, int j, int tj) -> bool {
             RunHost& R = rh[b];
@@ -1349,7 +1556,7 @@ class LzEngine final : public LzEngineBase {
             // extra candidates finished in FP64 beside the k wanted levels
             std::vector<int> cand = lev;
             for (int i = lev.empty() ? 0 : lev.back() + 1;
-                 i < static_cast<int>(an.theta.size()) && static_cast<int>(cand.size()) < std::min(k + 3, LZ_NVMAX); ++i) {
+                 i < static_cast<int>(an.theta.size()) && static_cast<int>(cand.size()) < std::min(k + 3, LZ_CMAX - R.nfz); ++i) {
                 if (std::fabs(an.theta[i] - an.theta[cand.back()]) <= tdedup * an.tnorm) continue;
                 cand.push_back(i);
             }
@@ -1357,13 +1564,67 @@ class LzEngine final : public LzEngineBase {
             std::vector<double> cthr(lev.size());
             for (size_t q = 0; q < lev.size(); ++q) {
                 cthr[q] = std::fmax(tol * std::fmax(eps23, std::fabs(an.theta[lev[q]])), floor);
-                if (lbound[q] >= cthr[q] && !exhausted) conv = false;
+                if (!mult && lbound[q] >= cthr[q] && !exhausted) conv = false;
+            }
+            // Eigenvalues with their multiplicity.  One pass (one start vector) meets each eigenspace once, so
+            // a pass that has converged freezes the levels it found and the lane starts again from a fresh
+            // vector kept orthogonal to every frozen vector: that pass meets the further copies of degenerate
+            // levels.  The lowest k values with multiplicity are the lowest k of the frozen values and the
+            // current pass's levels; kappa is the k-th of them.  A pass is complete when its levels at or below
+            // kappa have converged and its lowest level has (in a later pass that level tells whether the
+            // complement of the frozen vectors holds anything at or below kappa; when it lies clearly above
+            // kappa it needs only to be told apart from kappa).  A pass that adds nothing at or below kappa
+            // ends the solve.
+            double kappa = INFINITY;
+            const double kmarg = tdedup * an.tnorm;
+            int nmerged = static_cast<int>(lev.size());
+            if (mult) {
+                std::vector<double> mv(R.fz_theta);
+                for (int i : lev) mv.push_back(an.theta[i]);
+                std::sort(mv.begin(), mv.end());
+                nmerged = static_cast<int>(mv.size());
+                if (nmerged >= k) kappa = mv[k - 1];
+                // every Ritz value of this pass at or below kappa is among the analysed ones
+                const bool covered = static_cast<int>(lev.size()) == k ||
+                                     (!an.theta.empty() && an.theta.back() > kappa + kmarg) ||
+                                     static_cast<int>(an.theta.size()) >= jr;
+                conv = (nmerged >= k && covered) || exhausted;
+                if (lev.empty()) conv = conv && exhausted;
+                for (size_t q = 0; q < lev.size(); ++q) {
+                    const double th = an.theta[lev[q]];
+                    const bool below = th <= kappa + kmarg;
+                    double thr = cthr[q];
+                    if (!below && q == 0 && R.pass > 0) thr = std::fmax(thr, 1e-4 * (th - kappa));
+                    if ((below || q == 0) && lbound[q] >= thr && !exhausted) conv = false;
+                }
+            }
+            bool capped = false;
+            if (mult && conv && tj < maxsteps) {
+                std::vector<int> fz;  // this pass's levels among the lowest k
+                for (int i : lev)
+                    if (an.theta[i] <= kappa + kmarg) fz.push_back(i);
+                // frozen values now above kappa are no longer among the lowest k: their vectors are dropped (a
+                // later pass may meet them again, above kappa)
+                std::vector<int> keep;
+                for (int l = 0; l < R.nfz; ++l)
+                    if (R.fz_theta[l] <= kappa + kmarg) keep.push_back(l);
+                const int nnew = static_cast<int>(keep.size() + fz.size());
+                if (!fz.empty() && nnew < D && nnew <= ymax && R.pass + 1 < LZ_PASS_MAX) {
+                    plan_fz[b] = fz;
+                    plan_keep[b] = keep;
+                    deflate_now[b] = 1;
+                    return true;
+                }
+                // out of frozen slots or passes: the copies found so far are reported, the solve is not certified
+                capped = !fz.empty() && nnew < D;
             }
             if (conv || tj >= maxsteps) {
                 R.done = true;
                 R.exhausted = exhausted;
                 R.iters = tj;
-                R.status = conv && static_cast<int>(lev.size()) == k ? BH_OK : BH_ENOCONV;
+                R.status = conv && !capped && (mult ? (nmerged >= k || exhausted) : static_cast<int>(lev.size()) == k)
+                               ? BH_OK
+                               : BH_ENOCONV;
                 R.theta.clear();
                 R.s.clear();
                 for (int i : cand) {
@@ -1424,8 +1685,8 @@ class LzEngine final : public LzEngineBase {
                 // lock threshold, so that one level is never locked twice (near-parallel locked vectors
                 // would break the Gram-Schmidt step of the orthogonalisation)
                 bool have = false;
-                for (double th : R.lock_theta)
-                    if (std::fabs(th - an.theta[i]) <= lock_thr) have = true;
+                for (size_t l = R.nfz; l < R.lock_theta.size(); ++l)
+                    if (std::fabs(R.lock_theta[l] - an.theta[i]) <= lock_thr) have = true;
                 for (int q : new_locks[b])
                     if (std::fabs(an.theta[q] - an.theta[i]) <= lock_thr) have = true;
                 if (have) continue;
@@ -1437,7 +1698,7 @@ class LzEngine final : public LzEngineBase {
             // copy of the locked level from that remainder; a locked level whose Ritz bound has since dropped a
             // thousandfold gets its vector re-formed from the current Ritz vector.  FP32-stored runs lock far
             // earlier (sqrt(eps32) ||T||) and measured slower with refreshes, so they keep their first vector.
-            for (int l = 0; !MIXED && l < static_cast<int>(R.lock_theta.size()); ++l) {
+            for (int l = R.nfz; !MIXED && l < static_cast<int>(R.lock_theta.size()); ++l) {
                 int best = -1;
                 for (int i = 0; i < static_cast<int>(an.theta.size()); ++i)
                     if (std::fabs(an.theta[i] - R.lock_theta[l]) <= lock_thr && (best < 0 || an.bound[i] < an.bound[best]))
@@ -1487,7 +1748,7 @@ class LzEngine final : public LzEngineBase {
                 for (auto& th : pool) th.join();
             }
         }
-        bool any_lock = false, any_done = false, any_restart = false;
+        bool any_lock = false, any_done = false, any_restart = false, any_deflate = false;
         for (int b = 0; b < B; ++b) {
             if (done_now[b]) {
                 act[b] = 0;
@@ -1496,12 +1757,13 @@ class LzEngine final : public LzEngineBase {
             }
             if (!new_locks[b].empty()) any_lock = true;
             if (restart_now[b]) any_restart = true;
+            if (deflate_now[b]) any_deflate = true;
         }
         // roll lanes back to the step of their decision
         bool rolled = false;
         std::vector<int> src_slot(B, -1);  // current vector of lane b restored from this slot
         for (int b = 0; b < B; ++b) {
-            const bool decided = done_now[b] || !new_locks[b].empty() || restart_now[b];
+            const bool decided = done_now[b] || !new_locks[b].empty() || restart_now[b] || deflate_now[b];
             if (!decided || jan[b] >= jl[b]) continue;
             jl[b] = jan[b];
             tot[b] = tot_an[b];
@@ -1519,6 +1781,7 @@ class LzEngine final : public LzEngineBase {
         }
         if (any_lock) BH_TRY(lock(new_locks, lock_slot, ans));
         if (any_restart) BH_TRY(restart(restart_now));
+        if (any_deflate) BH_TRY(deflate(deflate_now, ans));
         if (any_done) {
             BH_HIP(ctx, hipMemcpyAsync(st.active, act.data(), B * 4, hipMemcpyHostToDevice, s));
             BH_HIP(ctx, hipStreamSynchronize(s));
@@ -1533,9 +1796,17 @@ class LzEngine final : public LzEngineBase {
         std::vector<double> hsc(static_cast<size_t>(jmax + 1) * B);
         BH_HIP(ctx, hipMemcpyAsync(hsc.data(), st.scale, hsc.size() * 8, hipMemcpyDeviceToHost, s));
         BH_HIP(ctx, hipStreamSynchronize(s));
-        int nvf = 0;
-        for (int b : lanes) nvf = std::max(nvf, static_cast<int>(rh[b].theta.size()));
-        nvf = std::min(nvf, LZ_NVMAX);
+        // candidates of lane b: its frozen vectors (Y slots 0 .. nf[b] - 1, earlier passes) first, then the
+        // Ritz vectors of its last pass
+        std::vector<int> nf(B, 0), nc(B, 0), nt(B, 0);
+        int nvf = 0, nfmax = 0;
+        for (int b : lanes) {
+            nf[b] = mult ? std::min(rh[b].nfz, LZ_CMAX) : 0;
+            nc[b] = std::min(static_cast<int>(rh[b].theta.size()), LZ_CMAX - nf[b]);
+            nt[b] = nf[b] + nc[b];
+            nvf = std::max(nvf, nt[b]);
+            nfmax = std::max(nfmax, nf[b]);
+        }
         std::vector<double> theta_f(static_cast<size_t>(std::max(nvf, 1)) * B, NAN), resid_f(theta_f.size(), NAN);
         std::vector<double> dots;
         std::vector<double> hy;
@@ -1543,20 +1814,26 @@ class LzEngine final : public LzEngineBase {
             int njmax = 0;
             std::vector<int> njb(B, 0);
             for (int b : lanes)
-                if (!rh[b].s.empty()) {
+                if (nc[b] > 0) {
                     njb[b] = static_cast<int>(rh[b].s[0].size());
                     njmax = std::max(njmax, njb[b]);
                 }
             std::vector<double> C(static_cast<size_t>(std::max(njmax, 1)) * nvf * B, 0.0);
             for (int b : lanes)
-                for (int i = 0; i < static_cast<int>(rh[b].s.size()) && i < nvf; ++i)
+                for (int i = 0; i < nc[b]; ++i)
                     for (int t = 0; t < static_cast<int>(rh[b].s[i].size()); ++t)
-                        C[(static_cast<size_t>(t) * nvf + i) * B + b] = rh[b].s[i][t] * hsc[static_cast<size_t>(t) * B + b];
+                        C[(static_cast<size_t>(t) * nvf + nf[b] + i) * B + b] = rh[b].s[i][t] * hsc[static_cast<size_t>(t) * B + b];
             DevArray<double> Yf, HY;
             BH_TRY(Yf.alloc(ctx, static_cast<size_t>(nvf) * vs));
             BH_TRY(HY.alloc(ctx, static_cast<size_t>(nvf) * vs));
             BH_TRY(form_vectors(njb, nvf, C, Yf.p, vs));
-            // exact FP64 operator on each Ritz vector (packed words, FP64 table, FP64 vectors)
+            if (nfmax > 0) {
+                BH_HIP(ctx, hipMemcpyAsync(dnj.p, nf.data(), B * sizeof(int), hipMemcpyHostToDevice, s));
+                hipLaunchKernelGGL((k_lz_copy_frozen<XT, B>), dim3(static_cast<unsigned>((vs + 255) / 256)), dim3(256), 0, s, Y,
+                                   Yf.p, vs, dnj.p, nfmax);
+                BH_HIP(ctx, hipStreamSynchronize(s));
+            }
+            // exact FP64 operator on each candidate (packed words, FP64 table, FP64 vectors)
             DevArray<double> dJ, dU;
             BH_TRY(dJ.alloc(ctx, B));
             BH_TRY(dU.alloc(ctx, B));
@@ -1569,13 +1846,19 @@ class LzEngine final : public LzEngineBase {
                 BH_TRY(bh_apply(ctx, h, &sa, nullptr, BH_FMT_PACKED, B, s));
             }
             BH_HIP(ctx, hipGetLastError());
-            BH_TRY(rayleigh_ritz(lanes, nvf, Yf.p, HY.p));
+            BH_TRY(rayleigh_ritz(lanes, nvf, nt, Yf.p, HY.p));
+            // Rayleigh quotients and residuals, LZ_NVMAX vectors per launch
             dots.assign(static_cast<size_t>(2 * nvf) * B, 0.0);
-            hipLaunchKernelGGL((k_lz_dots<double, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yf.p, HY.p, D, nvf, 0, nullptr,
-                               part.p);
-            reduce_plain(grid_k2, 2 * nvf * B, dnorm.p);
-            BH_HIP(ctx, hipMemcpyAsync(dots.data(), dnorm.p, dots.size() * 8, hipMemcpyDeviceToHost, s));
-            BH_HIP(ctx, hipStreamSynchronize(s));
+            std::vector<double> rr(static_cast<size_t>(nvf) * B);
+            for (int i0 = 0; i0 < nvf; i0 += LZ_NVMAX) {
+                const int nq = std::min(LZ_NVMAX, nvf - i0);
+                hipLaunchKernelGGL((k_lz_dots<double, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yf.p + static_cast<int64_t>(i0) * vs,
+                                   HY.p + static_cast<int64_t>(i0) * vs, D, nq, 0, nullptr, part.p);
+                reduce_plain(grid_k2, 2 * nq * B, dnorm.p);
+                BH_HIP(ctx, hipMemcpyAsync(dots.data() + static_cast<size_t>(2 * i0) * B, dnorm.p, static_cast<size_t>(2 * nq) * B * 8,
+                                           hipMemcpyDeviceToHost, s));
+                BH_HIP(ctx, hipStreamSynchronize(s));
+            }
             for (int i = 0; i < nvf; ++i)
                 for (int b = 0; b < B; ++b) {
                     const double yy = dots[(2 * i) * B + b], yhy = dots[(2 * i + 1) * B + b];
@@ -1587,11 +1870,15 @@ class LzEngine final : public LzEngineBase {
             for (auto& v : th0)
                 if (!std::isfinite(v)) v = 0.0;
             BH_HIP(ctx, hipMemcpyAsync(dth.p, th0.data(), th0.size() * 8, hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL((k_lz_dots<double, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yf.p, HY.p, D, nvf, 1, dth.p, part.p);
-            reduce_plain(grid_k2, nvf * B, dnorm.p);
-            std::vector<double> rr(static_cast<size_t>(nvf) * B);
-            BH_HIP(ctx, hipMemcpyAsync(rr.data(), dnorm.p, rr.size() * 8, hipMemcpyDeviceToHost, s));
-            BH_HIP(ctx, hipStreamSynchronize(s));
+            for (int i0 = 0; i0 < nvf; i0 += LZ_NVMAX) {
+                const int nq = std::min(LZ_NVMAX, nvf - i0);
+                hipLaunchKernelGGL((k_lz_dots<double, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yf.p + static_cast<int64_t>(i0) * vs,
+                                   HY.p + static_cast<int64_t>(i0) * vs, D, nq, 1, dth.p + static_cast<size_t>(i0) * B, part.p);
+                reduce_plain(grid_k2, nq * B, dnorm.p);
+                BH_HIP(ctx, hipMemcpyAsync(rr.data() + static_cast<size_t>(i0) * B, dnorm.p, static_cast<size_t>(nq) * B * 8,
+                                           hipMemcpyDeviceToHost, s));
+                BH_HIP(ctx, hipStreamSynchronize(s));
+            }
             for (int i = 0; i < nvf; ++i)
                 for (int b = 0; b < B; ++b) {
                     const double yy = dots[(2 * i) * B + b];
@@ -1612,48 +1899,76 @@ class LzEngine final : public LzEngineBase {
             L.evals.assign(k, NAN);
             L.resid.assign(k, NAN);
             L.errest.assign(k, NAN);
-            const int nl = std::min(static_cast<int>(R.theta.size()), nvf);
-            // Levels after the FP64 finish, ascending.  Rayleigh quotients of one level agree to
-            // ~||r||^2/gap and are merged.  Each finished pair certifies an eigenvalue within its residual
-            // of its Rayleigh quotient (Weinstein); a pair whose interval holds the value of a pair with a
-            // smaller residual certifies nothing new (it can be an unconverged copy of that level, which
-            // finite-precision Lanczos produces) and is dropped.  A mixed-precision solve that had to drop
-            // one among its wanted levels is reported unconverged, so that it is rerun in FP64.
+            const int nl = nt[b];
             std::vector<std::pair<double, int>> lv0, lv;
             for (int i = 0; i < nl; ++i) lv0.push_back({theta_f[i * B + b], i});
             std::sort(lv0.begin(), lv0.end());
             const double mtol = dedup * std::fmax(R.tnorm, 1e-300);
             auto rs_of = [&](const std::pair<double, int>& p) { return resid_f[p.second * B + b]; };
             bool dropped = false;
-            for (const auto& qq : lv0) {
-                if (!std::isfinite(qq.first)) continue;
-                if (!lv.empty() && std::fabs(qq.first - lv.back().first) <= mtol) {
-                    if (rs_of(qq) < rs_of(lv.back())) lv.back() = qq;
-                    continue;
+            if (mult) {
+                // With multiplicity.  The Rayleigh-Ritz vectors are orthonormal, so m pairs whose residuals are
+                // small certify m eigenvalues, copies of one level included (each within its residual).  A pair
+                // whose residual is large is kept as an uncertified value, unless its interval holds the value
+                // of a certified pair: then it is a direction left over from two approximations of one
+                // eigenvector and certifies nothing.
+                const double rfloor = MIXED ? 100.0 * eps_store * std::fmax(R.tnorm, 1e-300) : 0.0;
+                auto certified = [&](const std::pair<double, int>& p) {
+                    return rs_of(p) <= std::fmax(1e3 * tol * std::fmax(eps23, std::fabs(p.first)), rfloor);
+                };
+                std::vector<double> good;
+                for (const auto& qq : lv0)
+                    if (std::isfinite(qq.first) && certified(qq)) good.push_back(qq.first);
+                for (const auto& qq : lv0) {
+                    if (!std::isfinite(qq.first)) continue;
+                    if (!certified(qq)) {
+                        bool inside = false;
+                        for (double g : good) inside = inside || std::fabs(g - qq.first) <= rs_of(qq);
+                        if (inside) {
+                            if (static_cast<int>(lv.size()) < k) dropped = true;
+                            continue;
+                        }
+                    }
+                    lv.push_back(qq);
                 }
-                if (!lv.empty()) {
-                    const double sep = std::fabs(qq.first - lv.back().first);
-                    const double rq = rs_of(qq), rb = rs_of(lv.back());
-                    if (rq >= sep && rq >= rb) {
-                        if (static_cast<int>(lv.size()) < k) dropped = true;
+            } else {
+                // Distinct levels.  Rayleigh quotients of one level agree to ~||r||^2/gap and are merged.  Each
+                // finished pair certifies an eigenvalue within its residual of its Rayleigh quotient (Weinstein);
+                // a pair whose interval holds the value of a pair with a smaller residual certifies nothing new
+                // (it can be an unconverged copy of that level, which finite-precision Lanczos produces) and is
+                // dropped.
+                for (const auto& qq : lv0) {
+                    if (!std::isfinite(qq.first)) continue;
+                    if (!lv.empty() && std::fabs(qq.first - lv.back().first) <= mtol) {
+                        if (rs_of(qq) < rs_of(lv.back())) lv.back() = qq;
                         continue;
                     }
-                    if (rb >= sep && rb > rq) {
-                        if (static_cast<int>(lv.size()) <= k) dropped = true;
-                        lv.back() = qq;
-                        continue;
+                    if (!lv.empty()) {
+                        const double sep = std::fabs(qq.first - lv.back().first);
+                        const double rq = rs_of(qq), rb = rs_of(lv.back());
+                        if (rq >= sep && rq >= rb) {
+                            if (static_cast<int>(lv.size()) < k) dropped = true;
+                            continue;
+                        }
+                        if (rb >= sep && rb > rq) {
+                            if (static_cast<int>(lv.size()) <= k) dropped = true;
+                            lv.back() = qq;
+                            continue;
+                        }
                     }
+                    lv.push_back(qq);
                 }
-                lv.push_back(qq);
             }
+            // A mixed-precision solve that had to drop one among its wanted levels is reported unconverged, so
+            // that it is rerun in FP64
             if (MIXED && dropped && L.status == BH_OK) L.status = BH_ENOCONV;
-            // fewer than k distinct levels is an answer only when the Krylov space was exhausted
+            // fewer than k levels is an answer only when the Krylov space was exhausted
             if (L.status == BH_OK && static_cast<int>(lv.size()) < k && !R.exhausted) L.status = BH_ENOCONV;
             if (want_vectors) L.vecs.assign(static_cast<size_t>(k) * D, 0.0);
             for (int i = 0; i < k && i < static_cast<int>(lv.size()); ++i) {
                 const double ev = lv[i].first;
                 const double rs = rs_of(lv[i]);
-                double gap = INFINITY;  // to the nearest other finished value of the lane
+                double gap = INFINITY;  // to the nearest other finished value of the lane outside ev's cluster
                 for (const auto& qq : lv0)
                     if (std::isfinite(qq.first) && std::fabs(qq.first - ev) > mtol) gap = std::fmin(gap, std::fabs(qq.first - ev));
                 L.evals[i] = ev;
@@ -1711,34 +2026,59 @@ class LzEngine final : public LzEngineBase {
     // directions of G (copies of one level that finite-precision Lanczos produces), and Y, HY are replaced
     // in place by the Ritz vectors of the span and their images.  A candidate that mixes two eigenvectors
     // is separated when the other one is in the span.
-    int rayleigh_ritz(const std::vector<int>& lanes, int nvf, double* Yp, double* HYp) {
+    int rayleigh_ritz(const std::vector<int>& lanes, int nvf, const std::vector<int>& nt, double* Yp, double* HYp) {
         DevArray<double> dg, dz;
-        // rows i0 .. i0 + NI - 1 of [G | A] per launch, NV columns each (NV >= nvf)
-        const bool small = nvf <= 8;
-        const int NI = small ? 4 : 2, NV = small ? 8 : LZ_NVMAX;
-        const int nblk = (nvf + NI - 1) / NI;
-        const size_t per_blk = static_cast<size_t>(NI) * 2 * NV * B;
-        BH_TRY(dg.alloc(ctx, nblk * per_blk));
-        for (int q = 0; q < nblk; ++q) {
-            if (small)
-                hipLaunchKernelGGL((k_lz_gram_rows<B, 8, 4>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yp, HYp, D, nvf, q * NI, part.p);
-            else
-                hipLaunchKernelGGL((k_lz_gram_rows<B, LZ_NVMAX, 2>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yp, HYp, D, nvf, q * NI,
-                                   part.p);
-            reduce_plain(grid_k2, static_cast<int>(per_blk), dg.p + q * per_blk);
+        std::vector<double> g(static_cast<size_t>(nvf) * 2 * nvf * B);
+        if (nvf <= LZ_NVMAX) {
+            // rows i0 .. i0 + NI - 1 of [G | A] per launch, NV columns each (NV >= nvf)
+            const bool small = nvf <= 8;
+            const int NI = small ? 4 : 2, NV = small ? 8 : LZ_NVMAX;
+            const int nblk = (nvf + NI - 1) / NI;
+            const size_t per_blk = static_cast<size_t>(NI) * 2 * NV * B;
+            BH_TRY(dg.alloc(ctx, nblk * per_blk));
+            for (int q = 0; q < nblk; ++q) {
+                if (small)
+                    hipLaunchKernelGGL((k_lz_gram_rows<B, 8, 4>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yp, HYp, D, nvf, q * NI, part.p);
+                else
+                    hipLaunchKernelGGL((k_lz_gram_rows<B, LZ_NVMAX, 2>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yp, HYp, D, nvf, q * NI,
+                                       part.p);
+                reduce_plain(grid_k2, static_cast<int>(per_blk), dg.p + q * per_blk);
+            }
+            std::vector<double> gg(nblk * per_blk);
+            BH_HIP(ctx, hipMemcpyAsync(gg.data(), dg.p, gg.size() * 8, hipMemcpyDeviceToHost, s));
+            BH_HIP(ctx, hipStreamSynchronize(s));
+            for (int i = 0; i < nvf; ++i)
+                for (int w = 0; w < 2; ++w)
+                    for (int q = 0; q < nvf; ++q)
+                        for (int b = 0; b < B; ++b)
+                            g[(static_cast<size_t>(i) * 2 * nvf + w * nvf + q) * B + b] =
+                                gg[(i / NI) * per_blk + ((static_cast<size_t>(i % NI) * 2 + w) * NV + q) * B + b];
+        } else {
+            // blocks of 2 rows x LZ_NVMAX columns
+            constexpr int NI = 2, NV = LZ_NVMAX;
+            const int nri = (nvf + NI - 1) / NI, nqi = (nvf + NV - 1) / NV;
+            const size_t per_blk = static_cast<size_t>(NI) * 2 * NV * B;
+            BH_TRY(dg.alloc(ctx, static_cast<size_t>(nri) * nqi * per_blk));
+            for (int ri = 0; ri < nri; ++ri)
+                for (int qi = 0; qi < nqi; ++qi) {
+                    hipLaunchKernelGGL((k_lz_gram_blk<B, NV, NI>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yp, HYp, D, nvf, ri * NI,
+                                       qi * NV, part.p);
+                    reduce_plain(grid_k2, static_cast<int>(per_blk), dg.p + (static_cast<size_t>(ri) * nqi + qi) * per_blk);
+                }
+            std::vector<double> gg(static_cast<size_t>(nri) * nqi * per_blk);
+            BH_HIP(ctx, hipMemcpyAsync(gg.data(), dg.p, gg.size() * 8, hipMemcpyDeviceToHost, s));
+            BH_HIP(ctx, hipStreamSynchronize(s));
+            for (int i = 0; i < nvf; ++i)
+                for (int w = 0; w < 2; ++w)
+                    for (int q = 0; q < nvf; ++q)
+                        for (int b = 0; b < B; ++b)
+                            g[(static_cast<size_t>(i) * 2 * nvf + w * nvf + q) * B + b] =
+                                gg[((static_cast<size_t>(i / NI) * nqi + q / NV) * per_blk) +
+                                   ((static_cast<size_t>(i % NI) * 2 + w) * NV + q % NV) * B + b];
         }
-        std::vector<double> gg(nblk * per_blk), g(static_cast<size_t>(nvf) * 2 * nvf * B);
-        BH_HIP(ctx, hipMemcpyAsync(gg.data(), dg.p, gg.size() * 8, hipMemcpyDeviceToHost, s));
-        BH_HIP(ctx, hipStreamSynchronize(s));
-        for (int i = 0; i < nvf; ++i)
-            for (int w = 0; w < 2; ++w)
-                for (int q = 0; q < nvf; ++q)
-                    for (int b = 0; b < B; ++b)
-                        g[(static_cast<size_t>(i) * 2 * nvf + w * nvf + q) * B + b] =
-                            gg[(i / NI) * per_blk + ((static_cast<size_t>(i % NI) * 2 + w) * NV + q) * B + b];
         std::vector<double> Z(static_cast<size_t>(nvf) * nvf * B, 0.0);
         for (int b : lanes) {
-            const int n = std::min(static_cast<int>(rh[b].s.size()), nvf);
+            const int n = std::min(nt[b], nvf);
             if (n == 0) continue;
             auto G = [&](int i, int q) { return g[(static_cast<size_t>(i) * 2 * nvf + q) * B + b]; };
             auto A = [&](int i, int q) { return g[(static_cast<size_t>(i) * 2 * nvf + nvf + q) * B + b]; };
@@ -1774,8 +2114,18 @@ class LzEngine final : public LzEngineBase {
         BH_TRY(dz.alloc(ctx, Z.size()));
         BH_HIP(ctx, hipMemcpyAsync(dz.p, Z.data(), Z.size() * 8, hipMemcpyHostToDevice, s));
         const unsigned grid = static_cast<unsigned>((vs + 255) / 256);
-        hipLaunchKernelGGL((k_lz_combine<B>), dim3(grid), dim3(256), 0, s, Yp, vs, dz.p, nvf, nvf);
-        hipLaunchKernelGGL((k_lz_combine<B>), dim3(grid), dim3(256), 0, s, HYp, vs, dz.p, nvf, nvf);
+        if (nvf <= LZ_NVMAX) {
+            hipLaunchKernelGGL((k_lz_combine<B>), dim3(grid), dim3(256), 0, s, Yp, vs, dz.p, nvf, nvf);
+            hipLaunchKernelGGL((k_lz_combine<B>), dim3(grid), dim3(256), 0, s, HYp, vs, dz.p, nvf, nvf);
+        } else {
+            DevArray<double> tmp;
+            BH_TRY(tmp.alloc(ctx, static_cast<size_t>(nvf) * vs));
+            for (double* X : {Yp, HYp}) {
+                hipLaunchKernelGGL((k_lz_combine_wide<B>), dim3(grid), dim3(256), 0, s, X, tmp.p, vs, dz.p, nvf, nvf);
+                BH_HIP(ctx, hipMemcpyAsync(X, tmp.p, static_cast<size_t>(nvf) * vs * 8, hipMemcpyDeviceToDevice, s));
+            }
+            BH_HIP(ctx, hipStreamSynchronize(s));  // tmp is freed on return
+        }
         BH_HIP(ctx, hipGetLastError());
         BH_HIP(ctx, hipStreamSynchronize(s));  // Z is freed on return
         return BH_OK;
@@ -1943,7 +2293,8 @@ class LzEngine final : public LzEngineBase {
             // two classical Gram-Schmidt passes against every stored Lanczos vector of the lane
             KTimer kt(ctx, "lanczos_reorth", s);
             for (int pass = 0; pass < 2; ++pass) {
-                hipLaunchKernelGGL((k_lz_fr_dots<XT, B>), dim3(static_cast<unsigned>(cap)), dim3(LZ_WG), 0, s, a, dd.p);
+                hipLaunchKernelGGL((k_lz_fr_dots<XT, B>), dim3(static_cast<unsigned>(cap + (mult ? ymax : 0))), dim3(LZ_WG), 0,
+                                   s, a, dd.p);
                 hipLaunchKernelGGL((k_lz_fr_axpy<XT, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, a, dd.p, pass);
             }
         }
@@ -2129,6 +2480,178 @@ class LzEngine final : public LzEngineBase {
         return BH_OK;
     }
 
+    // Z (FP64 [D][B]) -= its components along Y slots 0 .. cnt[b] - 1 of each lane (two classical Gram-Schmidt
+    // passes, FP64 dot products); lanes with cnt[b] < 0 untouched.  cnt is a host array.
+    int ortho_frozen(double* Z, const std::vector<int>& cnt) {
+        int lmax = 0;
+        for (int b = 0; b < B; ++b) lmax = std::max(lmax, cnt[b]);
+        if (lmax <= 0) return BH_OK;
+        BH_HIP(ctx, hipMemcpyAsync(dnj.p, cnt.data(), B * sizeof(int), hipMemcpyHostToDevice, s));
+        for (int pass = 0; pass < 2; ++pass) {
+            hipLaunchKernelGGL((k_lz_ydots<XT, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, Y, Z, D, dnj.p, lmax, part.p);
+            reduce_plain(grid_k2, lmax * B, dnorm.p);
+            hipLaunchKernelGGL((k_lz_yaxpy<XT, B>), dim3(static_cast<unsigned>((vs + 255) / 256)), dim3(256), 0, s, Y, Z, vs,
+                               dnj.p, dnorm.p);
+        }
+        BH_HIP(ctx, hipGetLastError());
+        return BH_OK;
+    }
+    // 1 / ||Z[.][b]|| per lane (FP64 [D][B] or storage-precision vector), host result
+    template <typename ZT>
+    int inv_norms(const ZT* Z, std::vector<double>& f) {
+        hipLaunchKernelGGL((k_lz_dots<ZT, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, Z, nullptr, D, 1, 2, nullptr, part.p);
+        reduce_plain(grid_k2, B, dnorm.p);
+        f.assign(B, 0.0);
+        BH_HIP(ctx, hipMemcpyAsync(f.data(), dnorm.p, B * 8, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipStreamSynchronize(s));
+        for (auto& v : f) v = v > 0 ? 1.0 / std::sqrt(v) : 0.0;
+        return BH_OK;
+    }
+    // Deflation restart of the flagged lanes (plans made by the analysis, lanes rolled back to the analysed
+    // step): the Ritz vectors of the levels in plan_fz[b] are formed in FP64, orthonormalised against the
+    // lane's frozen vectors and among themselves, and stored in Y after them; the lane then starts its next
+    // pass at slot 0 from the Park-Miller sequence continued past the values of its earlier start vectors,
+    // orthogonalised against every frozen vector.  Under selective orthogonalisation the frozen vectors are
+    // the lane's locked set at the start of the pass (the pass's own locks are appended after them); under
+    // full reorthogonalisation every step projects them out (k_lz_fr_dots / k_lz_fr_axpy).
+    int deflate(const std::vector<int>& flag, const std::vector<Analysis>& ans) {
+        int nv = 0, jmax = 0;
+        for (int b = 0; b < B; ++b)
+            if (flag[b]) {
+                nv = std::max(nv, static_cast<int>(plan_fz[b].size()));
+                jmax = std::max(jmax, static_cast<int>(ans[b].s[plan_fz[b][0]].size()));
+            }
+        std::vector<double> hsc(static_cast<size_t>(jmax) * B);
+        BH_HIP(ctx, hipMemcpyAsync(hsc.data(), st.scale, hsc.size() * 8, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipStreamSynchronize(s));
+        std::vector<double> C(static_cast<size_t>(jmax) * nv * B, 0.0);
+        std::vector<int> njb(B, 0);
+        for (int b = 0; b < B; ++b) {
+            if (!flag[b]) continue;
+            njb[b] = static_cast<int>(ans[b].s[plan_fz[b][0]].size());
+            for (int i = 0; i < static_cast<int>(plan_fz[b].size()); ++i) {
+                const std::vector<double>& sv = ans[b].s[plan_fz[b][i]];
+                for (int t = 0; t < njb[b]; ++t) C[(static_cast<size_t>(t) * nv + i) * B + b] = sv[t] * hsc[static_cast<size_t>(t) * B + b];
+            }
+        }
+        DevArray<double> Zf;
+        BH_TRY(Zf.alloc(ctx, static_cast<size_t>(nv) * vs));
+        BH_TRY(form_vectors(njb, nv, C, Zf.p, vs));
+        const unsigned g256 = static_cast<unsigned>((vs + 255) / 256);
+        // drop the frozen vectors the plans do not keep: kept slots move down, in ascending order
+        {
+            std::vector<int> src(static_cast<size_t>(ymax) * B, -1);
+            int ns = 0;
+            for (int b = 0; b < B; ++b) {
+                if (!flag[b]) continue;
+                const std::vector<int>& kp = plan_keep[b];
+                std::vector<double> th;
+                for (size_t q = 0; q < kp.size(); ++q) {
+                    src[q * B + b] = kp[q];
+                    th.push_back(rh[b].fz_theta[kp[q]]);
+                }
+                ns = std::max(ns, static_cast<int>(kp.size()));
+                rh[b].fz_theta = th;
+                rh[b].nfz = static_cast<int>(kp.size());
+            }
+            if (ns > 0) {
+                BH_HIP(ctx, hipMemcpyAsync(lock_map.p, src.data(), static_cast<size_t>(ns) * B * sizeof(int), hipMemcpyHostToDevice, s));
+                hipLaunchKernelGGL((k_lz_compact<XT, B>), dim3(g256), dim3(256), 0, s, Y, vs, lock_map.p, ns);
+                BH_HIP(ctx, hipGetLastError());
+                BH_HIP(ctx, hipStreamSynchronize(s));
+            }
+        }
+        std::vector<int> cnt(B), slot(B);
+        std::vector<double> f;
+        // new frozen vector i of each lane: orthogonal to the lane's frozen vectors so far, then normalised
+        for (int i = 0; i < nv; ++i) {
+            for (int b = 0; b < B; ++b) {
+                const bool on = flag[b] && i < static_cast<int>(plan_fz[b].size());
+                cnt[b] = on ? rh[b].nfz + i : -1;
+                slot[b] = on ? rh[b].nfz + i : -1;
+            }
+            double* Zi = Zf.p + static_cast<int64_t>(i) * vs;
+            BH_TRY(ortho_frozen(Zi, cnt));
+            BH_TRY(inv_norms(static_cast<const double*>(Zi), f));
+            BH_HIP(ctx, hipMemcpyAsync(lock_map.p, slot.data(), B * sizeof(int), hipMemcpyHostToDevice, s));
+            BH_HIP(ctx, hipMemcpyAsync(dnorm.p, f.data(), B * 8, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL((k_lz_defl_store<XT, B>), dim3(g256), dim3(256), 0, s, Zi, Y, vs, lock_map.p, dnorm.p,
+                               static_cast<XT*>(nullptr));
+            BH_HIP(ctx, hipStreamSynchronize(s));  // the host arrays are reused by the next vector
+        }
+        // fresh start vectors in W (free between steps), orthogonal to every frozen vector
+        for (int b = 0; b < B; ++b) {
+            cnt[b] = flag[b] ? rh[b].nfz + static_cast<int>(plan_fz[b].size()) : -1;
+            slot[b] = flag[b] ? 0 : -1;
+            if (!flag[b]) continue;
+            const uint64_t p = 2147483647ull;
+            const uint64_t x0 = o.seed ? (o.seed & p) : 1;
+            hipLaunchKernelGGL((k_lz_pm_fill<B>), dim3(static_cast<unsigned>((D + 255) / 256)), dim3(256), 0, s, W, D, b, x0,
+                               static_cast<uint64_t>(rh[b].pass + 1) * static_cast<uint64_t>(D));
+        }
+        BH_TRY(ortho_frozen(W, cnt));
+        BH_TRY(inv_norms(static_cast<const double*>(W), f));
+        BH_HIP(ctx, hipMemcpyAsync(lock_map.p, slot.data(), B * sizeof(int), hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(dnorm.p, f.data(), B * 8, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL((k_lz_defl_store<XT, B>), dim3(g256), dim3(256), 0, s, W, V, vs, lock_map.p, dnorm.p, Xc);
+        BH_HIP(ctx, hipGetLastError());
+        // the inverse norm of the stored start vector (its rounding to the storage precision included)
+        std::vector<double> sc0;
+        BH_TRY(inv_norms(static_cast<const XT*>(V), sc0));
+        // lane state: the next pass starts at step 0 with the frozen set as its orthogonality constraint
+        std::vector<double> hs(B), hbeta(B), lth(static_cast<size_t>(LZ_LMAX) * B), tau(static_cast<size_t>(2 * LZ_LMAX) * B);
+        std::vector<int> hstep(B), hnl(B);
+        BH_HIP(ctx, hipMemcpyAsync(hs.data(), st.scale, B * 8, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipMemcpyAsync(hbeta.data(), st.beta, B * 8, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipMemcpyAsync(hstep.data(), st.step, B * 4, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipMemcpyAsync(hnl.data(), st.nlock, B * 4, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipMemcpyAsync(lth.data(), st.ltheta, lth.size() * 8, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipMemcpyAsync(tau.data(), st.tau, tau.size() * 8, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipStreamSynchronize(s));
+        for (int b = 0; b < B; ++b) {
+            if (!flag[b]) continue;
+            RunHost& R = rh[b];
+            for (int i : plan_fz[b]) R.fz_theta.push_back(ans[b].theta[i]);
+            R.nfz += static_cast<int>(plan_fz[b].size());
+            R.pass += 1;
+            R.alpha.clear();
+            R.beta.clear();
+            R.prev_tot = -1;
+            R.lock_theta = R.fz_theta;
+            R.lock_bound.assign(R.nfz, 0.0);
+            hs[b] = sc0[b];
+            hbeta[b] = 0.0;
+            hstep[b] = 0;
+            hnl[b] = R.nfz;
+            nlock[b] = R.nfz;
+            jl[b] = 0;
+            kept[b] = 0;
+            nextc[b] = std::max(2 * k + 8, 20);
+            if (!fullre) {
+                for (int l = 0; l < R.nfz && l < LZ_LMAX; ++l) {
+                    lth[static_cast<size_t>(l) * B + b] = R.fz_theta[l];
+                    tau[static_cast<size_t>(l) * B + b] = eps_store;
+                    tau[static_cast<size_t>(LZ_LMAX + l) * B + b] = eps_store;
+                }
+                lmax_used = std::max(lmax_used, R.nfz);
+            }
+            if (lz_debug())
+                std::fprintf(stderr, "[lz] deflate lane=%d tot=%d pass=%d frozen=%d\n", b, tot[b], R.pass, R.nfz);
+        }
+        BH_HIP(ctx, hipMemcpyAsync(st.scale, hs.data(), B * 8, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(st.beta, hbeta.data(), B * 8, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(st.step, hstep.data(), B * 4, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(st.nlock, hnl.data(), B * 4, hipMemcpyHostToDevice, s));
+        if (!fullre) {
+            const int so2[2] = {1, 0};  // the pass's first two steps orthogonalise exactly against the frozen set
+            BH_HIP(ctx, hipMemcpyAsync(st.ltheta, lth.data(), lth.size() * 8, hipMemcpyHostToDevice, s));
+            BH_HIP(ctx, hipMemcpyAsync(st.tau, tau.data(), tau.size() * 8, hipMemcpyHostToDevice, s));
+            BH_HIP(ctx, hipMemcpyAsync(st.so, so2, 2 * 4, hipMemcpyHostToDevice, s));
+        }
+        BH_HIP(ctx, hipStreamSynchronize(s));
+        return BH_OK;
+    }
+
     bh_ctx* ctx;
     bh_ham* h;
     hipStream_t s;
@@ -2164,7 +2687,11 @@ class LzEngine final : public LzEngineBase {
     std::vector<double> Jb, Ub, hlo, hhi, ha, hb;
     double an0[B] = {};
     Restart plan[B];
+    std::vector<int> plan_fz[B];    // Ritz indices of the levels a lane freezes at its deflation restart
+    std::vector<int> plan_keep[B];  // the lane's frozen slots kept at that restart (ascending)
     int lmax_used = 0;
+    bool mult = true;  // eigenvalues with multiplicity (deflation passes); false: distinct levels
+    int ymax = LZ_LMAX;
 };
 
 // an engine of B lanes (B rounded up to a power of two) in the requested precision
@@ -2206,15 +2733,16 @@ size_t bh_engine_bytes(int64_t D, int prec, int B, const bh_lanczos_opts& o) {
     const size_t es = (prec == BH_PREC_FP64 || fr) ? 8 : 4;
     const size_t vs = static_cast<size_t>(D) * Bp;
     const int64_t rpw = std::max<int64_t>(static_cast<int64_t>(LZ_WG / Bp) * LZ_K1_TILES, (D + LZ_K1_GRID_MAX - 1) / LZ_K1_GRID_MAX);
-    const size_t row_bytes = static_cast<size_t>(2 * LZ_NVMAX + 1 + 2 * LZ_LMAX) * Bp * 8;
+    const int ymax = lz_ymax(D, o);
+    const size_t row_bytes = static_cast<size_t>(std::max(2 * LZ_NVMAX + 1 + 2 * LZ_LMAX, ymax)) * Bp * 8;
     // partial-sum rows: the row-lane and K2 grids, or the staged K1's (up to LZ_PART_BYTES)
     const size_t part_rows = std::max<size_t>(
         static_cast<size_t>(std::max<int64_t>({LZ_GRID_MAX, LZ_K2_GRID, (D + rpw - 1) / rpw})),
         std::min<size_t>(LZ_K1_GRID_MAX, static_cast<size_t>(LZ_PART_BYTES) / row_bytes));
-    return (static_cast<size_t>(slots) + LZ_LMAX + 1) * vs * es + vs * 8 + static_cast<size_t>(D) * 8 +
+    return (static_cast<size_t>(slots) + ymax + 1) * vs * es + vs * 8 + static_cast<size_t>(D) * 8 +
            part_rows * row_bytes +
            (static_cast<size_t>(slots) + 2) * Bp * 24 + static_cast<size_t>(3 * LZ_LMAX + 1) * Bp * 8 +
-           (fr ? static_cast<size_t>(slots) * Bp * 8 : 0);
+           (fr ? static_cast<size_t>(slots + ymax) * Bp * 8 : 0);
 }
 
 // Options of the FP64 rerun of a rejected mixed solve: the plain FP64 solver (selective
@@ -2320,6 +2848,7 @@ void bh_lanczos_default_opts(bh_lanczos_opts* o) {
     o->want_vectors = 0;
     o->ncv = 0;
     o->fullre_d = 0;
+    o->distinct = 0;
 }
 
 namespace {
@@ -2339,8 +2868,36 @@ int solve_diagonal(bh_ctx* ctx, bh_ham* h, const std::vector<int>& runs, const d
         h->vlevels = v;
     }
     std::vector<double> r;
-    if (vecs) r = lz_start_vector(h->D, o.seed);
+    if (vecs && o.distinct) r = lz_start_vector(h->D, o.seed);
+    // with multiplicity: the k lowest diagonal entries, ties in position order, eigenvectors e_q
+    std::vector<int64_t> order;
+    if (!o.distinct) {
+        order.resize(static_cast<size_t>(h->D));
+        for (int64_t q = 0; q < h->D; ++q) order[q] = q;
+        std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return d[x] < d[y]; });
+    }
     for (int b : runs) {
+        if (!o.distinct) {
+            // U V is diagonal: its eigenvalues are the entries U V_q (for U < 0 the largest V come first)
+            std::vector<int64_t> ord(order);
+            if (U[b] < 0) std::stable_sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return d[x] > d[y]; });
+            const int nk = static_cast<int>(std::min<int64_t>(k, h->D));
+            for (int i = 0; i < k; ++i) {
+                const double e = i < nk ? U[b] * static_cast<double>(d[ord[i]]) : NAN;
+                if (res->evals) res->evals[b * k + i] = e;
+                if (res->resid) res->resid[b * k + i] = i < nk ? 0.0 : NAN;
+                if (res->errest) res->errest[b * k + i] = i < nk ? 0.0 : NAN;
+                if (vecs && i < nk) {
+                    double* y = vecs->data() + (static_cast<size_t>(b) * k + i) * h->D;
+                    std::fill(y, y + h->D, 0.0);
+                    y[ord[i]] = 1.0;
+                }
+            }
+            if (res->iters) res->iters[b] = 0;
+            if (res->used_fp64) res->used_fp64[b] = o.prec == BH_PREC_FP64 ? 1 : 0;
+            if (res->status) res->status[b] = nk >= k ? BH_OK : BH_ENOCONV;
+            continue;
+        }
         std::vector<double> lv;
         if (U[b] == 0.0) lv.push_back(0.0);
         else
@@ -2411,7 +2968,8 @@ int bh_lanczos(bh_ctx* ctx, bh_ham* h, int B, const double* J, const double* U, 
                bh_lanczos_result* res) {
     if (!ctx || !h || !J || !U || !opts || !res) return BH_EINVAL;
     if (B < 1 || B > 64) return bh_set_error(ctx, BH_EINVAL, "lanczos: 1 <= B <= 64");
-    if (opts->k < 1 || opts->k > LZ_NVMAX) return bh_set_error(ctx, BH_EINVAL, "lanczos: 1 <= k <= 16");
+    if (opts->k < 1 || opts->k > LZ_KMAX) return bh_set_error(ctx, BH_EINVAL, "lanczos: 1 <= k <= 64");
+    if (opts->distinct && opts->k > LZ_LMAX) return bh_set_error(ctx, BH_EINVAL, "lanczos: distinct levels need k <= 16");
     if (opts->prec != BH_PREC_FP64 && opts->prec != BH_PREC_MIXED) return bh_set_error(ctx, BH_EINVAL, "lanczos: prec");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     BH_HIP(ctx, hipSetDevice(ctx->device));
@@ -2465,6 +3023,46 @@ int bh_lanczos(bh_ctx* ctx, bh_ham* h, int B, const double* J, const double* U, 
     }
     for (int b = 0; b < B; ++b)
         if (st[b] != BH_OK) return bh_set_error(ctx, BH_ENOCONV, "lanczos: some solves did not converge within max_iter");
+    return BH_OK;
+}
+
+// Replaces Op::FOLM_diag (src/operator.cpp:39-71): `nb_iter` Lanczos steps of J T + U V from the normalised
+// Park-Miller start vector, FP64 with full reorthogonalisation (every new vector against the whole basis, two
+// Gram-Schmidt passes), no convergence test.  The reference's loop never subtracts alpha v_i (:47-54), stores
+// beta one row early (:64-67) and leaves T uninitialised after an early stop (:56-57); here the recurrence is
+// the textbook one and T is cut where beta_{t+1} < 1e-6 (the reference's stopping test, :56).
+int bh_lanczos_tridiag(bh_ctx* ctx, bh_ham* h, double J, double U, int nb_iter, uint64_t seed, double* alpha, double* beta,
+                       int* steps, double* basis) {
+    if (!ctx || !h || !alpha || !beta || !steps || nb_iter < 1) return BH_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    BH_HIP(ctx, hipSetDevice(ctx->device));
+    bh_lanczos_opts o;
+    bh_lanczos_default_opts(&o);
+    o.k = 1;
+    o.prec = BH_PREC_FP64;
+    o.fullre_d = std::numeric_limits<int>::max();
+    o.max_iter = nb_iter;
+    o.ncv = nb_iter;
+    o.seed = seed;
+    o.distinct = 1;
+    int rc = BH_OK;
+    auto e = bh_make_engine(ctx, h, BH_PREC_FP64, 1, o, ctx->stream, false, &rc);
+    if (!e) return rc;
+    BH_TRY(e->admit(0, J, U));
+    const int n = static_cast<int>(std::min<int64_t>(nb_iter, h->D));
+    BH_TRY(e->run(n));
+    std::vector<double> a(n), b(n + 1);
+    BH_TRY(e->export_tridiag(n, a.data(), b.data(), nullptr));
+    int used = n;
+    for (int t = 1; t < n; ++t)
+        if (b[t] < 1e-6) {
+            used = t;
+            break;
+        }
+    for (int t = 0; t < used; ++t) alpha[t] = a[t];
+    for (int t = 0; t < used; ++t) beta[t] = b[t];
+    *steps = used;
+    if (basis) BH_TRY(e->export_tridiag(used, a.data(), b.data(), basis));
     return BH_OK;
 }
 

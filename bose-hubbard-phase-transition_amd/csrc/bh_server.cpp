@@ -6,6 +6,7 @@
 // flight (its lanes are the batch), each on its own stream, lanes refilled as solves finish, sectors
 // admitted and evicted against an HBM budget.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -71,6 +72,10 @@ struct bh_server {
     std::vector<double> latencies;
     double first_submit = -1.0, last_done = 0.0;
     int64_t in_flight = 0;
+    // read by bh_server_stats_get while the worker runs: kept as counters the worker updates, so that the
+    // reader never walks the sector map (which the worker changes without the lock)
+    std::atomic<int> engines{0};
+    std::atomic<size_t> hbm_used{0};
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 };
@@ -133,8 +138,13 @@ void evict_for(bh_server* S, size_t need, Sector* keep) {
         }
         if (!victim) return;
         for (int p = 0; p < 2; ++p) {
+            // a check may have left speculative steps on the stream: drain it before the engine's graphs,
+            // its buffers and the stream itself are released
+            if (victim->stream[p]) (void)hipStreamSynchronize(victim->stream[p]);
+            if (victim->eng[p]) S->engines -= 1;
             victim->eng[p].reset();
             if (victim->stream[p]) (void)hipStreamDestroy(victim->stream[p]);
+            victim->stream[p] = nullptr;
         }
         if (victim->ham) bh_ham_destroy(victim->ham);
         if (victim->basis) bh_basis_destroy(victim->basis);
@@ -179,7 +189,9 @@ int ensure_engine(bh_server* S, Sector& sc, int p) {
         // rebuild an idle engine that is too small for its queue
         if (sc.eng[p]->active() > 0 || static_cast<int>(sc.queue[p].size()) <= sc.eng[p]->lanes()) return BH_OK;
         if (lanes_wanted(S, sc, p) <= sc.eng[p]->lanes()) return BH_OK;
+        (void)hipStreamSynchronize(sc.stream[p]);
         sc.eng[p].reset();
+        S->engines -= 1;
         sc.eng_bytes[p] = 0;
     }
     BH_TRY(ensure_ham(S, sc));
@@ -198,6 +210,7 @@ int ensure_engine(bh_server* S, Sector& sc, int p) {
     int rc = BH_OK;
     sc.eng[p] = bh_make_engine(S->ctx, sc.ham, prec, L, lo, sc.stream[p], false, &rc);
     if (!sc.eng[p]) return rc;
+    S->engines += 1;
     sc.eng_bytes[p] = sc.eng[p]->device_bytes();
     sc.lane_req[p].assign(sc.eng[p]->lanes(), Pending{});
     sc.lane_busy[p].assign(sc.eng[p]->lanes(), 0);
@@ -364,6 +377,7 @@ void worker_loop(bh_server* S) {
                 }
             }
         }
+        S->hbm_used = S->ctx->used;
         if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
 }
@@ -450,11 +464,8 @@ int bh_server_stats_get(bh_server* S, bh_server_stats* st) {
     st->completed = S->completed;
     st->in_flight = S->in_flight;
     st->queued = static_cast<int64_t>(S->inbox.size());
-    st->hbm_used = S->ctx->used;
-    int ne = 0;
-    for (auto& kv : S->sectors)
-        for (int p = 0; p < 2; ++p) ne += kv.second->eng[p] ? 1 : 0;
-    st->engines = ne;
+    st->hbm_used = S->hbm_used;
+    st->engines = S->engines;
     if (!S->latencies.empty()) {
         std::vector<double> l(S->latencies);
         std::sort(l.begin(), l.end());

@@ -322,9 +322,14 @@ SparseMatrix fixed_bosons_hamiltonian(const std::vector<std::vector<int>>& neigh
 
 namespace Op {
 
+namespace {
+constexpr int KMAX = 64;  // eigenvalues per Lanczos solve (bh_lanczos)
+}  // namespace
+
 std::vector<std::complex<double>> IRLM_eigen(const BH::SparseMatrix& O, int nb_eigen, bhx::MatrixXcd& eigenvectors) {
     if (!O.blocks.empty()) {
-        // block by block: the lowest nb_eigen levels of each block (all of a smaller block), merged
+        // block by block: the lowest nb_eigen eigenvalues of each block (all of a small block), merged
+        if (nb_eigen < 1) throw std::invalid_argument("nev must satisfy 1 <= nev <= n - 2, n is the size of matrix");
         struct Lv {
             double e;
             size_t blk;
@@ -336,22 +341,17 @@ std::vector<std::complex<double>> IRLM_eigen(const BH::SparseMatrix& O, int nb_e
         for (size_t q = 0; q < O.blocks.size(); ++q) {
             const auto& blk = O.blocks[q];
             off[q + 1] = off[q] + blk.rows();
-            const int kb = static_cast<int>(std::min<int64_t>(std::min(nb_eigen, 16), blk.rows()));
+            const int kb = static_cast<int>(std::min<int64_t>(nb_eigen, blk.rows()));
             if (kb < 1) continue;
             std::vector<std::complex<double>> e;
-            if (blk.rows() <= 1500) {
-                // small blocks (down to the one-state vacuum) by the dense host solve, reduced to distinct
-                // levels as the Lanczos path returns them (values within 1e-9 of the spectrum's scale merged)
+            if (2 * static_cast<int64_t>(kb) + 1 > blk.rows() || kb > KMAX) {
+                // a block the Lanczos solver cannot take (down to the one-state vacuum): dense solve
                 bhx::MatrixXd V;
                 const auto w = exact_eigen(blk, V);
-                const double scale = std::max({std::abs(w.front()), std::abs(w.back()), 1e-300});
-                std::vector<int> pick;
-                for (int i = 0; i < static_cast<int>(w.size()) && static_cast<int>(pick.size()) < kb; ++i)
-                    if (pick.empty() || std::abs(w[i] - w[pick.back()]) > 1e-9 * scale) pick.push_back(i);
-                bv[q].resize(blk.rows(), static_cast<int64_t>(pick.size()));
-                for (size_t i = 0; i < pick.size(); ++i) {
-                    e.emplace_back(w[pick[i]], 0.0);
-                    for (int64_t r = 0; r < blk.rows(); ++r) bv[q](r, static_cast<int64_t>(i)) = V(r, pick[i]);
+                bv[q].resize(blk.rows(), kb);
+                for (int i = 0; i < kb; ++i) {
+                    e.emplace_back(w[i], 0.0);
+                    for (int64_t r = 0; r < blk.rows(); ++r) bv[q](r, i) = V(r, i);
                 }
             } else {
                 e = IRLM_eigen(blk, kb, bv[q]);
@@ -359,7 +359,7 @@ std::vector<std::complex<double>> IRLM_eigen(const BH::SparseMatrix& O, int nb_e
             for (size_t i = 0; i < e.size(); ++i) all.push_back({e[i].real(), q, static_cast<int64_t>(i)});
         }
         std::stable_sort(all.begin(), all.end(), [](const Lv& a, const Lv& b) { return a.e < b.e; });
-        const int k = static_cast<int>(std::min<size_t>(static_cast<size_t>(std::min(nb_eigen, 16)), all.size()));
+        const int k = static_cast<int>(std::min<size_t>(static_cast<size_t>(nb_eigen), all.size()));
         if (k < 1) throw std::runtime_error("Eigenvalue computation failed.");
         eigenvectors.resize(O.rows(), k);
         std::vector<std::complex<double>> out;
@@ -371,8 +371,14 @@ std::vector<std::complex<double>> IRLM_eigen(const BH::SparseMatrix& O, int nb_e
         return out;
     }
     if (!O.sec) throw std::invalid_argument("empty operator");
-    if (nb_eigen < 1) throw std::invalid_argument("nb_eigen must be positive");
-    const int k = std::min(nb_eigen, 16);
+    const int64_t D = O.rows();
+    // Spectra's argument checks for nev = nb_eigen, ncv = 2 nb_eigen + 1 (GenEigsBase.h:335-339)
+    if (nb_eigen < 1 || nb_eigen > D - 2)
+        throw std::invalid_argument("nev must satisfy 1 <= nev <= n - 2, n is the size of matrix");
+    if (2 * static_cast<int64_t>(nb_eigen) + 1 > D)
+        throw std::invalid_argument("ncv must satisfy nev + 2 <= ncv <= n, n is the size of matrix");
+    if (nb_eigen > KMAX) throw std::invalid_argument("IRLM_eigen: nb_eigen above 64 is not supported");
+    const int k = nb_eigen;
     bh_lanczos_opts o;
     bh_lanczos_default_opts(&o);
     o.k = k;
@@ -384,7 +390,6 @@ std::vector<std::complex<double>> IRLM_eigen(const BH::SparseMatrix& O, int nb_e
     bh_ctx* ctx = bhx::context();
     const int rc = bh_lanczos(ctx, O.sec->ham, 1, &O.J, &O.U, &o, &r);
     if (rc != BH_OK || st != BH_OK) throw std::runtime_error("Eigenvalue computation failed.");
-    const int64_t D = O.rows();
     std::vector<double> vec(static_cast<size_t>(k * D));
     bhx::check(bh_lanczos_copy_vectors(ctx, 1, k, vec.data()), "IRLM_eigen vectors");
     std::vector<std::complex<double>> out;
@@ -396,21 +401,43 @@ std::vector<std::complex<double>> IRLM_eigen(const BH::SparseMatrix& O, int nb_e
     return out;
 }
 
+std::vector<double> FOLM_eigen(const BH::SparseMatrix& O, int nb_iter, bhx::MatrixXd& eigenvectors) {
+    if (!O.blocks.empty() || !O.sec) throw std::invalid_argument("FOLM_eigen: one sector operator expected");
+    if (nb_iter < 1) throw std::invalid_argument("FOLM_eigen: nb_iter must be positive");
+    const int64_t D = O.rows();
+    const int n = static_cast<int>(std::min<int64_t>(nb_iter, D));
+    std::vector<double> a(n), b(n), basis(static_cast<size_t>(n) * D);
+    int steps = 0;
+    bhx::check(bh_lanczos_tridiag(bhx::context(), O.sec->ham, O.J, O.U, nb_iter, 0, a.data(), b.data(), &steps, basis.data()),
+               "FOLM_eigen");
+    // every eigenpair of T (steps x steps) by Jacobi rotations; T's vectors carried to H's space by V
+    std::vector<double> T(static_cast<size_t>(steps) * steps, 0.0), w(steps), S(static_cast<size_t>(steps) * steps);
+    for (int t = 0; t < steps; ++t) {
+        T[static_cast<size_t>(t) * steps + t] = a[t];
+        if (t > 0) T[static_cast<size_t>(t) * steps + t - 1] = T[static_cast<size_t>(t - 1) * steps + t] = b[t];
+    }
+    bhtri::jacobi_eig(steps, T.data(), w.data(), S.data());
+    eigenvectors.resize(D, steps);
+    for (int c = 0; c < steps; ++c)
+        for (int t = 0; t < steps; ++t) {
+            const double f = S[static_cast<size_t>(t) * steps + c];
+            const double* v = basis.data() + static_cast<size_t>(t) * D;
+            for (int64_t r = 0; r < D; ++r) eigenvectors(r, c) += f * v[r];
+        }
+    for (auto& x : w) x += O.Mu * O.sec->n;
+    return w;
+}
+
 std::vector<double> exact_eigen(const BH::SparseMatrix& O, bhx::MatrixXd& eigenvectors) {
     const int64_t D = O.rows();
-    if (D > 1500) throw std::invalid_argument("exact_eigen: dense host solve limited to D <= 1500");
     std::vector<int64_t> rp;
     std::vector<int32_t> c;
     std::vector<double> v;
     O.to_csr(rp, c, v);
-    const int n = static_cast<int>(D);
-    std::vector<double> A(static_cast<size_t>(n) * n, 0.0), w(n), V(static_cast<size_t>(n) * n);
-    for (int r = 0; r < n; ++r)
-        for (int64_t q = rp[r]; q < rp[r + 1]; ++q) A[static_cast<size_t>(r) * n + c[q]] += v[q];
-    bhtri::jacobi_eig(n, A.data(), w.data(), V.data());
+    std::vector<double> w(static_cast<size_t>(D));
     eigenvectors.resize(D, D);
-    for (int r = 0; r < n; ++r)
-        for (int cc = 0; cc < n; ++cc) eigenvectors(r, cc) = V[static_cast<size_t>(r) * n + cc];
+    const int rc = bh_dense_eigen(bhx::context(), D, rp.data(), c.data(), v.data(), w.data(), eigenvectors.v.data());
+    if (rc != BH_OK) throw std::runtime_error("Eigenvalue computation failed.");
     return w;
 }
 
@@ -575,6 +602,7 @@ double condensate_fraction(const bhx::MatrixXcd& spdm) {
 
 void exact_parameters(int m, int n, double J, double U, double mu, double s, double r, std::string fixed_param,
                       const std::string& out) {
+    constexpr int KMAX_SWEEP = 64;  // eigenvalues per Lanczos solve (bh_lanczos)
     const double eps = std::numeric_limits<double>::epsilon();
     if (std::abs(J) < eps && std::abs(U) < eps && std::abs(mu) < eps) {
         std::cerr << "Error: At least one of the parameters J, U, mu must be different from zero." << std::endl;
@@ -621,8 +649,9 @@ void exact_parameters(int m, int n, double J, double U, double mu, double s, dou
     for (const auto& h : H) lev[{h.J, h.U}];
     int nb_eigen = 20;
     auto sec = JH.sec;
+    const int64_t Dsec = sec->basis ? bh_basis_dim(sec->basis) : 0;
     for (;;) {
-        const int k = std::min(nb_eigen, 16);
+        const int k = nb_eigen;
         std::vector<std::pair<double, double>> keys;
         for (auto& kv : lev) keys.push_back(kv.first);
         for (size_t q0 = 0; q0 < keys.size(); q0 += 64) {
@@ -651,12 +680,14 @@ void exact_parameters(int m, int n, double J, double U, double mu, double s, dou
             gr[idx] = rr.empty() ? 0.0 : std::accumulate(rr.begin(), rr.end(), 0.0) / rr.size();
         }
         // the reference recomputes with 5 more eigenvalues while the gap ratios barely vary
-        // (src/analysis.cpp:364-381); here the level count stops at 16
+        // (src/analysis.cpp:364-381); its solver throws once 2 nb_eigen + 1 exceeds D (GenEigsBase.h:335-339),
+        // where this loop stops with the last grid it solved (also at the solver's 64 eigenvalues)
         const double mean = std::accumulate(gr.begin(), gr.end(), 0.0) / npt;
         double var = 0.0;
         for (double g : gr) var += (g - mean) * (g - mean);
         var /= npt;
-        if (var > 1e-8 * mean || k == 16) break;
+        if (var > 1e-8 * mean) break;
+        if (nb_eigen + 5 > KMAX_SWEEP || 2 * static_cast<int64_t>(nb_eigen + 5) + 1 > Dsec) break;
         nb_eigen += 5;
     }
     // condensate fraction and coherence from the ground state of each distinct (J, U) pair (-mu N leaves the
@@ -679,6 +710,7 @@ void exact_parameters(int m, int n, double J, double U, double mu, double s, dou
             o.k = 1;
             o.prec = bhx::settings().prec;
             o.want_vectors = 1;
+            o.distinct = 1;  // one ground state vector: the pass that finds E0 is enough
             bh_lanczos_result res{ev.data(), nullptr, nullptr, nullptr, nullptr, st.data()};
             const int rc = bh_lanczos(bhx::context(), sec->ham, B, Jb.data(), Ub.data(), &o, &res);
             if (rc != BH_OK && rc != BH_ENOCONV) throw std::runtime_error("Eigenvalue computation failed.");

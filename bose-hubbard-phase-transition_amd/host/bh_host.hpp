@@ -4,7 +4,8 @@
 //   class Neighbours               include/neighbours.hpp:12-62   (src/neighbours.cpp:21-71, 126-128)
 //   BH::fixed_set_basis            include/hamiltonian.hpp:64     (src/hamiltonian.cpp:143-149)
 //   BH::fixed_bosons_hamiltonian   include/hamiltonian.hpp:87     (src/hamiltonian.cpp:238-256)
-//   Op::IRLM_eigen, Op::exact_eigen include/operator.hpp:34, 57   (src/operator.cpp:22-33, 92-101)
+//   Op::IRLM_eigen, Op::FOLM_eigen, Op::exact_eigen  include/operator.hpp:34, 45, 55
+//                                                     (src/operator.cpp:22-33, 39-87, 92-101)
 //   Analysis::gap_ratios, Analysis::exact_parameters  include/analysis.hpp (src/analysis.cpp:207-387,433-454)
 // over the C ABI of include/bh_capi.h.  Eigen types are replaced by the small dense containers below
 // (column-major, as Eigen's defaults).  A sparse matrix is held as a linear combination
@@ -140,13 +141,24 @@ SparseMatrix fixed_bosons_hamiltonian(const std::vector<std::vector<int>>& neigh
 
 namespace Op {
 
-// Lowest nb_eigen eigenvalues (distinct levels, ascending; at most 16) and eigenvectors (D x nb_eigen)
-// of O by the device Lanczos solver; std::runtime_error("Eigenvalue computation failed.") when it does
-// not converge (src/operator.cpp:26-29).  A block-diagonal O is solved block by block and the lowest
-// nb_eigen values over all blocks are returned (levels of different blocks are different states).
+// Lowest nb_eigen eigenvalues of O counted with their multiplicity, ascending, and eigenvectors
+// (D x nb_eigen), by the device Lanczos solver (deflation passes, bh_lanczos).  As Spectra's GenEigsSolver
+// with nev = nb_eigen and ncv = 2 nb_eigen + 1 (src/operator.cpp:24, GenEigsBase.h:335-339) it throws
+// std::invalid_argument unless 1 <= nb_eigen <= D - 2 and 2 nb_eigen + 1 <= D; nb_eigen above 64 is refused
+// the same way (the reference has no such limit); std::runtime_error("Eigenvalue computation failed.") when
+// the solve does not converge (src/operator.cpp:26-29).  A block-diagonal O is solved block by block (a block
+// too small for the Lanczos solver by the dense solve) and the lowest nb_eigen values over all blocks are
+// returned.
 std::vector<std::complex<double>> IRLM_eigen(const BH::SparseMatrix& O, int nb_eigen, bhx::MatrixXcd& eigenvectors);
 
-// All eigenvalues (ascending) and eigenvectors of O by a dense host solve (D <= 1500).
+// nb_iter Lanczos steps from a start vector of the Park-Miller sequence (bh_lanczos_tridiag), then every
+// eigenpair of the tridiagonal T: its eigenvalues ascending, and eigenvectors V S (D x steps) for the Lanczos
+// basis V and T's eigenvectors S (src/operator.cpp:74-87).  Fewer than nb_iter values when the recurrence
+// stops at beta < 1e-6.
+std::vector<double> FOLM_eigen(const BH::SparseMatrix& O, int nb_iter, bhx::MatrixXd& eigenvectors);
+
+// All eigenvalues (ascending) and eigenvectors (columns) of O by a dense FP64 solve on the device
+// (bh_dense_eigen); std::runtime_error when it fails (src/operator.cpp:92-101).
 std::vector<double> exact_eigen(const BH::SparseMatrix& O, bhx::MatrixXd& eigenvectors);
 
 }  // namespace Op

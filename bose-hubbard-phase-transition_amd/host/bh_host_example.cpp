@@ -12,6 +12,10 @@
 // v_i = 1 / (i + 1) (host only, no device); with --scmf, Analysis::SCMF(mu, J, q, psi0, precision).
 //   bhx-host-example --hmf PSI P MU J Q
 //   bhx-host-example --scmf MU J Q PSI0 PRECISION
+// With --folm, Op::FOLM_eigen(H, NITER) of the same H: its values, the Lanczos steps taken and the residual
+// norm |H v - e v| of each returned vector.
+//   bhx-host-example --folm M N J U MU NITER
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -108,8 +112,51 @@ static int run_scmf(int argc, char** argv) {
     return 0;
 }
 
+static int run_folm(int argc, char** argv) {
+    if (argc < 8) {
+        std::fprintf(stderr, "usage: bhx-host-example --folm M N J U MU NITER\n");
+        return 1;
+    }
+    const int m = std::atoi(argv[2]), n = std::atoi(argv[3]), nit = std::atoi(argv[7]);
+    const double J = std::atof(argv[4]), U = std::atof(argv[5]), mu = std::atof(argv[6]);
+    try {
+        Neighbours nb(m);
+        nb.chain_neighbours();
+        const auto nei = nb.getNeighbours();
+        auto [tags, basis] = BH::fixed_set_basis(m, n);
+        const BH::SparseMatrix H = BH::fixed_bosons_hamiltonian(nei, basis, tags, m, n, 1, 0, 0) * J +
+                                   BH::fixed_bosons_hamiltonian(nei, basis, tags, m, n, 0, 1, 0) * U +
+                                   BH::fixed_bosons_hamiltonian(nei, basis, tags, m, n, 0, 0, 1) * mu;
+        std::vector<int64_t> rp;
+        std::vector<int32_t> col;
+        std::vector<double> val;
+        H.to_csr(rp, col, val);
+        bhx::MatrixXd vecs;
+        const auto ev = Op::FOLM_eigen(H, nit, vecs);
+        std::printf("{\"steps\": %zu, \"values\": [", ev.size());
+        for (size_t i = 0; i < ev.size(); ++i) std::printf("%s%.17g", i ? ", " : "", ev[i]);
+        std::printf("], \"residual\": [");
+        for (size_t q = 0; q < ev.size(); ++q) {
+            double r2 = 0.0;
+            for (int64_t r = 0; r + 1 < static_cast<int64_t>(rp.size()); ++r) {
+                double hv = 0.0;
+                for (int64_t e = rp[r]; e < rp[r + 1]; ++e) hv += val[e] * vecs(col[e], q);
+                const double d = hv - ev[q] * vecs(r, q);
+                r2 += d * d;
+            }
+            std::printf("%s%.6g", q ? ", " : "", std::sqrt(r2));
+        }
+        std::printf("]}\n");
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 3;
+    }
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc > 1 && std::string(argv[1]) == "--max") return run_max(argc, argv);
+    if (argc > 1 && std::string(argv[1]) == "--folm") return run_folm(argc, argv);
     if (argc > 1 && std::string(argv[1]) == "--hmf") return run_hmf(argc, argv);
     if (argc > 1 && std::string(argv[1]) == "--scmf") return run_scmf(argc, argv);
     if (argc < 6) {
@@ -149,12 +196,24 @@ int main(int argc, char** argv) {
                         return t;
                     }());
         std::printf(", \"vec_rows\": %lld, \"exact\": [", static_cast<long long>(vecs.rows()));
-        if (H.rows() <= 1500) {
+        double exres = 0.0;
+        if (H.rows() <= 8000) {
+            // the dense solve: its lowest values and the largest residual |H v - e v| over its first vectors
             bhx::MatrixXd ex;
             const auto w = Op::exact_eigen(H, ex);
-            for (size_t i = 0; i < w.size() && i < 8; ++i) std::printf("%s%.17g", i ? ", " : "", w[i]);
+            for (size_t i = 0; i < w.size() && i < 12; ++i) std::printf("%s%.17g", i ? ", " : "", w[i]);
+            for (int64_t q = 0; q < ex.cols() && q < 12; ++q) {
+                double r2 = 0.0;
+                for (int64_t r = 0; r + 1 < static_cast<int64_t>(rp.size()); ++r) {
+                    double hv = 0.0;
+                    for (int64_t e = rp[r]; e < rp[r + 1]; ++e) hv += val[e] * ex(col[e], q);
+                    const double d = hv - w[q] * ex(r, q);
+                    r2 += d * d;
+                }
+                exres = std::max(exres, std::sqrt(r2));
+            }
         }
-        std::printf("]}\n");
+        std::printf("], \"exact_residual\": %.6g}\n", exres);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return 3;

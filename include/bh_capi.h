@@ -181,10 +181,14 @@ int bh_spmv_bench(bh_ctx* ctx, bh_ham* h, int prec, int B, int reps, double* avg
  * src/operator.cpp:22-33, Spectra GenEigs nev/ncv=2nev+1/SmallestReal/tol 1e-10).
  * Lanczos with selective orthogonalisation against converged Ritz vectors, FP64 alpha/beta
  * reductions, FP64 tridiagonal analysis; B independent solves (J_b, U_b) share one CSR pass.
- * Results are the lowest k distinct eigenvalues of H_b (a single start vector meets each
- * degenerate level once; values within dedup_rel*||T|| count as one level). */
+ * Results are the lowest k eigenvalues of H_b counted with their multiplicity, ascending, as the
+ * reference's Spectra solve returns them (the two members of each +-momentum pair of a periodic chain
+ * appear twice).  A single start vector meets each degenerate eigenspace once, so a lane whose pass has
+ * converged freezes the vectors it found and starts a further pass from a fresh vector orthogonal to
+ * them; the solve ends with a pass that finds nothing new among the lowest k.  With distinct = 1 the
+ * result is instead the lowest k distinct levels (one pass; values within dedup_rel*||T|| count as one). */
 typedef struct bh_lanczos_opts {
-    int k;             /* wanted eigenvalues (distinct levels), >= 1 */
+    int k;             /* wanted eigenvalues, 1 <= k <= 64 (distinct levels: k <= 16) */
     int prec;          /* BH_PREC_FP64 or BH_PREC_MIXED */
     int max_iter;      /* Lanczos steps (operator applications) per solve, over all restarts (0: 10000) */
     int chunk;         /* steps between convergence checks (0: 16) */
@@ -198,7 +202,9 @@ typedef struct bh_lanczos_opts {
     int ncv;           /* Krylov vectors kept per solve (Spectra's ncv); a solve that fills them is thick-
                           restarted with its lowest Ritz vectors (0: min(D, 512); at least 2k + 8) */
     int fullre_d;      /* sectors with D <= fullre_d run FP64 Lanczos with full reorthogonalisation whatever
-                          prec asks (0: 16384; -1: never); used_fp64 reports it */
+                          prec asks (0: 16384; -1: never); used_fp64 reports it.  With multiplicities, k > 6
+                          also runs that way */
+    int distinct;      /* 0 (default): eigenvalues with multiplicity; 1: distinct levels */
 } bh_lanczos_opts;
 
 typedef struct bh_lanczos_result {
@@ -217,6 +223,26 @@ int bh_lanczos(bh_ctx* ctx, bh_ham* h, int B, const double* J, const double* U, 
 /* Ritz vectors of the last bh_lanczos call with want_vectors (host [B][k][D] doubles). */
 int bh_lanczos_copy_vectors(bh_ctx* ctx, int B, int k, double* vecs);
 
+/* Replaces Op::FOLM_diag (include/operator.hpp:45 with FOLM_eigen, src/operator.cpp:39-71): nb_iter steps of
+ * Lanczos on J T + U V from the normalised Park-Miller start vector of `seed` (the reference draws
+ * Eigen's VectorXd::Random, :77), FP64 with full reorthogonalisation and no convergence test.  Outputs the
+ * tridiagonal T: alpha[t] (t < *steps) on the diagonal, beta[t] coupling t-1 and t (beta[0] = 0), and, when
+ * basis is not NULL, the normalised Lanczos vectors as basis[t*D + r].  *steps is nb_iter (at most D) or,
+ * when some beta_t < 1e-6 (the reference's stopping test, :56), the first such t.  The reference's defects
+ * (alpha v_i never subtracted, beta stored one row early, T left uninitialised after a stop: :47-67) are
+ * not reproduced.  alpha, beta need nb_iter entries, basis nb_iter * D. */
+int bh_lanczos_tridiag(bh_ctx* ctx, bh_ham* h, double J, double U, int nb_iter, uint64_t seed, double* alpha, double* beta,
+                       int* steps, double* basis);
+
+/* Replaces Op::exact_eigen (include/operator.hpp:55, src/operator.cpp:92-101: a dense copy of the sparse
+ * matrix and Eigen's SelfAdjointEigenSolver): all n eigenvalues of the symmetric n x n matrix given as a host
+ * CSR (row_ptr[n+1], col, val; entries summed where repeated), ascending, and, when evecs is not NULL, the
+ * orthonormal eigenvectors as its columns (column-major: evecs[c*n + r] is component r of eigenvector c).
+ * The matrix is densified in HBM and solved by rocSOLVER's divide-and-conquer dsyevd (n*n doubles on the
+ * device). */
+int bh_dense_eigen(bh_ctx* ctx, int64_t n, const int64_t* row_ptr, const int32_t* col, const double* val, double* evals,
+                   double* evecs);
+
 /* ---- batched phase-diagram sweep ----------------------------------------------------------------- */
 /* Replaces the grid loop of Analysis::exact_parameters / calculate_and_save (src/analysis.cpp:207-387),
  * which assembles H = H_fixed + H1 p1 + H2 p2 per grid point and solves each point on its own.
@@ -224,7 +250,8 @@ int bh_lanczos_copy_vectors(bh_ctx* ctx, int B, int k, double* vecs);
  * values of a sector are solved together, one CSR pass per Lanczos step for the whole batch.  The
  * chemical potential enters afterwards as the N-dependent shift -mu N.  Outputs per sector q
  * (0: n-1, 1: n, 2: n+1) and J point i:
- *   E[(q*nJ + i)*k + l]   lowest k distinct levels of J T + U V (no mu term), ascending
+ *   E[(q*nJ + i)*k + l]   lowest k eigenvalues of J T + U V (no mu term), ascending, with multiplicity
+ *                         (distinct levels when opts->distinct)
  *   gap_ratio[q*nJ + i]   mean of min/max adjacent gaps over those levels (src/analysis.cpp:433-454)
  * and per J point i and mu point c:
  *   density[i*nmu + c]          N* / m with N* = argmin_{N in n-1..n+1} E0(N) - mu N (ties: smaller N)

@@ -37,6 +37,9 @@
  *                            (Util/SimpleRandom.h:30-96, LinAlg/Arnoldi.h:147-154) and convergence
  *                            rule (GenEigsBase.h:141-158); plain Lanczos with full
  *                            reorthogonalisation instead of the implicitly restarted Arnoldi
+ *   orc_lanczos_mult         src/operator.cpp:22-33: the lowest k eigenvalues with their multiplicity, as the
+ *                            reference's Spectra solve returns them (deflation passes from fresh start
+ *                            vectors, full reorthogonalisation, a final Rayleigh-Ritz step)
  *   orc_gap_ratios           src/analysis.cpp:433-454
  *   orc_mf_ground            src/hamiltonian.cpp:290-309 (BH::h_MF) and the ground state the reference asks
  *                            of Spectra's SymEigsSolver (src/analysis.cpp:128-133), with SF_density
@@ -610,6 +613,327 @@ out:
     free(h);
     free(td);
     free(z);
+    return rc;
+}
+
+/* ---- eigenvalues with multiplicity ---------------------------------------------------------- */
+
+/* x_{off+1} .. x_{off+D} of the Park-Miller sequence of orc_start_vector (x_i = 16807^i x_0 mod 2^31-1) as
+ * values x / (2^31 - 1) - 0.5: the sequence continued past its first off values */
+void orc_start_vector_at(int64_t D, uint64_t seed, uint64_t off, double *r) {
+    const uint64_t p = 2147483647ull;
+    uint64_t e = off, base = 16807ull, pw = 1;
+    while (e) {
+        if (e & 1) pw = (pw * base) % p;
+        base = (base * base) % p;
+        e >>= 1;
+    }
+    uint64_t x = (pw * (seed ? (seed & p) : 1)) % p;
+    for (int64_t i = 0; i < D; ++i) {
+        x = (16807ull * x) % p;
+        r[i] = (double)x / (double)p - 0.5;
+    }
+}
+
+/* z -= sum_c <Q_c, z> Q_c over the nq columns of Q and the nr columns of R (D x n, column-major), with both
+ * sets of coefficients taken from the same z (two passes); returns ||z||.  Projecting out one set after the
+ * other would let the second subtraction put back components along the first. */
+static double orc_cgs2(int64_t D, const double *Q, int nq, const double *R, int nr, double *z, double *h) {
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int c = 0; c < nq; ++c) h[c] = dot(D, Q + (size_t)D * c, z);
+        for (int c = 0; c < nr; ++c) h[nq + c] = dot(D, R + (size_t)D * c, z);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < D; ++i) {
+            double s = 0.0;
+            for (int c = 0; c < nq; ++c) s += Q[(size_t)D * c + i] * h[c];
+            for (int c = 0; c < nr; ++c) s += R[(size_t)D * c + i] * h[nq + c];
+            z[i] -= s;
+        }
+    }
+    return sqrt(dot(D, z, z));
+}
+
+/* eigenpairs of a small symmetric matrix A (n x n, row-major, destroyed) by cyclic Jacobi rotations:
+ * w ascending, V[i*n + c] = component i of eigenvector c */
+static void orc_jacobi(int n, double *A, double *w, double *V) {
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) V[i * n + j] = i == j ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                tot += A[i * n + j] * A[i * n + j];
+                if (i != j) off += A[i * n + j] * A[i * n + j];
+            }
+        if (off <= 1e-32 * tot || off == 0.0) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = A[p * n + q];
+                if (apq == 0.0) continue;
+                const double th = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+                for (int r = 0; r < n; ++r) {
+                    const double arp = A[r * n + p], arq = A[r * n + q];
+                    A[r * n + p] = c * arp - sn * arq;
+                    A[r * n + q] = sn * arp + c * arq;
+                }
+                for (int r = 0; r < n; ++r) {
+                    const double apr = A[p * n + r], aqr = A[q * n + r];
+                    A[p * n + r] = c * apr - sn * aqr;
+                    A[q * n + r] = sn * apr + c * aqr;
+                }
+                for (int r = 0; r < n; ++r) {
+                    const double vrp = V[r * n + p], vrq = V[r * n + q];
+                    V[r * n + p] = c * vrp - sn * vrq;
+                    V[r * n + q] = sn * vrp + c * vrq;
+                }
+            }
+    }
+    for (int i = 0; i < n; ++i) w[i] = A[i * n + i];
+    /* ascending order (selection sort of values and vector columns) */
+    for (int i = 0; i < n; ++i) {
+        int m = i;
+        for (int j = i + 1; j < n; ++j)
+            if (w[j] < w[m]) m = j;
+        if (m == i) continue;
+        double t = w[i];
+        w[i] = w[m];
+        w[m] = t;
+        for (int r = 0; r < n; ++r) {
+            t = V[r * n + i];
+            V[r * n + i] = V[r * n + m];
+            V[r * n + m] = t;
+        }
+    }
+}
+
+static int cmp_double(const void *a, const void *b) {
+    const double x = *(const double *)a, y = *(const double *)b;
+    return (x > y) - (x < y);
+}
+
+/*
+ * Lowest k eigenvalues of H = J*T + U*V counted with their multiplicity, as Op::IRLM_eigen returns them
+ * (src/operator.cpp:22-33: Spectra GenEigs, nev = k, SmallestReal, tol).  FP64, full reorthogonalisation.
+ * A single start vector meets each eigenspace once, so the method runs passes: pass 0 starts from
+ * H r / ||H r|| (Spectra's Arnoldi init, r = orc_start_vector(seed)), pass p > 0 from the Park-Miller sequence
+ * continued past p*D values; every vector of a pass is orthogonalised (two classical Gram-Schmidt passes)
+ * against the vectors frozen by the earlier passes and against the pass's own basis.  A pass ends when its
+ * distinct levels (Ritz values closer than 1e-10 ||T|| count once) at or below kappa -- the k-th lowest of the
+ * frozen values and the pass's levels -- and its lowest level have converged (Spectra's rule
+ * |beta s_last| < tol max(eps^(2/3), |theta|), GenEigsBase.h:141-158).  Its levels at or below kappa are then
+ * frozen (Ritz vectors, orthonormalised against the frozen set; frozen values now above kappa are dropped)
+ * and a new pass starts; a pass that adds nothing at or below kappa ends the solve.  A last Rayleigh-Ritz step over the frozen vectors and the last
+ * pass's levels gives evals (ascending) and resid (||H y - theta y||).  max_iter bounds the steps of a pass.
+ */
+int orc_lanczos_mult(int64_t D, const int64_t *row_ptr, const int32_t *col, const double *val, const double *diag,
+                     double J, double U, int k, double tol, int max_iter, uint64_t seed, double *evals, double *resid,
+                     int *iters) {
+    if (k < 1 || max_iter < 1 || D < 1) return ORC_EINVAL;
+    if (max_iter > D) max_iter = (int)D;
+    const int fcap = 2 * k + 8;
+    const double eps = 2.220446049250313e-16, eps23 = pow(eps, 2.0 / 3.0);
+    double *F = (double *)malloc(sizeof(double) * (size_t)D * (size_t)fcap);   /* frozen vectors */
+    double *V = (double *)malloc(sizeof(double) * (size_t)D * (size_t)(max_iter + 1));
+    double *w = (double *)malloc(sizeof(double) * (size_t)D);
+    double *alpha = (double *)malloc(sizeof(double) * (size_t)(max_iter + 1));
+    double *beta = (double *)malloc(sizeof(double) * (size_t)(max_iter + 2));
+    double *h = (double *)malloc(sizeof(double) * (size_t)(max_iter + fcap + 1));
+    double *td = (double *)malloc(sizeof(double) * (size_t)(max_iter + 1));
+    double *fth = (double *)malloc(sizeof(double) * (size_t)fcap);
+    double *z = NULL, *mv = NULL;
+    int *lev = (int *)malloc(sizeof(int) * (size_t)(max_iter + 1));
+    int rc = ORC_ENOCONV, nf = 0, total = 0, last_n = 0, nlast = 0;
+    int *last = (int *)malloc(sizeof(int) * (size_t)(k + 1));
+    if (!F || !V || !w || !alpha || !beta || !h || !td || !fth || !lev || !last) {
+        rc = ORC_ENOMEM;
+        goto out;
+    }
+    for (int pass = 0;; ++pass) {
+        if (pass == 0) {
+            orc_start_vector(D, seed, w);
+            orc_spmv(D, row_ptr, col, val, diag, J, U, w, V);
+            if (sqrt(dot(D, V, V)) == 0.0) orc_start_vector(D, seed, V);
+        } else {
+            orc_start_vector_at(D, seed, (uint64_t)pass * (uint64_t)D, V);
+        }
+        double nrm = orc_cgs2(D, F, nf, NULL, 0, V, h);
+        if (nrm <= 1e-10) { /* the frozen vectors span the whole space */
+            rc = ORC_OK;
+            last_n = 0;
+            break;
+        }
+        for (int64_t i = 0; i < D; ++i) V[i] /= nrm;
+        beta[0] = 0.0;
+        int done = 0, nstep = 0;
+        for (int j = 0; j < max_iter && !done; ++j) {
+            double *vj = V + (size_t)D * j;
+            orc_spmv(D, row_ptr, col, val, diag, J, U, vj, w);
+            alpha[j] = dot(D, vj, w);
+            /* against the frozen vectors and the pass's basis together */
+            const double b = orc_cgs2(D, F, nf, V, j + 1, w, h);
+            beta[j + 1] = b;
+            nstep = j + 1;
+            ++total;
+            double anorm = fabs(alpha[0]);
+            for (int t = 0; t < nstep; ++t) anorm = fmax(anorm, fabs(alpha[t]) + beta[t + 1]);
+            const int exhausted = (b <= 1e-12 * anorm) || nstep + nf >= D;
+            if (exhausted || nstep == max_iter || (nstep >= 3 && nstep % 5 == 0)) {
+                free(z);
+                z = (double *)malloc(sizeof(double) * (size_t)nstep * nstep);
+                for (int t = 0; t < nstep; ++t) td[t] = alpha[t];
+                if (!z || orc_tridiag_eig(nstep, td, beta + 1, z) != ORC_OK) {
+                    rc = ORC_ENOCONV;
+                    goto out;
+                }
+                const double tnorm = fmax(fabs(td[0]), fabs(td[nstep - 1]));
+                int nl = 0;
+                for (int t = 0; t < nstep && nl < k; ++t) {
+                    if (nl > 0 && fabs(td[t] - td[lev[nl - 1]]) <= 1e-10 * tnorm) continue;
+                    lev[nl++] = t;
+                }
+                free(mv);
+                mv = (double *)malloc(sizeof(double) * (size_t)(nf + nl + 1));
+                for (int q = 0; q < nf; ++q) mv[q] = fth[q];
+                for (int q = 0; q < nl; ++q) mv[nf + q] = td[lev[q]];
+                qsort(mv, (size_t)(nf + nl), sizeof(double), cmp_double);
+                const double kappa = nf + nl >= k ? mv[k - 1] : INFINITY;
+                const double marg = 1e-10 * tnorm;
+                int conv = nf + nl >= k || exhausted; /* every Ritz value of T is analysed */
+                for (int q = 0; q < nl; ++q) {
+                    const double bnd = fabs(b * z[(nstep - 1) + (size_t)nstep * lev[q]]);
+                    const int below = td[lev[q]] <= kappa + marg;
+                    if ((below || q == 0) && bnd >= tol * fmax(eps23, fabs(td[lev[q]])) && !exhausted) conv = 0;
+                }
+                if (!conv && nstep < max_iter) goto next_step;
+                /* the pass's levels at or below kappa */
+                int nnew = 0;
+                for (int q = 0; q < nl; ++q)
+                    if (td[lev[q]] <= kappa + marg) last[nnew < k ? nnew : k] = lev[q], ++nnew;
+                if (nnew > k) nnew = k;
+                int nkeep = 0;
+                for (int q = 0; q < nf; ++q) nkeep += fth[q] <= kappa + marg;
+                if (nnew == 0 || nkeep + nnew > fcap || nkeep + nnew >= D || nstep == max_iter) {
+                    /* the solve ends: the last pass's levels at or below kappa (and its lowest) join the RR */
+                    last_n = nstep;
+                    nlast = 0;
+                    for (int q = 0; q < nl && nlast < k; ++q)
+                        if (q == 0 || td[lev[q]] <= kappa + marg) last[nlast++] = lev[q];
+                    rc = (nstep == max_iter && !conv) || (nnew > 0 && nkeep + nnew > fcap) ? ORC_ENOCONV : ORC_OK;
+                    done = 2;
+                    break;
+                }
+                /* frozen values above kappa are no longer among the lowest k: their vectors are dropped (a later
+                 * pass may meet them again, above kappa) */
+                {
+                    int keep = 0;
+                    for (int q = 0; q < nf; ++q) {
+                        if (fth[q] > kappa + marg) continue;
+                        if (keep != q) {
+                            memcpy(F + (size_t)D * keep, F + (size_t)D * q, sizeof(double) * (size_t)D);
+                            fth[keep] = fth[q];
+                        }
+                        ++keep;
+                    }
+                    nf = keep;
+                }
+                /* freeze: Ritz vectors V z, orthonormalised against the frozen set */
+                for (int q = 0; q < nnew; ++q) {
+                    double *y = F + (size_t)D * nf;
+                    const double *zc = z + (size_t)nstep * last[q];
+#pragma omp parallel for schedule(static)
+                    for (int64_t i = 0; i < D; ++i) {
+                        double sacc = 0.0;
+                        for (int t = 0; t < nstep; ++t) sacc += V[(size_t)D * t + i] * zc[t];
+                        y[i] = sacc;
+                    }
+                    const double yn = orc_cgs2(D, F, nf, NULL, 0, y, h);
+                    for (int64_t i = 0; i < D; ++i) y[i] /= yn;
+                    fth[nf++] = td[last[q]];
+                }
+                done = 1;
+                break;
+            }
+        next_step:
+            if (j + 1 < max_iter) {
+                double *vn = V + (size_t)D * (j + 1);
+                for (int64_t i = 0; i < D; ++i) vn[i] = w[i] / b;
+            }
+        }
+        if (done == 2) break;
+        if (!done) {
+            rc = ORC_ENOCONV;
+            last_n = 0;
+            break;
+        }
+    }
+    {
+        /* Rayleigh-Ritz over the frozen vectors and the last pass's Ritz vectors */
+        const int m = nf + (last_n > 0 ? nlast : 0);
+        double *Y = (double *)malloc(sizeof(double) * (size_t)D * (size_t)(m > 0 ? m : 1));
+        double *HY = (double *)malloc(sizeof(double) * (size_t)D * (size_t)(m > 0 ? m : 1));
+        double *A = (double *)malloc(sizeof(double) * (size_t)(m > 0 ? m * m : 1));
+        double *S = (double *)malloc(sizeof(double) * (size_t)(m > 0 ? m * m : 1));
+        double *th = (double *)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+        if (!Y || !HY || !A || !S || !th) {
+            free(Y), free(HY), free(A), free(S), free(th);
+            rc = ORC_ENOMEM;
+            goto out;
+        }
+        memcpy(Y, F, sizeof(double) * (size_t)D * (size_t)nf);
+        int mm = nf;
+        for (int q = 0; last_n > 0 && q < nlast; ++q) {
+            double *y = Y + (size_t)D * mm;
+            const double *zc = z + (size_t)last_n * last[q];
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < D; ++i) {
+                double sacc = 0.0;
+                for (int t = 0; t < last_n; ++t) sacc += V[(size_t)D * t + i] * zc[t];
+                y[i] = sacc;
+            }
+            const double yn = orc_cgs2(D, Y, mm, NULL, 0, y, h);
+            if (yn <= 1e-8) continue;
+            for (int64_t i = 0; i < D; ++i) y[i] /= yn;
+            ++mm;
+        }
+        for (int c = 0; c < mm; ++c) orc_spmv(D, row_ptr, col, val, diag, J, U, Y + (size_t)D * c, HY + (size_t)D * c);
+        for (int a = 0; a < mm; ++a)
+            for (int c = a; c < mm; ++c) A[a * mm + c] = A[c * mm + a] = dot(D, Y + (size_t)D * a, HY + (size_t)D * c);
+        orc_jacobi(mm, A, th, S);
+        for (int q = 0; q < k; ++q) {
+            evals[q] = NAN;
+            if (resid) resid[q] = NAN;
+        }
+        for (int q = 0; q < k && q < mm; ++q) {
+            evals[q] = th[q];
+            if (resid) {
+#pragma omp parallel for schedule(static)
+                for (int64_t i = 0; i < D; ++i) {
+                    double sacc = 0.0;
+                    for (int c = 0; c < mm; ++c) sacc += (HY[(size_t)D * c + i] - th[q] * Y[(size_t)D * c + i]) * S[c * mm + q];
+                    w[i] = sacc;
+                }
+                resid[q] = sqrt(dot(D, w, w));
+            }
+        }
+        if (mm < k && rc == ORC_OK && nf + last_n < D) rc = ORC_ENOCONV;
+        free(Y), free(HY), free(A), free(S), free(th);
+    }
+    *iters = total;
+out:
+    free(F);
+    free(V);
+    free(w);
+    free(alpha);
+    free(beta);
+    free(h);
+    free(td);
+    free(fth);
+    free(z);
+    free(mv);
+    free(lev);
+    free(last);
     return rc;
 }
 

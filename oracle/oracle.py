@@ -9,7 +9,8 @@ Function-to-reference map (reference file:line):
     square_neighbours    src/neighbours.cpp:40-71 (shadowing local at :45 fixed; Lx x Ly extension)
     basis                src/hamiltonian.cpp:60-123, 143-149
     hamiltonian          src/hamiltonian.cpp:170-211 (intended operator, exact target search)
-    lanczos              src/operator.cpp:22-33 (+ Spectra start vector / convergence rule)
+    lanczos              src/operator.cpp:22-33 (+ Spectra start vector / convergence rule), distinct levels
+    lanczos_mult         src/operator.cpp:22-33, eigenvalues with multiplicity as Spectra returns them
     gap_ratio            src/analysis.cpp:433-454
 """
 from __future__ import annotations
@@ -65,6 +66,11 @@ def lib():
         L.orc_lanczos.argtypes = [C.c_int64, _i64p, _i32p, _f64p, _f64p, C.c_double, C.c_double, C.c_int,
                                   C.c_double, C.c_int, C.c_uint64, C.c_double, _f64p, _f64p,
                                   C.POINTER(C.c_int)]
+        L.orc_lanczos_mult.restype = C.c_int
+        L.orc_lanczos_mult.argtypes = [C.c_int64, _i64p, _i32p, _f64p, _f64p, C.c_double, C.c_double, C.c_int,
+                                       C.c_double, C.c_int, C.c_uint64, _f64p, _f64p, C.POINTER(C.c_int)]
+        L.orc_start_vector_at.restype = None
+        L.orc_start_vector_at.argtypes = [C.c_int64, C.c_uint64, C.c_uint64, _f64p]
         L.orc_gap_ratio.restype = C.c_double
         L.orc_gap_ratio.argtypes = [C.c_int, _f64p, C.c_void_p]
         L.orc_spdm.restype = C.c_int
@@ -276,6 +282,22 @@ def lanczos(h: CSR, J: float, U: float, k: int, tol: float = 1e-10, max_iter: in
     return rc, ev, bd, it.value
 
 
+def lanczos_mult(h: CSR, J: float, U: float, k: int, tol: float = 1e-10, max_iter: int = 1000, seed: int = 0):
+    """(rc, evals, resid, steps): the lowest k eigenvalues of J T + U V with their multiplicity (orc_lanczos_mult)."""
+    ev = np.zeros(k)
+    rs = np.zeros(k)
+    it = C.c_int(0)
+    rc = lib().orc_lanczos_mult(h.D, h.row_ptr, _c32(h.col), _c64(h.val), h.diag, J, U, k, tol, max_iter, seed, ev, rs,
+                                C.byref(it))
+    return rc, ev, rs, it.value
+
+
+def start_vector_at(D: int, seed: int, off: int):
+    r = np.zeros(D)
+    lib().orc_start_vector_at(D, seed, off, r)
+    return r
+
+
 def spdm(states, phi):
     """<phi| b_i^+ b_j |phi> (m x m) for a real state phi in the order of `states` (D x m occupations)."""
     states = np.ascontiguousarray(states, np.uint8)
@@ -302,6 +324,18 @@ def coherence(rho) -> float:
 def gap_ratio(ev) -> float:
     ev = np.ascontiguousarray(ev, np.float64)
     return float(lib().orc_gap_ratio(ev.size, ev, None))
+
+
+def ref_gap_ratio(ev) -> float:
+    """Mean gap ratio by the reference's own Analysis::gap_ratios (src/analysis.cpp:433-454), compiled in
+    oracle/_ref, as src/analysis.cpp:317-318 averages it."""
+    ev = np.ascontiguousarray(ev, np.float64)
+    n = ev.size
+    if n < 3:
+        return 0.0
+    r = np.zeros(n - 2)
+    ref().ref_gap_ratios(ev, n, r)
+    return float(r.mean())
 
 
 def distinct_levels(ev, k: int, rel: float = 1e-9):

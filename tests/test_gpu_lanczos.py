@@ -1,7 +1,8 @@
-"""GPU Lanczos against the oracle (dense FP64 eigh of the oracle's H, the oracle's FP64 Lanczos),
-the reference's own Spectra solver run on the same H (oracle/_ref), and closed forms.
-Tolerances: FP64 path 1e-9 relative on every level; mixed path 1e-9 relative on E0 (the acceptance
-target) and 1e-5 relative on the other levels (FP32-stored Krylov vectors leave residuals of
+"""GPU Lanczos against the oracle (dense FP64 eigh of the oracle's H, the oracle's FP64 Lanczos with
+multiplicities), the reference's own Spectra solver run on the same H (oracle/_ref), and closed forms.
+Eigenvalues are compared with their multiplicity, as the reference's Op::IRLM_eigen returns them.
+Tolerances: FP64 path 1e-9 relative on every value; mixed path 1e-9 relative on E0 (the acceptance
+target) and 1e-5 relative on the other values (FP32-stored Krylov vectors leave residuals of
 ~eps32 ||H|| sqrt(steps); excited levels in near-degenerate clusters keep ~||r||^2/gap of that)."""
 MIXED_EXCITED = 1e-5
 import math
@@ -52,9 +53,9 @@ def test_lanczos_matches_dense(ctx, geom, n, J, U, k, mode):
     m, off, sites, b, h = build(ctx, geom, n)
     ho = O.hamiltonian(m, n, off, sites)
     if ho.D <= 7000:
-        want = O.distinct_levels(np.linalg.eigvalsh(ho.dense(J, U)), k)
+        want = np.linalg.eigvalsh(ho.dense(J, U))[:k]
     else:
-        rc, want, _, _ = O.lanczos(ho, J, U, k)
+        rc, want, _, _ = O.lanczos_mult(ho, J, U, k)
         assert rc == 0
     r = h.lanczos(J, U, k=k, prec=prec, **kw)
     assert r.status[0] == 0
@@ -77,7 +78,7 @@ def test_batched_equals_individual(ctx, mode):
     Js = np.array([0.02, 0.07, 0.15, 0.3, 0.6])
     r = h.lanczos(Js, 1.0, k=4, prec=prec, **kw)
     for i, J in enumerate(Js):
-        want = O.distinct_levels(np.linalg.eigvalsh(ho.dense(J, 1.0)), 4)
+        want = np.linalg.eigvalsh(ho.dense(J, 1.0))[:4]
         assert rel(r.evals[i, 0], want[0]) <= 1e-9
         assert np.all(rel(r.evals[i], want) <= (1e-9 if prec == bhx.PREC_FP64 else MIXED_EXCITED))
         single = h.lanczos(J, 1.0, k=4, prec=prec, **kw)
@@ -91,30 +92,36 @@ def test_closed_forms(ctx):
     m, off, sites, b, h = build(ctx, ("chain", 10, True), 10)
     r = h.lanczos(0.7, 0.0, k=2, prec=bhx.PREC_FP64)
     assert abs(r.evals[0, 0] - (-2 * 0.7 * 10)) <= 1e-9 * 14
-    # U = 0: second level = one boson moved to k = +-2 pi / m
+    # U = 0: the next two values = one boson moved to k = +2 pi / m or -2 pi / m (a degenerate pair)
     e1 = -2 * 0.7 * 9 - 2 * 0.7 * math.cos(2 * math.pi / 10)
     assert abs(r.evals[0, 1] - e1) <= 1e-8
-    # J = 0, unit filling: levels 0, U, 2U, 3U (doubly occupied sites)
+    r = h.lanczos(0.7, 0.0, k=3, prec=bhx.PREC_FP64)
+    assert np.allclose(r.evals[0, 1:], [e1, e1], atol=1e-8)
+    # J = 0, unit filling: 0 once (the Mott state), then U with multiplicity m (m - 1) (one doubly occupied
+    # and one empty site)
     r = h.lanczos(0.0, 1.0, k=4, prec=bhx.PREC_MIXED)
+    assert np.allclose(r.evals[0], [0.0, 1.0, 1.0, 1.0], atol=1e-9)
+    r = h.lanczos(0.0, 1.0, k=4, prec=bhx.PREC_MIXED, distinct=1)
     assert np.allclose(r.evals[0], [0.0, 1.0, 2.0, 3.0], atol=1e-9)
     h.close()
     b.close()
 
 
 @pytest.mark.skipif(not O.ref_available(), reason="reference build oracle/_ref not present")
-@pytest.mark.parametrize("J", [1.0, 0.1])
-def test_matches_reference_spectra(ctx, J):
-    """The reference's Op::IRLM_eigen (Spectra GenEigs, nev=10) on the same Hamiltonian."""
+@pytest.mark.parametrize("J,k", [(1.0, 10), (0.1, 10), (1.0, 20), (0.1, 25)])
+def test_matches_reference_spectra(ctx, J, k):
+    """The reference's Op::IRLM_eigen (Spectra GenEigs, nev=k, ncv=2k+1) on the same Hamiltonian: the same k
+    values, degenerate pairs included, and the same gap ratio by the reference's own Analysis::gap_ratios."""
     m, n = 8, 8
     _, off, sites, b, h = build(ctx, ("chain", m, True), n)
     ho = O.hamiltonian(m, n, off, sites)
-    ir = np.zeros(10)
-    assert O.ref().ref_irlm_eigen(ho.D, ho.row_ptr, ho.col, ho.val, ho.diag, J, 1.0, 10, ir) == 0
-    want = O.distinct_levels(ir, 4)
+    ir = np.zeros(k)
+    assert O.ref().ref_irlm_eigen(ho.D, ho.row_ptr, ho.col, ho.val, ho.diag, J, 1.0, k, ir) == 0
     for prec in (bhx.PREC_FP64, bhx.PREC_MIXED):
-        r = h.lanczos(J, 1.0, k=4, prec=prec)
-        assert rel(r.evals[0, 0], want[0]) <= 1e-9
-        assert np.all(rel(r.evals[0], want) <= (1e-9 if prec == bhx.PREC_FP64 else MIXED_EXCITED))
+        r = h.lanczos(J, 1.0, k=k, prec=prec)
+        assert r.status[0] == 0
+        assert np.all(rel(r.evals[0], ir) <= 1e-9), (r.evals[0], ir)
+        assert O.ref_gap_ratio(r.evals[0]) == pytest.approx(O.ref_gap_ratio(ir), abs=1e-6)
     h.close()
     b.close()
 
@@ -175,7 +182,7 @@ def test_thick_restart_small_basis(ctx, mode):
     prec, kw = MODES[mode]
     m, off, sites, b, h = build(ctx, ("chain", 8, True), 8)
     ho = O.hamiltonian(m, 8, off, sites)
-    want = O.distinct_levels(np.linalg.eigvalsh(ho.dense(0.3, 1.0)), 4)
+    want = np.linalg.eigvalsh(ho.dense(0.3, 1.0))[:4]
     r = h.lanczos(0.3, 1.0, k=4, prec=prec, ncv=24, **kw)
     print("restarted", r.iters, r.used_fp64, r.evals[0], flush=True)
     assert r.status[0] == 0 and r.iters[0] > 24
@@ -192,7 +199,7 @@ def test_small_hopping_needs_restarts(ctx, mode):
     prec, kw = MODES[mode]
     m, off, sites, b, h = build(ctx, ("chain", 8, True), 7)
     ho = O.hamiltonian(m, 7, off, sites)
-    want = O.distinct_levels(np.linalg.eigvalsh(ho.dense(0.004, 1.0)), 4)
+    want = np.linalg.eigvalsh(ho.dense(0.004, 1.0))[:4]
     r = h.lanczos(0.004, 1.0, k=4, prec=prec, ncv=32, **kw)
     print("small J", r.iters, r.used_fp64, r.evals[0], want, flush=True)
     assert r.status[0] == 0 and r.iters[0] > 32
@@ -217,5 +224,90 @@ def test_speculative_analyses_full_size(ctx):
         r64 = h.lanczos(Jv, 1.0, k=4, prec=bhx.PREC_FP64)
         e = rel(r.evals[i], r64.evals[0])
         assert e[0] <= 1e-9 and np.all(e <= (1e-9 if r.used_fp64[i] else MIXED_EXCITED)), (Jv, r.evals[i], r64.evals[0])
+    h.close()
+    b.close()
+
+
+@pytest.mark.parametrize("prec", [bhx.PREC_FP64, bhx.PREC_MIXED])
+def test_distinct_levels_option(ctx, prec):
+    """distinct=1 keeps the one-pass solve: the lowest k distinct levels."""
+    m, off, sites, b, h = build(ctx, ("chain", 8, True), 8)
+    ho = O.hamiltonian(m, 8, off, sites)
+    want = O.distinct_levels(np.linalg.eigvalsh(ho.dense(1.0, 1.0)), 6)
+    r = h.lanczos(1.0, 1.0, k=6, prec=prec, distinct=1)
+    assert r.status[0] == 0 and np.all(rel(r.evals[0], want) <= 1e-9)
+    h.close()
+    b.close()
+
+
+@pytest.mark.parametrize("geom,n,J,k", [(("square", 3, 3, True), 7, 0.15, 6), (("chain", 9, True), 8, 0.0, 5),
+                                        (("chain", 9, True), 8, 0.4, 6)])
+def test_multiplicities_of_symmetric_lattices(ctx, geom, n, J, k):
+    """The 3x3 torus has levels of multiplicity up to 4 and more: every copy among the lowest k is returned,
+    in the FP64 path and in the FP32-stored path with selective orthogonalisation (mixed_so)."""
+    print(f"multiplicities {geom} n={n} J={J} k={k}", flush=True)
+    m, off, sites, b, h = build(ctx, geom, n)
+    ho = O.hamiltonian(m, n, off, sites)
+    want = np.linalg.eigvalsh(ho.dense(J, 1.0))[:k]
+    for mode in ("fp64", "mixed_so"):
+        prec, kw = MODES[mode]
+        r = h.lanczos(J, 1.0, k=k, prec=prec, **kw)
+        assert r.status[0] == 0
+        tol = 1e-9 if r.used_fp64[0] else MIXED_EXCITED
+        assert rel(r.evals[0, 0], want[0]) <= 1e-9 and np.all(np.abs(r.evals[0] - want) <= tol * np.maximum(np.abs(want), 1.0)), (
+            mode, r.evals[0], want)
+    h.close()
+    b.close()
+
+
+def test_dense_eigen_matches_numpy(ctx):
+    """bh_dense_eigen (rocSOLVER dsyevd on the device) at the size of the reference's small exact runs
+    (chain m=8, n=8: D = 6435): every eigenvalue against LAPACK, orthonormal eigenvectors with small residuals."""
+    ho = O.hamiltonian(8, 8, *O.chain_neighbours(8, True))
+    H = ho.dense(1.0, 1.0)
+    rows = np.repeat(np.arange(ho.D), np.diff(ho.row_ptr))
+    # the CSR of J T + U V: off-diagonal entries then the diagonal
+    rp = np.zeros(ho.D + 1, np.int64)
+    np.add.at(rp, rows + 1, 1)
+    rp[1:] += 1
+    rp = np.cumsum(rp)
+    col = np.zeros(rp[-1], np.int32)
+    val = np.zeros(rp[-1])
+    for r in range(ho.D):
+        a, e = ho.row_ptr[r], ho.row_ptr[r + 1]
+        q = rp[r]
+        col[q:q + e - a] = ho.col[a:e]
+        val[q:q + e - a] = ho.val[a:e]
+        col[q + e - a] = r
+        val[q + e - a] = ho.diag[r]
+    w, V = ctx.dense_eigen(rp, col, val, vectors=True)
+    want = np.linalg.eigvalsh(H)
+    assert np.allclose(w, want, rtol=0, atol=1e-11)
+    for c in (0, 1, 2, 100, ho.D - 1):
+        assert np.linalg.norm(H @ V[:, c] - w[c] * V[:, c]) < 1e-10
+    assert abs(V[:, 0] @ V[:, 1]) < 1e-12
+
+
+def test_tridiag_folm(ctx):
+    """bh_lanczos_tridiag: T is the projection V^T H V of H on the returned orthonormal Lanczos basis, and an
+    invariant subspace stops the recurrence (n = 1: H = J T on the ring, a space of m states)."""
+    m, off, sites, b, h = build(ctx, ("chain", 8, True), 6)
+    ho = O.hamiltonian(m, 6, off, sites)
+    H = ho.dense(0.3, 1.0)
+    a, bb, V = h.tridiag(0.3, 1.0, 40, basis=True)
+    assert a.size == 40 and bb[0] == 0.0
+    assert np.allclose(V @ V.T, np.eye(40), atol=1e-12)
+    T = V @ H @ V.T
+    Tw = np.diag(a) + np.diag(bb[1:], 1) + np.diag(bb[1:], -1)
+    assert np.allclose(T, Tw, atol=1e-10)
+    h.close()
+    b.close()
+    m, off, sites, b, h = build(ctx, ("chain", 8, True), 1)
+    a, bb, _ = h.tridiag(1.0, 0.0, 30)
+    # the single-particle ring has 5 distinct levels: the recurrence stops at an invariant space of size 5
+    assert a.size == 5
+    want = np.unique(np.round(-2 * np.cos(2 * np.pi * np.arange(8) / 8), 12))
+    Tw = np.diag(a) + np.diag(bb[1:], 1) + np.diag(bb[1:], -1)
+    assert np.allclose(np.linalg.eigvalsh(Tw), want, atol=1e-10)
     h.close()
     b.close()

@@ -1,5 +1,7 @@
 """GPU serving loop: continuous batching over sector engines.  Results against individual FP64 solves
-(E0 within 1e-9 relative, levels include -mu n), alone-vs-mix equality, budget rejection."""
+(E0 within 1e-9 relative, levels include -mu n, values with their multiplicity), alone-vs-mix equality,
+budget rejection.  Sectors of at most 16384 states are served in FP64 with full reorthogonalisation; the
+larger ones (MIXED_SECTORS) run FP32-stored lanes with FP64 reductions and an FP64 rerun on rejection."""
 import numpy as np
 import pytest
 
@@ -70,10 +72,64 @@ def test_alone_equals_in_mix(ctx):
 def test_diagonal_request_and_budget(ctx):
     with bhx.Server(ctx, lanes=4, hbm_budget=64 << 20) as srv:
         r = srv.wait(srv.submit("chain", 6, 0.0, 1.0, 0.5, m=6))
-        assert r.status == 0 and np.allclose(r.evals[:4], np.array([0.0, 1.0, 2.0, 3.0]) - 0.5 * 6)
+        # J = 0, unit filling: the Mott state, then U with multiplicity m (m - 1)
+        assert r.status == 0 and np.allclose(r.evals[:4], np.array([0.0, 1.0, 1.0, 1.0]) - 0.5 * 6)
         big = srv.submit("chain", 13, 0.1, 1.0, 0.0, m=13)  # D = C(24, 12): the CSR alone exceeds 64 MiB
         small = srv.submit("chain", 6, 0.1, 1.0, 0.0, m=6)
         rs = srv.wait(small)
         rb = srv.wait(big, 120.0)
         assert rs.status == 0
         assert rb.status == bhx.EBUDGET
+
+
+# sectors above the full-reorthogonalisation size: chain m=10 n=10 (D = 92,378), 3x3 n=11 (D = 75,582),
+# chain m=12 n=11 (D = 705,432)
+MIXED_SECTORS = [("chain", 10, 0, 0, 10), ("square", 0, 3, 3, 11), ("chain", 12, 0, 0, 11)]
+
+
+def test_server_mixed_lanes(ctx):
+    """Requests on sectors served by FP32-stored lanes: every result's E0 within 1e-9 of an individual FP64 solve,
+    the other three values within 1e-5 (1e-9 for a result that was rerun in FP64), most results from mixed lanes."""
+    rng = np.random.default_rng(11)
+    reqs = []
+    with bhx.Server(ctx, lanes=8) as srv:
+        for i in range(18):
+            g = MIXED_SECTORS[i % len(MIXED_SECTORS)]
+            J = float(10 ** rng.uniform(-1.5, -0.3))
+            mu = float(rng.uniform(0, 3))
+            reqs.append((srv.submit(g[0], g[4], J, 1.0, mu, m=g[1], Lx=g[2], Ly=g[3]), g, J, mu))
+        assert srv.drain(600.0)
+        res = [(srv.poll(rid), g, J, mu) for rid, g, J, mu in reqs]
+    mixed = 0
+    for r, g, J, mu in res:
+        print("served", g, J, r.status, r.used_fp64, list(r.evals[:4]), flush=True)
+        assert r is not None and r.status == 0
+        want = fp64_levels(ctx, g[0], g[1], g[2], g[3], g[4], J, 1.0) - mu * g[4]
+        got = np.array(r.evals[:4])
+        assert abs(got[0] - want[0]) <= 1e-9 * abs(want[0])
+        tol = 1e-9 if r.used_fp64 else 1e-5
+        assert np.all(np.abs(got - want) <= tol * np.maximum(np.abs(want), 1.0)), (g, J, got, want)
+        mixed += r.used_fp64 == 0
+    assert mixed >= len(res) // 2
+
+
+def test_alone_equals_in_mix_mixed_lane(ctx):
+    """A request on an FP32-stored lane gives the same values alone and in a mix of other sectors' requests
+    (within 1e-10 relative on E0 and 1e-8 on the others: the engine's width follows the queue, and the
+    operator variant and the partial sums of the reductions follow the width)."""
+    with bhx.Server(ctx, lanes=8) as srv:
+        alone = srv.wait(srv.submit("chain", 10, 0.2, 1.0, 0.7, m=10), 300.0)
+        rng = np.random.default_rng(5)
+        ids, target = [], None
+        for i in range(16):
+            g = (MIXED_SECTORS + MIX)[i % (len(MIXED_SECTORS) + len(MIX))]
+            ids.append(srv.submit(g[0], g[4], float(rng.uniform(0.02, 0.5)), 1.0, 0.0, m=g[1], Lx=g[2], Ly=g[3]))
+            if i == 5:
+                target = srv.submit("chain", 10, 0.2, 1.0, 0.7, m=10)
+        mixed = srv.wait(target, 300.0)
+        assert srv.drain(600.0)
+    assert alone.status == 0 and mixed.status == 0
+    assert alone.used_fp64 == 0 and mixed.used_fp64 == 0
+    a, b = np.array(alone.evals[:4]), np.array(mixed.evals[:4])
+    assert abs(a[0] - b[0]) <= 1e-10 * abs(a[0])
+    assert np.all(np.abs(a - b) <= 1e-8 * np.maximum(np.abs(a), 1.0)), (a, b)

@@ -108,7 +108,7 @@ def test_lanczos_k1_variants(ctx, prec):
     b = ctx.basis(10, 9, bhx.ORDER_LEX)
     h = b.hamiltonian(off, sites)
     J = np.array([0.02, 0.05, 0.1, 0.2, 0.3, 0.5, 0.7, 1.0])
-    rc, ref, _, _ = O.lanczos(O.hamiltonian(10, 9, off, sites), J[3], 1.0, 4)
+    rc, ref, _, _ = O.lanczos_mult(O.hamiltonian(10, 9, off, sites), J[3], 1.0, 4)
     assert rc == 0
     opts = {"fullre_d": -1} if prec == bhx.PREC_MIXED else {}  # D = 48,620: keep the selective path
     res = {}

@@ -52,27 +52,34 @@ def dense_levels(m, n, J, U, mu, k):
     off, sites = O.chain_neighbours(m, True)
     h = O.hamiltonian(m, n, off, sites)
     ev = np.linalg.eigvalsh(h.dense(J, U)) - mu * n
-    return ev, O.distinct_levels(ev, k), h
+    return ev, ev[:k], h
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("m,n,J,U,mu", [(6, 6, 0.3, 1.0, 0.5), (5, 4, 0.1, 1.0, 0.0), (4, 6, 0.7, 2.0, 1.0)])
-def test_host_api_matches_oracle(m, n, J, U, mu):
-    print(f"host api m={m} n={n} J={J} U={U} mu={mu}", flush=True)
-    p = run([EXAMPLE, str(m), str(n), str(J), str(U), str(mu), "4"])
+@pytest.mark.parametrize("m,n,J,U,mu,k", [(6, 6, 0.3, 1.0, 0.5, 4), (5, 4, 0.1, 1.0, 0.0, 4), (4, 6, 0.7, 2.0, 1.0, 4),
+                                          (8, 8, 1.0, 1.0, 0.5, 20)])
+def test_host_api_matches_oracle(m, n, J, U, mu, k):
+    """The reference's call sequence through the C++ host surface: Op::IRLM_eigen(H, k) returns k values with
+    their multiplicity (k = 20 at the reference's small exact size, chain m=8 n=8, D = 6435), Op::exact_eigen
+    solves the whole sector on the device (all D values, every returned vector an eigenvector)."""
+    print(f"host api m={m} n={n} J={J} U={U} mu={mu} k={k}", flush=True)
+    p = run([EXAMPLE, str(m), str(n), str(J), str(U), str(mu), str(k)])
     assert p.returncode == 0, p.stderr
     out = json.loads(p.stdout)
-    ev, lv, h = dense_levels(m, n, J, U, mu, 4)
+    ev, lv, h = dense_levels(m, n, J, U, mu, k)
     st, tags, _ = O.basis(m, n, O.ORDER_REFERENCE)
     assert out["D"] == h.D and out["nnz"] == h.nnz + h.D
     assert np.array_equal(out["tags"], tags[:4]) and np.array_equal(out["state0"], st[0])
+    assert len(out["lanczos"]) == k
     assert np.allclose(out["lanczos"], lv, rtol=1e-9, atol=1e-9)
+    assert len(out["exact"]) == min(12, h.D)
     assert np.allclose(out["exact"], ev[:len(out["exact"])], rtol=1e-9, atol=1e-9)
+    assert out["exact_residual"] < 1e-9 * max(1.0, abs(ev[0]))
     assert out["vec_rows"] == h.D
     # ground-state density matrix (Analysis::SPDM divides by the number of eigenvectors, as the reference)
     w, v = np.linalg.eigh(h.dense(J, U))
     rho = O.spdm(st, v[:, 0])
-    assert out["spdm_trace"] == pytest.approx(n / 4, rel=1e-9)
+    assert out["spdm_trace"] == pytest.approx(n / k, rel=1e-9)
     assert out["condensate_fraction"] == pytest.approx(O.condensate_fraction(rho), abs=1e-6)
     assert out["coherence"] == pytest.approx(O.coherence(rho), abs=1e-6)
 
@@ -140,7 +147,7 @@ def test_host_max_bosons(m, nmin, nmax, J, U, mu, k):
             ev = np.linalg.eigvalsh(h.dense(0.0, U))
         else:
             ev = np.full(h.D, -mu * N)
-        want += list(O.distinct_levels(ev, k))
+        want += list(np.sort(ev)[:k])
     want = np.sort(want)[:k]
     assert out["D"] == D and out["blocks"] == max(nmax, lo) - lo + 1
     assert np.allclose(out["lanczos"], want, rtol=1e-9, atol=1e-9)
@@ -206,8 +213,9 @@ def test_cli_phase_file(tmp_path, fixed):
     off, sites = O.chain_neighbours(m, True)
     h = O.hamiltonian(m, n, off, sites)
     st, _, _ = O.basis(m, n, O.ORDER_REFERENCE)
+    spectra = []
     for row in rows:
-        p1, p2, g = row[0], row[1], row[2]
+        p1, p2 = row[0], row[1]
         # H = H_fixed + H1 p1 + H2 p2 with the reference's wiring (src/analysis.cpp:247-256)
         if fixed == "J":
             Jv, Uv, muv = J, p1, p2
@@ -216,9 +224,19 @@ def test_cli_phase_file(tmp_path, fixed):
         else:
             Jv, Uv, muv = p1, p2, mu
         w, v = np.linalg.eigh(h.dense(Jv, Uv))
-        lv = O.distinct_levels(w - muv * n, 16)
-        assert g == pytest.approx(ref_gap_ratio(lv), abs=1e-6)
+        spectra.append((w - muv * n, v[:, 0]))
+    # the reference's count: nb_eigen = 20 values with multiplicity, 5 more while the variance of the mean gap
+    # ratios over the grid is at most 1e-8 of their mean (src/analysis.cpp:277, 361-381); its solver allows
+    # 2 nb_eigen + 1 <= D (here D = 126), the library at most 64 values
+    nb = 20
+    while True:
+        gr = np.array([ref_gap_ratio(w[:nb]) for w, _ in spectra])
+        if gr.var() > 1e-8 * gr.mean() or nb + 5 > 64 or 2 * (nb + 5) + 1 > h.D:
+            break
+        nb += 5
+    for row, g_want, (w, v0) in zip(rows, gr, spectra):
+        assert row[2] == pytest.approx(g_want, abs=1e-6)
         # condensate fraction and coherence of the ground state (src/analysis.cpp:327-336)
-        rho = O.spdm(st, v[:, 0])
+        rho = O.spdm(st, v0)
         assert row[3] == pytest.approx(O.condensate_fraction(rho), abs=1e-6)
         assert row[4] == pytest.approx(O.coherence(rho), abs=1e-6)

@@ -253,8 +253,8 @@ def test_reference_spectra_and_eigen_on_oracle_h(m, n, J, U):
     ir = np.zeros(10)
     assert R.ref_irlm_eigen(h.D, h.row_ptr, h.col, h.val, h.diag, J, U, 10, ir) == 0
     assert np.allclose(ir, w[:10], atol=1e-9)  # Spectra returns levels with multiplicity
-    rc, ev, _, _ = O.lanczos(h, J, U, 4)
-    assert np.allclose(ev, O.distinct_levels(ir, 4), atol=1e-9)
+    rc, ev, rs, _ = O.lanczos_mult(h, J, U, 10)
+    assert rc == 0 and np.allclose(ev, ir, rtol=0, atol=1e-9) and np.all(rs < 1e-8)
 
 
 def test_gap_ratio():
@@ -454,3 +454,32 @@ def test_mf_start_values_splitmix64():
     z = O.mf_psi0(0, [0, 1])
     assert int(z[0] * 2.0 ** 53) == 0xE220A8397B1DCDAF >> 11
     assert int(z[1] * 2.0 ** 53) == 0x6E789E6AA1B965F4 >> 11
+
+
+@pytest.mark.parametrize("m,n,J,U,k", [(8, 8, 1.0, 1.0, 20), (8, 8, 0.1, 1.0, 25), (6, 6, 0.3, 1.0, 12),
+                                       (7, 5, 0.0, 1.0, 8), (8, 8, 0.5, 0.0, 15)])
+def test_oracle_lanczos_mult_matches_dense(m, n, J, U, k):
+    """The oracle's eigenvalues with multiplicity against LAPACK on the same H: degenerate pairs of the periodic
+    chain, the atomic limit (J = 0: U with multiplicity m (m - 1) at unit filling) and free bosons (U = 0)."""
+    h = O.hamiltonian(m, n, *O.chain_neighbours(m))
+    w = np.linalg.eigvalsh(h.dense(J, U))
+    rc, ev, rs, _ = O.lanczos_mult(h, J, U, k)
+    assert rc == 0
+    assert np.allclose(ev, w[:k], rtol=0, atol=1e-9 * max(1.0, abs(w[0])))
+
+
+def test_oracle_lanczos_mult_free_bosons_closed_form():
+    """U = 0 on the periodic chain: many-body values are sums of single-particle energies -2J cos(2 pi q / m);
+    the lowest k with their multiplicity, at a size dense eigh does not reach (m = 10, n = 10: D = 92378)."""
+    m, n, J, k = 10, 10, 0.5, 6
+    h = O.hamiltonian(m, n, *O.chain_neighbours(m))
+    eps = -2 * J * np.cos(2 * np.pi * np.arange(m) / m)
+    want = np.sort(free_boson_spectrum(eps, n))[:k]
+    rc, ev, rs, _ = O.lanczos_mult(h, J, 0.0, k)
+    assert rc == 0 and np.allclose(ev, want, rtol=0, atol=1e-9)
+
+
+def test_start_vector_continuation():
+    """The pass start vectors continue the Park-Miller sequence of the first one."""
+    r = O.start_vector(40, 7)
+    assert np.array_equal(O.start_vector_at(25, 7, 15), r[15:])
