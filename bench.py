@@ -2,7 +2,8 @@
 
 Workload (the single large solve of the benchmark configurations): periodic chain, m = 12 sites,
 n = 12 bosons (D = C(23, 11) = 1,352,078), H = -J sum (b_i^+ b_j + h.c.) + U/2 sum n(n-1) - mu N at
-J/U = 0.1, U = 1, mu = 0.5, lowest k = 4 eigenvalues, seeded Park-Miller start vector.
+J/U = 0.1, U = 1, mu = 0.5, lowest k = 4 eigenvalues counted with their multiplicity (as the reference's
+Spectra solve returns them), seeded Park-Miller start vector.
 
 A step is one batched mixed-precision eigensolve: B = 16 solves of the sector (J/U = 0.1 and 15
 neighbouring sweep points 0.1 * (1 + 0.01 b)) sharing one CSR pass per Lanczos step, on the resident
@@ -11,11 +12,14 @@ residuals).  value = eigensolves per second.  The solver layout is the lexicogra
 (same spectrum as the reference order; the CSR is rebuilt in it, 3 ms).
 
 Beside it: the plain FP64 GPU Lanczos (FP64 CSR and FP64 vectors, one solve at a time, its fastest
-engine), the error of the mixed E0 against it, the dominant kernel's HBM roofline (HIP events on the
-library stream over the timed region), and the oracle's FP64 CPU Lanczos on the host cores.
+engine), the error of every mixed value against the FP64 solve of its J point, the dominant kernel's HBM
+roofline (HIP events on the library stream over the timed region), and the reference's own CPU solver
+(Op::IRLM_eigen = Spectra GenEigs, compiled from the reference's sources into oracle/_ref) on the host
+cores, one solve per OpenMP thread as the reference's grid loop runs them.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -26,7 +30,10 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 import numpy as np  # noqa: E402
 
-import bhx  # noqa: E402
+if __name__ == "__main__" and len(sys.argv) > 2 and sys.argv[1] == "--cpu-child":
+    bhx = None  # the CPU child of a baseline leg never touches the GPU library
+else:
+    import bhx  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
@@ -35,23 +42,124 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(m, n, J, U, k, gpu_steps, sample_steps=300):
-    """The oracle's FP64 CPU Lanczos (full reorthogonalisation, OpenMP over this host's cores) on the
-    same sector: the first `sample_steps` steps are timed and the full solve is extrapolated linearly
-    to the step count of the FP64 GPU solve (linear is generous to the CPU: its per-step
-    reorthogonalisation grows with the step index)."""
+def cpu_model():
+    """CPU model and logical CPU count of this host (lscpu / os.cpu_count)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def run_cpu_child(mode, args, cap_s, threads, setup_cap_s=600.0):
+    """Runs `bench.py --cpu-child MODE ARGS` (the reference's CPU solver, no GPU) with OMP_NUM_THREADS = threads;
+    the child's setup (building the sector on the host) is not timed, then its solves get at most cap_s seconds
+    of wall time.  Returns (records of the solves finished within the cap, elapsed seconds)."""
+    import selectors
+    import subprocess
+
+    env = dict(os.environ)
+    env["OMP_NUM_THREADS"] = str(threads)
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--cpu-child", mode] + [str(a) for a in args]
+    t_launch = time.perf_counter()
+    proc = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True, env=env)
+    sel = selectors.DefaultSelector()
+    sel.register(proc.stdout, selectors.EVENT_READ)
+    recs, start = [], None
+    while True:
+        deadline = start + cap_s if start is not None else t_launch + setup_cap_s
+        left = deadline - time.perf_counter()
+        if left <= 0:
+            break
+        if not sel.select(timeout=min(left, 1.0)):
+            if proc.poll() is not None:
+                break
+            continue
+        line = proc.stdout.readline()
+        if not line:
+            break
+        if line.startswith("point "):
+            _, p, rc, sec = line.split()
+            rec = {"point": int(p), "rc": int(rc), "s": float(sec)}
+        else:
+            rec = json.loads(line)
+        if rec.get("event") == "start":
+            start = time.perf_counter()
+        else:
+            recs.append(rec)
+    elapsed = min(time.perf_counter(), start + cap_s) - start if start is not None else 0.0
+    if proc.poll() is None:
+        proc.kill()
+    proc.wait()
+    return recs, elapsed
+
+
+def cpu_child(mode, argv):
+    """Child side of run_cpu_child: builds the sector(s) with the oracle (host CSR in the reference's basis
+    order), then runs the reference's own solver, printing one JSON record per finished solve."""
     import oracle as O
 
-    off, sites = O.chain_neighbours(m, True)
-    h = O.hamiltonian(m, n, off, sites, order=O.ORDER_ASCENDING)
-    t0 = time.perf_counter()
-    rc, ev, bd, it = O.lanczos(h, J, U, k, max_iter=sample_steps)
-    dt = time.perf_counter() - t0
-    per_solve = dt / it * gpu_steps
-    return {"value": 1.0 / per_solve, "unit": "eigensolves/s", "cores": omp_threads(), "kind": "port",
-            "sample": f"first {it} steps of the oracle's FP64 Lanczos (full reorthogonalisation, OpenMP CSR) on the "
-                      f"m=12 n=12 sector took {dt:.2f} s; extrapolated linearly to the {gpu_steps} steps of the FP64 "
-                      f"GPU solve"}
+    if mode == "sweep":
+        # the reference's exact sweep on one sector: Op::IRLM_eigen per J point, one point per OpenMP thread
+        # (src/analysis.cpp:302-314), H assembled per point (:311)
+        kind, m, n, k = argv[0], int(argv[1]), int(argv[2]), int(argv[3])
+        J = np.array([float(x) for x in argv[4].split(",")])
+        off, sites = O.chain_neighbours(m, True) if kind == "chain" else O.square_neighbours(4, 4, True)
+        h = O.hamiltonian(m, n, off, sites)
+        print(json.dumps({"event": "start", "D": h.D}), flush=True)
+        # the driver reports each finished point on stderr ("point p rc seconds"): sent to the parent as it happens
+        os.dup2(sys.stdout.fileno(), 2)
+        O.ref_irlm_sweep(h, J, 1.0, k, progress=True)
+        return
+    if mode == "requests":
+        # one request per thread: (geometry, m, Lx, Ly, n, J, mu) each built with the oracle and solved by the
+        # reference's Op::IRLM_eigen (Spectra GenEigs nev=4, ncv=9)
+        from concurrent.futures import ThreadPoolExecutor, as_completed
+
+        reqs = json.loads(argv[0])
+        nthreads = int(os.environ.get("OMP_NUM_THREADS", "1"))
+        os.environ["OMP_NUM_THREADS"] = "1"
+        R = O.ref()
+        print(json.dumps({"event": "start"}), flush=True)
+
+        def solve(q):
+            g, m, Lx, Ly, n, J, mu = q
+            mm = m if g == "chain" else Lx * Ly
+            off, sites = O.chain_neighbours(mm, True) if g == "chain" else O.square_neighbours(Lx, Ly, True)
+            t0 = time.perf_counter()
+            h = O.hamiltonian(mm, n, off, sites)
+            ev = np.zeros(4)
+            rc = R.ref_irlm_eigen(h.D, h.row_ptr, h.col, h.val, h.diag, J, 1.0, 4, ev)
+            return {"D": h.D, "rc": int(rc), "s": time.perf_counter() - t0}
+
+        with ThreadPoolExecutor(nthreads) as ex:
+            for f in as_completed([ex.submit(solve, q) for q in reqs]):
+                print(json.dumps(f.result()), flush=True)
+        return
+    raise SystemExit(f"unknown child mode {mode}")
+
+
+def cpu_baseline(Js, k, cap_s=45.0):
+    """The reference's own CPU path on the headline workload: Op::IRLM_eigen (Spectra GenEigs, nev = k,
+    ncv = 2k + 1, compiled from the reference's sources in oracle/_ref) on the m=12 n=12 sector at the batch's
+    J points, one point per OpenMP thread as src/analysis.cpp:302 runs its grid, for at most cap_s seconds.
+    value = solves converged / elapsed (0 when none converges within the cap)."""
+    model, ncpu = cpu_model()
+    threads = min(len(Js), omp_threads() or ncpu)
+    recs, el = run_cpu_child("sweep", ["chain", 12, 12, k, ",".join(f"{x:.17g}" for x in Js)], cap_s, threads)
+    ok = [r for r in recs if r["rc"] == 0]
+    return {"value": len(ok) / max(el, 1e-9), "unit": "eigensolves/s", "cores": threads, "kind": "reference",
+            "cpu_model": model, "nproc": ncpu,
+            "sample": f"the reference's Op::IRLM_eigen (Spectra nev={k}, ncv={2 * k + 1}) on the m=12 n=12 sector at the "
+                      f"{len(Js)} J points of the batch, one per OpenMP thread ({threads}), for {el:.0f} s: "
+                      f"{len(ok)} converged, {len(recs) - len(ok)} failed, {len(Js) - len(recs)} still running at the "
+                      f"cap (a full run of the same sweep failed on every point after ~500 s: Spectra's 1000-restart "
+                      f"limit)",
+            "finished": recs}
 
 
 def omp_threads():
@@ -141,40 +249,103 @@ def serving_sectors(dmax=5_000_000):
     return out
 
 
-def serving(ctx, nreq, rate, seed=7):
-    """A Poisson stream of `nreq` requests at `rate` req/s offered load: each a seed-drawn admissible sector,
-    J/U log-uniform in [0.01, 1], mu/U uniform in [0, 3], k = 4, mixed precision with FP64 rerun.  Returns
-    p50/p99 latency (submit to result) and completed solves per second over the stream's span."""
+def request_stream(nreq, seed):
+    """nreq requests of the serving mix: (geometry, m, Lx, Ly, n, J, mu), J/U log-uniform in [0.01, 1], mu/U uniform
+    in [0, 3], the sector uniform over the admissible ones."""
     rng = np.random.default_rng(seed)
     secs = serving_sectors()
-    with bhx.Server(ctx, lanes=16, k=4, ncv=160) as srv:
-        t0 = time.perf_counter()
-        tnext = t0
-        ids = []
-        for _ in range(nreq):
-            g, m, Lx, Ly, n = secs[rng.integers(len(secs))]
-            J = float(10 ** rng.uniform(-2, 0))
-            mu = float(rng.uniform(0, 3))
-            tnext += rng.exponential(1.0 / rate)
-            dt = tnext - time.perf_counter()
-            if dt > 0:
-                time.sleep(dt)
-            ids.append(srv.submit(g, n, J, 1.0, mu, m=m, Lx=Lx, Ly=Ly))
-        ok = srv.drain(600.0)
-        span = time.perf_counter() - t0
-        st = srv.stats()
-        res = [srv.poll(i) for i in ids]
-    bad = sum(1 for r in res if r is None or r.status != 0)
-    reruns = sum(1 for r in res if r is not None and r.used_fp64)
+    out = []
+    for _ in range(nreq):
+        g, m, Lx, Ly, n = secs[rng.integers(len(secs))]
+        out.append((g, m, Lx, Ly, n, float(10 ** rng.uniform(-2, 0)), float(rng.uniform(0, 3))))
+    return out, rng
+
+
+def stream(srv, reqs, rate, rng):
+    """Submits reqs with exponential gaps of mean 1/rate (Poisson arrivals), samples the server's queue
+    (requests submitted and not completed) as it goes, drains.  Returns results, span, queue samples."""
+    t0 = time.perf_counter()
+    tnext = t0
+    ids, qs = [], []
+    for q in reqs:
+        g, m, Lx, Ly, n, J, mu = q
+        tnext += rng.exponential(1.0 / rate)
+        dt = tnext - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        ids.append(srv.submit(g, n, J, 1.0, mu, m=m, Lx=Lx, Ly=Ly))
+        if len(ids) % 20 == 0:
+            qs.append((time.perf_counter() - t0, int(srv.stats().in_flight)))
+    t_sub = time.perf_counter() - t0
+    ok = srv.drain(900.0)
+    span = time.perf_counter() - t0
+    res = [srv.poll(i) for i in ids]
+    return res, span, t_sub, qs, ok
+
+
+def stream_summary(res, span, rate, t_sub, qs, ok):
     lat = np.array([r.t_done - r.t_submit for r in res if r is not None])
-    return {"requests": nreq, "offered_rps": rate, "completed": int(st.completed), "failed": bad, "drained": bool(ok),
+    return {"requests": len(res), "offered_rps": rate, "completed": int(sum(r is not None for r in res)),
+            "failed": int(sum(1 for r in res if r is None or r.status != 0)), "drained": bool(ok),
             "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
-            "solves_per_s": float(st.completed / span), "fp64_results": reruns, "sectors": len(secs),
+            "solves_per_s": float(len(lat) / span), "submit_s": t_sub, "span_s": span,
+            "fp64_results": int(sum(1 for r in res if r is not None and r.used_fp64)),
+            "queue_first_half_mean": float(np.mean([q for t, q in qs if t < t_sub / 2])) if qs else None,
+            "queue_second_half_mean": float(np.mean([q for t, q in qs if t >= t_sub / 2])) if qs else None}
+
+
+def serving(ctx, nreq, rate, sustained_n, seed=7):
+    """The serving stream: nreq requests at `rate` req/s offered load (Poisson) from a cold server (sector
+    CSRs built and the operator tuned on first use), then, on the same warm server, sustained_n requests at
+    0.7 of the rate the first stream completed (the sustained operating point: the queue is sampled to show it
+    does not grow).  k = 4, mixed precision with FP64 rerun on rejection, values with multiplicity."""
+    reqs, rng = request_stream(nreq, seed)
+    with bhx.Server(ctx, lanes=16, k=4, ncv=160) as srv:
+        res, span, t_sub, qs, ok = stream(srv, reqs, rate, rng)
+        cold = stream_summary(res, span, rate, t_sub, qs, ok)
+        log("serving cold:", json.dumps(cold))
+        sus = None
+        if sustained_n > 0:
+            r2 = 0.7 * cold["solves_per_s"]
+            reqs2, rng2 = request_stream(sustained_n, seed + 1)
+            res2, span2, t_sub2, qs2, ok2 = stream(srv, reqs2, r2, rng2)
+            sus = stream_summary(res2, span2, r2, t_sub2, qs2, ok2)
+            log("serving sustained:", json.dumps(sus))
+    return {"cold_stream": cold, "sustained": sus, "sectors": len(serving_sectors()),
             "mix": "chain m=6..13, square 3x3 and 3x4, n in [m-2, m+2], D <= 5e6; J/U log-uniform [0.01, 1], "
                    "mu/U uniform [0, 3], k=4"}
 
 
+def serving_cpu(nreq=24, seed=11, cap_s=40.0):
+    """CPU-only rate on the same mix: nreq requests drawn as the stream draws them, each built on the host by the
+    oracle and solved by the reference's Op::IRLM_eigen (Spectra, nev=4, ncv=9), one request per thread, for at
+    most cap_s seconds (host build included: the reference builds its Hamiltonian too)."""
+    reqs, _ = request_stream(nreq, seed)
+    model, ncpu = cpu_model()
+    threads = min(nreq, omp_threads() or ncpu)
+    recs, el = run_cpu_child("requests", [json.dumps(reqs)], cap_s, threads, setup_cap_s=60.0)
+    ok = [r for r in recs if r["rc"] == 0]
+    return {"solves_per_s": len(ok) / max(el, 1e-9), "cores": threads, "kind": "reference", "cpu_model": model,
+            "sample": f"{nreq} requests of the mix, {len(ok)} solved ({len(recs) - len(ok)} failed) in {el:.0f} s "
+                      f"(cap {cap_s:.0f} s), mean D of the solved {np.mean([r['D'] for r in ok]) if ok else 0:.0f}, "
+                      f"of the sample {np.mean([math.comb((q[1] if q[0] == 'chain' else q[2] * q[3]) + q[4] - 1, q[4]) for q in reqs]):.0f}"}
+
+
+def sweep_cpu(Js, cap_s=40.0):
+    """The reference's CPU sweep on the chain m=10, n=10 sector (D = 92,378) at the sweep's J points: one point per
+    OpenMP thread (src/analysis.cpp:302), Spectra nev=4, ncv=9, at most cap_s seconds."""
+    model, ncpu = cpu_model()
+    threads = min(len(Js), omp_threads() or ncpu)
+    recs, el = run_cpu_child("sweep", ["chain", 10, 10, 4, ",".join(f"{x:.17g}" for x in Js)], cap_s, threads)
+    ok = [r for r in recs if r["rc"] == 0]
+    return {"solves_per_s": len(ok) / max(el, 1e-9), "cores": threads, "kind": "reference", "cpu_model": model,
+            "sample": f"sector n=10 at the {len(Js)} J points: {len(ok)} solved, {len(recs) - len(ok)} failed in {el:.0f} s"}
+
+
 def main():
+    if len(sys.argv) > 2 and sys.argv[1] == "--cpu-child":
+        cpu_child(sys.argv[2], sys.argv[3:])
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -183,8 +354,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy kernel of BYTES bytes (calibrates HBM counters) and exit")
-    ap.add_argument("--serve", type=int, default=120, metavar="N",
+    ap.add_argument("--serve", type=int, default=2000, metavar="N",
                     help="requests of the serving stream at 200 req/s offered load (0: skip)")
+    ap.add_argument("--serve-sustained", type=int, default=400, metavar="N",
+                    help="requests of the sustained-rate stream on the warm server (0: skip)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the phase-diagram sweep leg")
     ap.add_argument("--no-mean-field", action="store_true", help="skip the mean-field phase-diagram leg")
     ap.add_argument("--mean-field-only", action="store_true",
@@ -226,8 +399,9 @@ def main():
     build_s = time.perf_counter() - t0
     log(f"build D={ham.D} nnz={ham.nnz} {build_s * 1e3:.1f} ms")
 
-    # plain FP64 GPU Lanczos: FP64 CSR and vectors, unbatched (accuracy comparator and baseline rate)
-    fp64_s, r64 = float("nan"), None
+    # plain FP64 GPU Lanczos: FP64 CSR and vectors, unbatched (accuracy comparator and baseline rate), and the
+    # FP64 values of every J point of the batch (the comparator of each mixed lane's four values)
+    fp64_s, r64, E64 = float("nan"), None, None
     if not args.quick:
         r64 = ham.lanczos(J0, U, k=k, prec=bhx.PREC_FP64)
         reps64 = 3
@@ -236,6 +410,7 @@ def main():
             r64 = ham.lanczos(J0, U, k=k, prec=bhx.PREC_FP64)
         fp64_s = (time.perf_counter() - t0) / reps64
         log(f"plain fp64: {fp64_s * 1e3:.1f} ms/solve, steps={r64.iters[0]}, E={r64.evals[0]}")
+        E64 = np.stack([ham.lanczos(Jv, U, k=k, prec=bhx.PREC_FP64).evals[0] for Jv in Js])
 
     for _ in range(args.warmup):
         ham.lanczos(Js, U, k=k, prec=bhx.PREC_MIXED)
@@ -256,6 +431,8 @@ def main():
     ms_step = dt / args.steps * 1e3
     value = B * args.steps / dt
     rel0 = abs(res.evals[0, 0] - r64.evals[0, 0]) / abs(r64.evals[0, 0]) if r64 is not None else float("nan")
+    # |dE_i| / |E_i| of every value of every lane against its own FP64 solve (values with multiplicity)
+    relerr = (np.abs(res.evals - E64) / np.abs(E64)) if E64 is not None else None
     log(f"mixed batched B={B}: {ms_step:.1f} ms/step ({value:.1f} solves/s), steps={res.iters.tolist()}, "
         f"fp64 reruns={int(res.used_fp64.sum())}, |dE0|/|E0| vs fp64 = {rel0:.2e}")
 
@@ -301,6 +478,7 @@ def main():
         # operator pass K1 at width 2 is this sector's SpMV inside the solver
         J3 = np.array([0.05, 0.2])
         h3.lanczos(J3, 1.0, k=4, prec=bhx.PREC_MIXED)
+        E3_64 = np.stack([h3.lanczos(Jv, 1.0, k=4, prec=bhx.PREC_FP64).evals[0] for Jv in J3])
         ctx.set_timing(True)
         ctx.kernel_stats("lanczos_spmv", reset=True)
         t3 = time.perf_counter()
@@ -311,6 +489,8 @@ def main():
         k1b3 = 4.0 * h3.nnz + 4.0 * (h3.D + 1) + 4.0 * h3.D + 2 * h3.D * (4 + 4 + 8)
         k1us3 = tot3 / max(c3, 1) * 1e3
         spmv3["lanczos_mixed_B2"] = {"s": t3, "iters": r3.iters.tolist(), "E0": r3.evals[:, 0].tolist(),
+                                     "evals": r3.evals.tolist(), "evals_fp64": E3_64.tolist(),
+                                     "rel_err_vs_fp64": (np.abs(r3.evals - E3_64) / np.abs(E3_64)).tolist(),
                                      "fp64_reruns": int(r3.used_fp64.sum()), "k1_us": k1us3,
                                      "k1_alg_GBs": k1b3 / k1us3 / 1e3, "k1_frac_of_peak": k1b3 / k1us3 / 1e3 / HBM_PEAK_GBS,
                                      "tuned": h3.tuned(0, bhx.PREC_MIXED, 2)}
@@ -330,6 +510,8 @@ def main():
         tsw = time.perf_counter() - t0
         sweep = {"s": tsw, "solves": 3 * Jsw.size, "solves_per_s": 3 * Jsw.size / tsw, "width": rsw["width"].tolist(),
                  "fp64_reruns": int(rsw["used_fp64"].sum()), "all_ok": bool(np.all(rsw["status"] == 0))}
+        if not args.no_cpu:
+            sweep["cpu_reference"] = sweep_cpu(Jsw)
         log("sweep chain m=10 n=9/10/11:", json.dumps(sweep))
 
     mf = None
@@ -339,12 +521,14 @@ def main():
 
     serve = None
     if args.serve > 0 and not args.quick:
-        serve = serving(ctx, args.serve, 200.0)
+        serve = serving(ctx, args.serve, 200.0, args.serve_sustained)
+        if not args.no_cpu:
+            serve["cpu_reference"] = serving_cpu()
         log("serving:", json.dumps(serve))
 
     cpu = None
     if not args.no_cpu and not args.quick:
-        cpu = cpu_baseline(m, n, J0, U, k, int(r64.iters[0]))
+        cpu = cpu_baseline(Js, k)
         log("cpu baseline:", json.dumps(cpu))
 
     out = {
@@ -365,6 +549,9 @@ def main():
         "plain_fp64_eigensolves_per_s": 1.0 / fp64_s,
         "speedup_vs_plain_fp64": value * fp64_s,
         "rel_err_E0_vs_fp64": rel0,
+        "rel_err_vs_fp64_per_level_max": relerr.max(axis=0).tolist() if relerr is not None else None,
+        "rel_err_vs_fp64": relerr.tolist() if relerr is not None else None,
+        "evals_lane0": res.evals[0].tolist(),
         "spmv": spmv,
         "spmv_square4x4_n10": spmv3 or None,
         "sweep_chain10": sweep,

@@ -44,6 +44,9 @@ class LzEngineBase {
     // lane 0's Lanczos coefficients after `steps` steps without analyses (alpha[t], t < steps; beta[t] couples
     // t - 1 and t, t = 1 .. steps) and, when basis is given, its normalised vectors v_t as [steps][D]
     virtual int export_tridiag(int steps, double* alpha, double* beta, double* basis) = 0;
+    // a lane that completes a deflation pass waits until no lane iterates, then the waiting lanes start their
+    // next passes together (off: each lane starts its next pass at once, as the server wants)
+    virtual void set_deflation_barrier(bool on) = 0;
 };
 
 // sectors with D at most opts.fullre_d (0: 16384, < 0: none) run FP64 with full reorthogonalisation

@@ -1316,6 +1316,7 @@ class LzEngine final : public LzEngineBase {
         rh.assign(B, RunHost());
         nextc.assign(B, 0);
         act.assign(B, 0);
+        waiting.assign(B, 0);
         nlock.assign(B, 0);
         jl.assign(B, 0);
         jan.assign(B, 0);
@@ -1333,6 +1334,7 @@ class LzEngine final : public LzEngineBase {
     }
 
     int lanes() const override { return B; }
+    void set_deflation_barrier(bool on) override { defl_barrier = on; }
     int export_tridiag(int steps, double* alpha, double* beta, double* basis) override {
         if (steps < 1 || steps > cap) return bh_set_error(ctx, BH_EINVAL, "tridiag: steps beyond the Krylov slots");
         std::vector<double> ha2(static_cast<size_t>(steps) * B), hb2(static_cast<size_t>(steps + 1) * B),
@@ -1379,9 +1381,11 @@ class LzEngine final : public LzEngineBase {
     }
 
     int admit(int slot, double J, double U) override {
-        if (slot < 0 || slot >= B || act[slot]) return bh_set_error(ctx, BH_EINVAL, "engine: lane busy or out of range");
+        if (slot < 0 || slot >= B || act[slot] || waiting[slot])
+            return bh_set_error(ctx, BH_EINVAL, "engine: lane busy or out of range");
         rh[slot] = RunHost();
         act[slot] = 1;
+        waiting[slot] = 0;
         nlock[slot] = 0;
         jl[slot] = 0;
         tot[slot] = 0;
@@ -1610,8 +1614,14 @@ class LzEngine final : public LzEngineBase {
                     if (R.fz_theta[l] <= kappa + kmarg) keep.push_back(l);
                 const int nnew = static_cast<int>(keep.size() + fz.size());
                 if (!fz.empty() && nnew < D && nnew <= ymax && R.pass + 1 < LZ_PASS_MAX) {
-                    plan_fz[b] = fz;
-                    plan_keep[b] = keep;
+                    Deflation& P = dplan[b];
+                    P.theta.clear();
+                    P.s.clear();
+                    for (int i : fz) {
+                        P.theta.push_back(an.theta[i]);
+                        P.s.push_back(an.s[i]);
+                    }
+                    P.keep = keep;
                     deflate_now[b] = 1;
                     return true;
                 }
@@ -1781,7 +1791,38 @@ class LzEngine final : public LzEngineBase {
         }
         if (any_lock) BH_TRY(lock(new_locks, lock_slot, ans));
         if (any_restart) BH_TRY(restart(restart_now));
-        if (any_deflate) BH_TRY(deflate(deflate_now, ans));
+        if (any_deflate) {
+            if (!defl_barrier) {
+                BH_TRY(deflate(deflate_now));
+            } else {
+                // the lane waits (inactive, not finished) until every unfinished lane has completed its pass
+                for (int b = 0; b < B; ++b)
+                    if (deflate_now[b]) {
+                        waiting[b] = 1;
+                        act[b] = 0;
+                    }
+                any_done = true;  // the device's active flags are refreshed below
+            }
+        }
+        // deflation barrier: once no lane iterates, the waiting lanes start their next pass together (one
+        // formation of their frozen vectors, one read of the Krylov basis)
+        if (defl_barrier) {
+            bool any_act = false, any_wait = false;
+            for (int b = 0; b < B; ++b) {
+                any_act = any_act || act[b];
+                any_wait = any_wait || waiting[b];
+            }
+            if (!any_act && any_wait) {
+                std::vector<int> flag(waiting);
+                BH_TRY(deflate(flag));
+                for (int b = 0; b < B; ++b)
+                    if (flag[b]) {
+                        waiting[b] = 0;
+                        act[b] = 1;
+                    }
+                any_done = true;
+            }
+        }
         if (any_done) {
             BH_HIP(ctx, hipMemcpyAsync(st.active, act.data(), B * 4, hipMemcpyHostToDevice, s));
             BH_HIP(ctx, hipStreamSynchronize(s));
@@ -2508,18 +2549,18 @@ class LzEngine final : public LzEngineBase {
         return BH_OK;
     }
     // Deflation restart of the flagged lanes (plans made by the analysis, lanes rolled back to the analysed
-    // step): the Ritz vectors of the levels in plan_fz[b] are formed in FP64, orthonormalised against the
+    // step): the Ritz vectors of the levels in dplan[b] are formed in FP64, orthonormalised against the
     // lane's frozen vectors and among themselves, and stored in Y after them; the lane then starts its next
     // pass at slot 0 from the Park-Miller sequence continued past the values of its earlier start vectors,
     // orthogonalised against every frozen vector.  Under selective orthogonalisation the frozen vectors are
     // the lane's locked set at the start of the pass (the pass's own locks are appended after them); under
     // full reorthogonalisation every step projects them out (k_lz_fr_dots / k_lz_fr_axpy).
-    int deflate(const std::vector<int>& flag, const std::vector<Analysis>& ans) {
+    int deflate(const std::vector<int>& flag) {
         int nv = 0, jmax = 0;
         for (int b = 0; b < B; ++b)
             if (flag[b]) {
-                nv = std::max(nv, static_cast<int>(plan_fz[b].size()));
-                jmax = std::max(jmax, static_cast<int>(ans[b].s[plan_fz[b][0]].size()));
+                nv = std::max(nv, static_cast<int>(dplan[b].theta.size()));
+                jmax = std::max(jmax, static_cast<int>(dplan[b].s[0].size()));
             }
         std::vector<double> hsc(static_cast<size_t>(jmax) * B);
         BH_HIP(ctx, hipMemcpyAsync(hsc.data(), st.scale, hsc.size() * 8, hipMemcpyDeviceToHost, s));
@@ -2528,9 +2569,9 @@ class LzEngine final : public LzEngineBase {
         std::vector<int> njb(B, 0);
         for (int b = 0; b < B; ++b) {
             if (!flag[b]) continue;
-            njb[b] = static_cast<int>(ans[b].s[plan_fz[b][0]].size());
-            for (int i = 0; i < static_cast<int>(plan_fz[b].size()); ++i) {
-                const std::vector<double>& sv = ans[b].s[plan_fz[b][i]];
+            njb[b] = static_cast<int>(dplan[b].s[0].size());
+            for (int i = 0; i < static_cast<int>(dplan[b].s.size()); ++i) {
+                const std::vector<double>& sv = dplan[b].s[i];
                 for (int t = 0; t < njb[b]; ++t) C[(static_cast<size_t>(t) * nv + i) * B + b] = sv[t] * hsc[static_cast<size_t>(t) * B + b];
             }
         }
@@ -2544,7 +2585,7 @@ class LzEngine final : public LzEngineBase {
             int ns = 0;
             for (int b = 0; b < B; ++b) {
                 if (!flag[b]) continue;
-                const std::vector<int>& kp = plan_keep[b];
+                const std::vector<int>& kp = dplan[b].keep;
                 std::vector<double> th;
                 for (size_t q = 0; q < kp.size(); ++q) {
                     src[q * B + b] = kp[q];
@@ -2566,7 +2607,7 @@ class LzEngine final : public LzEngineBase {
         // new frozen vector i of each lane: orthogonal to the lane's frozen vectors so far, then normalised
         for (int i = 0; i < nv; ++i) {
             for (int b = 0; b < B; ++b) {
-                const bool on = flag[b] && i < static_cast<int>(plan_fz[b].size());
+                const bool on = flag[b] && i < static_cast<int>(dplan[b].theta.size());
                 cnt[b] = on ? rh[b].nfz + i : -1;
                 slot[b] = on ? rh[b].nfz + i : -1;
             }
@@ -2581,7 +2622,7 @@ class LzEngine final : public LzEngineBase {
         }
         // fresh start vectors in W (free between steps), orthogonal to every frozen vector
         for (int b = 0; b < B; ++b) {
-            cnt[b] = flag[b] ? rh[b].nfz + static_cast<int>(plan_fz[b].size()) : -1;
+            cnt[b] = flag[b] ? rh[b].nfz + static_cast<int>(dplan[b].theta.size()) : -1;
             slot[b] = flag[b] ? 0 : -1;
             if (!flag[b]) continue;
             const uint64_t p = 2147483647ull;
@@ -2611,8 +2652,8 @@ class LzEngine final : public LzEngineBase {
         for (int b = 0; b < B; ++b) {
             if (!flag[b]) continue;
             RunHost& R = rh[b];
-            for (int i : plan_fz[b]) R.fz_theta.push_back(ans[b].theta[i]);
-            R.nfz += static_cast<int>(plan_fz[b].size());
+            for (double th : dplan[b].theta) R.fz_theta.push_back(th);
+            R.nfz += static_cast<int>(dplan[b].theta.size());
             R.pass += 1;
             R.alpha.clear();
             R.beta.clear();
@@ -2687,8 +2728,16 @@ class LzEngine final : public LzEngineBase {
     std::vector<double> Jb, Ub, hlo, hhi, ha, hb;
     double an0[B] = {};
     Restart plan[B];
-    std::vector<int> plan_fz[B];    // Ritz indices of the levels a lane freezes at its deflation restart
-    std::vector<int> plan_keep[B];  // the lane's frozen slots kept at that restart (ascending)
+    // deflation restart of a lane: the levels it freezes (Ritz values and their eigenvectors of T at the analysed
+    // step) and its frozen slots that stay (ascending)
+    struct Deflation {
+        std::vector<double> theta;
+        std::vector<std::vector<double>> s;
+        std::vector<int> keep;
+    };
+    Deflation dplan[B];
+    std::vector<int> waiting;   // lanes whose pass is complete, waiting for the deflation barrier
+    bool defl_barrier = false;  // batch solves: lanes deflate together once none is iterating
     int lmax_used = 0;
     bool mult = true;  // eigenvalues with multiplicity (deflation passes); false: distinct levels
     int ymax = LZ_LMAX;
@@ -2789,6 +2838,9 @@ static int lanczos_batch(bh_ctx* ctx, bh_ham* h, int prec, const double* J, cons
     const auto ts0 = std::chrono::steady_clock::now();
     auto e = bh_make_engine(ctx, h, prec, nreal, o, ctx->stream, true, &rc);
     if (!e) return rc;
+    // the lanes of one batch finish together: their deflation passes start together too (one formation of the
+    // frozen vectors for all of them; a lane's arithmetic does not depend on when its pass starts)
+    e->set_deflation_barrier(true);
     for (int b = 0; b < nreal; ++b) BH_TRY(e->admit(b, J[b], U[b]));
     e->prof.setup_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts0).count();
     std::vector<int> fin, all;

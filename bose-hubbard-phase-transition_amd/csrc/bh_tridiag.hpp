@@ -242,7 +242,10 @@ inline void jacobi_eig(int n, double* A, double* w, double* V) {
     for (int i = 0; i < n * n; ++i) V[i] = Vs[i];
 }
 
-// all eigenvalues of a symmetric tridiagonal matrix, ascending (implicit QL, Wilkinson shifts)
+// all eigenvalues of a symmetric tridiagonal matrix, ascending: the implicit QL algorithm with Wilkinson
+// shifts of Bowdler, Martin, Reinsch and Wilkinson (Numer. Math. 11, 293-306, 1968), the procedure tql1 of
+// the Handbook for Automatic Computation vol. II and of EISPACK, whose step sequence (shift g, rotation
+// parameters r, s, c, p, the search for a negligible off-diagonal element) the loop below follows
 inline int ql_eigvals(int n, const double* d_in, const double* e_in, double* w) {
     std::vector<double> d(d_in, d_in + n), e(static_cast<size_t>(n) + 1, 0.0);
     for (int i = 0; i + 1 < n; ++i) e[i] = e_in[i];

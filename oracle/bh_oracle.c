@@ -416,6 +416,9 @@ void orc_start_vector(int64_t D, uint64_t seed, double *r) {
 /*
  * d[0..n-1] diagonal, e[0..n-2] off-diagonal.  On return d holds the eigenvalues (ascending) and,
  * when z != NULL, z (n*n, column-major, z[i + n*c] = component i of vector c) the eigenvectors.
+ * The implicit QL algorithm with Wilkinson shifts of Bowdler, Martin, Reinsch and Wilkinson (Numer. Math.
+ * 11, 293-306, 1968): the procedures tql1 (values) and tql2 (vectors) of the Handbook for Automatic
+ * Computation vol. II and of EISPACK, whose step sequence the loop below follows.
  */
 int orc_tridiag_eig(int n, double *d, const double *e_in, double *z) {
     double *e = (double *)calloc((size_t)n + 1, sizeof(double));

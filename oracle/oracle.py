@@ -111,6 +111,9 @@ def ref():
         R.ref_h_mf.argtypes = [C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, _f64p]
         R.ref_sf_density.restype = C.c_double
         R.ref_sf_density.argtypes = [_f64p, C.c_int]
+        R.ref_irlm_sweep.restype = C.c_int
+        R.ref_irlm_sweep.argtypes = [C.c_int64, _i64p, _i32p, _f64p, _f64p, C.c_int, _f64p, _f64p, C.c_int, _i32p,
+                                     _f64p, _f64p, C.c_int]
         R.ref_gap_ratios.restype = None
         R.ref_gap_ratios.argtypes = [_f64p, C.c_int, _f64p]
         R.ref_coherence.restype = C.c_double
@@ -324,6 +327,19 @@ def coherence(rho) -> float:
 def gap_ratio(ev) -> float:
     ev = np.ascontiguousarray(ev, np.float64)
     return float(lib().orc_gap_ratio(ev.size, ev, None))
+
+
+def ref_irlm_sweep(h: CSR, J, U, nb_eigen: int, progress: bool = False):
+    """The reference's OpenMP sweep (src/analysis.cpp:302-314) over the points (J[p], U[p]) of one sector:
+    (threads, rc[p], seconds[p], evals[p, nb_eigen]); progress: a line per finished point on stderr."""
+    J = np.ascontiguousarray(J, np.float64)
+    U = np.ascontiguousarray(np.broadcast_to(U, J.shape), np.float64)
+    rc = np.zeros(J.size, np.int32)
+    secs = np.zeros(J.size)
+    ev = np.zeros((J.size, nb_eigen))
+    th = ref().ref_irlm_sweep(h.D, h.row_ptr, _c32(h.col), _c64(h.val), h.diag, J.size, J, U, nb_eigen, rc, secs,
+                              ev.reshape(-1), int(progress))
+    return th, rc, secs, ev
 
 
 def ref_gap_ratio(ev) -> float:

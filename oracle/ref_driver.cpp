@@ -8,7 +8,11 @@
 // past the end of a one-column buffer (src/hamiltonian.cpp:180 -> :94), so its output is undefined.
 // The reference's own solvers (Op::IRLM_eigen = Spectra GenEigs, Op::exact_eigen = Eigen's dense
 // SelfAdjointEigenSolver) are run on a Hamiltonian handed over as CSR arrays.
+#include <omp.h>
+
 #include <algorithm>
+#include <cstdio>
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <exception>
@@ -120,6 +124,47 @@ int ref_irlm_eigen(int64_t D, const int64_t* row_ptr, const int32_t* col, const 
     } catch (const std::exception&) {
         return 1;
     }
+}
+
+// The reference's exact sweep over a list of points (the grid loop of Analysis::calculate_and_save,
+// src/analysis.cpp:302-314): `#pragma omp parallel for schedule(dynamic)` over the points, each assembling
+// its own H = JH * J + UH * U (:311) and calling Op::IRLM_eigen(H, nb_eigen) (:314).  Outputs per point:
+// rc (0 converged, 1 the solver threw: Spectra's "Eigenvalue computation failed" after its 1000 restarts),
+// seconds of the solve, the nb_eigen real parts ascending.  With progress != 0 a line "point p rc seconds" goes
+// to stderr as each point ends.  Returns the OpenMP threads used.
+int ref_irlm_sweep(int64_t D, const int64_t* row_ptr, const int32_t* col, const double* val, const double* diag,
+                   int npts, const double* J, const double* U, int nb_eigen, int* rc, double* secs, double* evals,
+                   int progress) {
+    const Eigen::SparseMatrix<double> JH = to_sparse(D, row_ptr, col, val, diag, 1.0, 0.0);
+    const Eigen::SparseMatrix<double> UH = to_sparse(D, row_ptr, col, val, diag, 0.0, 1.0);
+    int threads = 1;
+#pragma omp parallel
+    {
+#pragma omp single
+        threads = omp_get_num_threads();
+    }
+#pragma omp parallel for schedule(dynamic)
+    for (int p = 0; p < npts; ++p) {
+        const auto t0 = std::chrono::steady_clock::now();
+        try {
+            Eigen::SparseMatrix<double> H = JH * J[p] + UH * U[p];
+            Eigen::MatrixXcd vecs;
+            Eigen::VectorXcd ev = Op::IRLM_eigen(H, nb_eigen, vecs);
+            std::vector<double> re(ev.size());
+            for (Eigen::Index i = 0; i < ev.size(); ++i) re[i] = ev[i].real();
+            std::sort(re.begin(), re.end());
+            for (int i = 0; i < nb_eigen; ++i) evals[static_cast<size_t>(p) * nb_eigen + i] = re[i];
+            rc[p] = 0;
+        } catch (const std::exception&) {
+            rc[p] = 1;
+        }
+        secs[p] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (progress) {
+            std::fprintf(stderr, "point %d %d %.3f\n", p, rc[p], secs[p]);
+            std::fflush(stderr);
+        }
+    }
+    return threads;
 }
 
 // Op::exact_eigen (src/operator.cpp:92-101): dense SelfAdjointEigenSolver, all D eigenvalues ascending
