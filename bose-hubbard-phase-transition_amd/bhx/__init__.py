@@ -120,6 +120,7 @@ def lib():
             "bh_ham_copy_packed": (C.c_int, [_vp, _vp, C.POINTER(C.c_int)]),
             "bh_ham_prepare": (C.c_int, [_vp, C.c_int]),
             "bh_spmv": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]),
+            "bh_spmv_noalloc": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]),
             "bh_spmv_host": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
             "bh_spmv_bench": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
                                         C.POINTER(C.c_double)]),
@@ -377,6 +378,20 @@ class Hamiltonian:
         Y = np.zeros_like(X)
         self.ctx.check(lib().bh_spmv_host(self.ctx.h, self.h, prec, B, _ptr(Jv), _ptr(Uv), _ptr(X), _ptr(Y)))
         return Y[:, 0] if one else Y
+
+    def prepare(self, prec: int):
+        """Materialises what `prec` needs (the FP64 CSR) ahead of the first operator application."""
+        self.ctx.check(lib().bh_ham_prepare(self.h, prec))
+
+    def spmv_device(self, J, U, x_ptr: int, y_ptr: int, B: int, prec: int = PREC_FP64, noalloc: bool = False,
+                    stream: int = 0):
+        """y = H x on device buffers [D][B] (double for FP64, float for mixed) given by address; noalloc
+        selects the entry point that allocates nothing (the handle must be prepared for `prec`)."""
+        Jv = np.ascontiguousarray(np.broadcast_to(np.asarray(J, np.float64), (B,)))
+        Uv = np.ascontiguousarray(np.broadcast_to(np.asarray(U, np.float64), (B,)))
+        f = lib().bh_spmv_noalloc if noalloc else lib().bh_spmv
+        self.ctx.check(f(self.ctx.h, self.h, prec, B, _ptr(Jv), _ptr(Uv), _vp(x_ptr), _vp(y_ptr),
+                         _vp(stream) if stream else None))
 
     def spmv_bench(self, prec: int, B: int = 1, reps: int = 20):
         ms, by = C.c_double(), C.c_double()

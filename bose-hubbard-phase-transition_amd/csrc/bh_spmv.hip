@@ -7,6 +7,7 @@
 // The kernel is HBM-bound: per pass it reads the matrix once (4 B per entry in the packed format,
 // 12 B in FP64 CSR) plus the row pointers and the diagonal, and B * (x gather + y write).
 #include <cmath>
+#include <mutex>
 
 #include "bh_kernels.hpp"
 #include "bh_spmv.hpp"
@@ -238,7 +239,79 @@ double bh_spmv_bytes(const bh_ham* h, int prec, int B) {
 
 int bh_is_pow2_le64(int B) { return B >= 1 && B <= 64 && (B & (B - 1)) == 0; }
 
+// ---- the SpMV entry point that allocates nothing -------------------------------------------------------------
+// bh_spmv stages the per-lane coefficients J_b, U_b in two small device arrays it allocates per call.  Callers that
+// cannot allocate (code under HIP graph capture, tight serving loops) use bh_spmv_noalloc: the coefficients go to
+// this file-scope array instead, whose device address is looked up once per device with hipGetSymbolAddress (the
+// array's name in host code is a host-side stand-in, never a device address).  Every call on a device shares
+// that device's copy, so calls that overlap on one device race.
+namespace {
+constexpr int SPMV_SCRATCH_LANES = 64;  // the largest batch width of the SpMV kernels
+static __device__ __align__(256) double scratch[2 * SPMV_SCRATCH_LANES];
+constexpr int SPMV_SCRATCH_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[SPMV_SCRATCH_DEVICES] = {};
+
+// device address of `scratch` on the context's device (the device is current), looked up on first use
+int scratch_ptr(bh_ctx* ctx, double** out) {
+    if (ctx->device < 0 || ctx->device >= SPMV_SCRATCH_DEVICES)
+        return bh_set_error(ctx, BH_EINVAL, "spmv_noalloc: device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[ctx->device]) {
+        void* p = nullptr;
+        BH_HIP(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[ctx->device] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[ctx->device];
+    return BH_OK;
+}
+}  // namespace
+
 extern "C" {
+
+int bh_spmv_noalloc(bh_ctx* ctx, bh_ham* h, int prec, int B, const double* J, const double* U, const void* x, void* y,
+                    void* stream) {
+    if (!ctx || !h || !J || !U || !x || !y) return BH_EINVAL;
+    if (!bh_is_pow2_le64(B) || B > SPMV_SCRATCH_LANES)
+        return bh_set_error(ctx, BH_EINVAL, "spmv_noalloc: B must be a power of two <= 64 (the scratch holds 64 lanes)");
+    if (prec != BH_PREC_FP64 && prec != BH_PREC_MIXED) return bh_set_error(ctx, BH_EINVAL, "spmv_noalloc: prec");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    BH_HIP(ctx, hipSetDevice(ctx->device));
+    if (prec == BH_PREC_FP64 && !h->col64)
+        return bh_set_error(ctx, BH_EINVAL, "spmv_noalloc: FP64 CSR not materialised (bh_ham_prepare first)");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    double* sc = nullptr;
+    BH_TRY(scratch_ptr(ctx, &sc));
+    BH_HIP(ctx, hipMemcpyAsync(sc, J, B * sizeof(double), hipMemcpyHostToDevice, s));
+    BH_HIP(ctx, hipMemcpyAsync(sc + SPMV_SCRATCH_LANES, U, B * sizeof(double), hipMemcpyHostToDevice, s));
+    // the variant the autotuner chose for this sector (no tuning here: it times with events), else the default
+    // bh_apply uses under capture; the staged engine only when its row-block table for B already exists
+    const int key = bh_tune_key(1, prec, B);
+    auto it = h->tuned.find(key);
+    int var;
+    if (it != h->tuned.end()) {
+        var = it->second.variant;
+    } else {
+        const int eng = bh_engine_choice(B, h);
+        var = eng == 2 ? 1 : (eng == 1 ? 0 : 100 + bh_pick_lanes(h, B));
+    }
+    if (var == 0 && h->rowblocks.find(B) == h->rowblocks.end()) var = 1;
+    if (prec == BH_PREC_FP64) {
+        SpmvArgs<double> a{h->D, h->row_ptr, h->words, h->col64, h->val64, nullptr, h->diag64, h->table, h->code_bits,
+                           sc, sc + SPMV_SCRATCH_LANES, static_cast<const double*>(x), static_cast<double*>(y), 1};
+        BH_TRY(spmv_launch(h, var, &a, nullptr, BH_FMT_CSR64, B, s));
+    } else {
+        SpmvArgs<float> a{h->D, h->row_ptr, h->words, nullptr, nullptr, h->diag, nullptr, h->table, h->code_bits,
+                          sc, sc + SPMV_SCRATCH_LANES, static_cast<const float*>(x), static_cast<float*>(y), 1};
+        BH_TRY(spmv_launch(h, var, nullptr, &a, BH_FMT_PACKED, B, s));
+    }
+    BH_HIP(ctx, hipGetLastError());
+    // outside a capture the call returns with its work done (the host's J and U may be reused at once)
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    BH_HIP(ctx, hipStreamIsCapturing(s, &cs));
+    if (cs == hipStreamCaptureStatusNone) BH_HIP(ctx, hipStreamSynchronize(s));
+    return BH_OK;
+}
 
 int bh_spmv(bh_ctx* ctx, bh_ham* h, int prec, int B, const double* J, const double* U, const void* x, void* y,
             void* stream) {

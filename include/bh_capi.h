@@ -169,6 +169,16 @@ int bh_ham_set_tuned(bh_ham* h, int kind, int prec, int B, int variant);
  * float for BH_PREC_MIXED (FP64 accumulation).  stream: a hipStream_t or NULL for the library's. */
 int bh_spmv(bh_ctx* ctx, bh_ham* h, int prec, int B, const double* J, const double* U, const void* x, void* y,
             void* stream);
+/* bh_spmv that allocates nothing.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops that forbid hipMalloc).  J and U
+ * are staged in a library-owned device array of 2 x 64 doubles (one copy per GPU, held until the process
+ * ends) instead of per-call buffers, so B <= 64, and calls that overlap on one GPU race on it.  The
+ * handle must already hold what the precision needs (FP64: call bh_ham_prepare first; BH_EINVAL
+ * otherwise); the variant is the autotuner's choice for (h, prec, B) if one was made, else the default,
+ * and nothing is tuned.  Same results as bh_spmv with the same variant.  Outside stream capture the call
+ * returns after its work on `stream` is done. */
+int bh_spmv_noalloc(bh_ctx* ctx, bh_ham* h, int prec, int B, const double* J, const double* U, const void* x,
+                    void* y, void* stream);
 /* Host-array convenience: x, y are host doubles [D][B]; FP64 accumulation in both precisions. */
 int bh_spmv_host(bh_ctx* ctx, bh_ham* h, int prec, int B, const double* J, const double* U, const double* x,
                  double* y);
