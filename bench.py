@@ -38,6 +38,10 @@ else:
 HBM_PEAK_GBS = 8000.0
 
 
+def bhx_prec(name):
+    return getattr(bhx, "PREC_" + name) if bhx is not None else None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -162,6 +166,33 @@ def cpu_baseline(Js, k, cap_s=45.0):
             "finished": recs}
 
 
+def rerun_causes(ctx, Js, widths, m=10, k=4):
+    """Why mixed solves of the sweep are rerun in FP64 (untimed): the sweep's three sectors solved again in
+    mixed precision with the rerun switched off (fallback = -1), each rejected solve classified by the first
+    failing test of the acceptance rule -- 'uncertified': the mixed solve itself reports it did not certify its
+    k values (a copy of a degenerate level missed or left unconverged in a dense cluster); 'E0_bound':
+    ||r||^2 / gap of E0 above 1e-9 |E0| (tiny |E0| of the Mott regime); 'excited_bound': a level above E0
+    whose bound exceeds 1e-6 of its scale."""
+    off, sites = bhx.lattice("chain", m=m)
+    out = {"uncertified": 0, "E0_bound": 0, "excited_bound": 0}
+    for n, W in zip((m - 1, m, m + 1), widths):
+        b = ctx.basis(m, n)
+        h = b.hamiltonian(off, sites)
+        W = W or 32
+        for c0 in range(0, len(Js), W):
+            r = h.lanczos(Js[c0:c0 + W], 1.0, k=k, prec=bhx.PREC_MIXED, fallback=-1)
+            for ev, ee, st in zip(r.evals, r.errest, r.status):
+                if st != 0:
+                    out["uncertified"] += 1
+                elif not ee[0] <= 1e-9 * abs(ev[0]):
+                    out["E0_bound"] += 1
+                elif not np.all(ee[1:] <= 1e-6 * np.maximum(np.abs(ev[1:]), np.nanmax(ev) - ev[0])):
+                    out["excited_bound"] += 1
+        h.close()
+        b.close()
+    return out
+
+
 def omp_threads():
     """threads the oracle's OpenMP loops use on this host"""
     try:
@@ -267,6 +298,7 @@ def stream(srv, reqs, rate, rng):
     t0 = time.perf_counter()
     tnext = t0
     ids, qs = [], []
+    done0 = srv.stats().completed
     for q in reqs:
         g, m, Lx, Ly, n, J, mu = q
         tnext += rng.exponential(1.0 / rate)
@@ -276,8 +308,20 @@ def stream(srv, reqs, rate, rng):
         ids.append(srv.submit(g, n, J, 1.0, mu, m=m, Lx=Lx, Ly=Ly))
         if len(ids) % 20 == 0:
             qs.append((time.perf_counter() - t0, int(srv.stats().in_flight)))
+        if len(ids) % 500 == 0:
+            log(f"  submitted {len(ids)}/{len(reqs)}, {srv.stats().completed - done0} served")
     t_sub = time.perf_counter() - t0
-    ok = srv.drain(900.0)
+    # drain, reporting each further tenth of the stream served
+    ok, step = False, max(len(reqs) // 10, 1)
+    nxt = srv.stats().completed - done0 + step
+    while time.perf_counter() - t0 < 900.0:
+        ok = srv.drain(2.0)
+        served = srv.stats().completed - done0
+        if ok or served >= nxt:
+            log(f"  served {served}/{len(reqs)} at {time.perf_counter() - t0:.1f} s")
+            nxt = served + step
+        if ok:
+            break
     span = time.perf_counter() - t0
     res = [srv.poll(i) for i in ids]
     return res, span, t_sub, qs, ok
@@ -342,6 +386,34 @@ def sweep_cpu(Js, cap_s=40.0):
             "sample": f"sector n=10 at the {len(Js)} J points: {len(ok)} solved, {len(recs) - len(ok)} failed in {el:.0f} s"}
 
 
+def spmv_table(h, Bs, precs=((bhx_prec("MIXED"), "mixed"), (bhx_prec("FP64"), "fp64"))):
+    out = {}
+    for prec, name in precs:
+        for B in Bs:
+            ms, by = h.spmv_bench(prec, B, 30)
+            out[f"{name}_B{B}"] = {"us": ms * 1e3, "alg_GBs": by / ms / 1e6, "frac_of_peak": by / ms / 1e6 / HBM_PEAK_GBS,
+                                   "tuned": h.tuned(1, prec, B)}
+    return out
+
+
+def spmv_only():
+    """The SpMV table alone: chain m=12 n=12 (lexicographic layout) at B = 1, 16 and the 4x4 torus n=10 at B = 1,
+    both precisions, 30 launches each after the tuner's choice."""
+    ctx = bhx.Context(0)
+    out = {}
+    for key, (geom, m, Lx, Ly, sites, n, Bs) in {"chain12": ("chain", 12, 0, 0, 12, 12, (1, 16)),
+                                                 "square4x4_n10": ("square", 0, 4, 4, 16, 10, (1,))}.items():
+        off, nb = bhx.lattice(geom, m=m, Lx=Lx, Ly=Ly, periodic=True)
+        b = ctx.basis(sites, n, bhx.ORDER_LEX)
+        h = b.hamiltonian(off, nb)
+        out[key] = {"D": h.D, "nnz": h.nnz, **spmv_table(h, Bs)}
+        log(key, json.dumps(out[key]))
+        h.close()
+        b.close()
+    ctx.close()
+    return out
+
+
 def main():
     if len(sys.argv) > 2 and sys.argv[1] == "--cpu-child":
         cpu_child(sys.argv[2], sys.argv[3:])
@@ -354,10 +426,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy kernel of BYTES bytes (calibrates HBM counters) and exit")
-    ap.add_argument("--serve", type=int, default=2000, metavar="N",
-                    help="requests of the serving stream at 200 req/s offered load (0: skip)")
-    ap.add_argument("--serve-sustained", type=int, default=400, metavar="N",
+    ap.add_argument("--serve", type=int, default=600, metavar="N",
+                    help="requests of the serving stream at 200 req/s offered load (0: skip; the full stream is "
+                         "--serving-only --serve 2000)")
+    ap.add_argument("--serve-sustained", type=int, default=200, metavar="N",
                     help="requests of the sustained-rate stream on the warm server (0: skip)")
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="run only the SpMV table (chain m=12 n=12 at B = 1, 16; 4x4 n=10 at B = 1) and print its JSON: "
+                         "for counter passes")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="run only the serving legs (and their CPU comparison unless --no-cpu) and print their JSON")
     ap.add_argument("--no-sweep", action="store_true", help="skip the phase-diagram sweep leg")
     ap.add_argument("--no-mean-field", action="store_true", help="skip the mean-field phase-diagram leg")
     ap.add_argument("--mean-field-only", action="store_true",
@@ -379,6 +457,20 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         log(f"copied {args.copy_calib} bytes")
+        return
+
+    if args.spmv_only:
+        print(json.dumps(spmv_only()))
+        return
+
+    if args.serving_only:
+        ctx = bhx.Context(0)
+        log("device:", bhx.device_name(0))
+        serve = serving(ctx, args.serve, 200.0, args.serve_sustained)
+        if not args.no_cpu:
+            serve["cpu_reference"] = serving_cpu()
+        print(json.dumps({"serving": serve}))
+        ctx.close()
         return
 
     if args.mean_field_only:
@@ -509,7 +601,8 @@ def main():
         rsw = ctx.sweep("chain", 10, Jsw, 1.0, musw, m=10, k=4)
         tsw = time.perf_counter() - t0
         sweep = {"s": tsw, "solves": 3 * Jsw.size, "solves_per_s": 3 * Jsw.size / tsw, "width": rsw["width"].tolist(),
-                 "fp64_reruns": int(rsw["used_fp64"].sum()), "all_ok": bool(np.all(rsw["status"] == 0))}
+                 "fp64_reruns": int(rsw["used_fp64"].sum()), "all_ok": bool(np.all(rsw["status"] == 0)),
+                 "rerun_causes": rerun_causes(ctx, Jsw, rsw["width"].tolist())}
         if not args.no_cpu:
             sweep["cpu_reference"] = sweep_cpu(Jsw)
         log("sweep chain m=10 n=9/10/11:", json.dumps(sweep))

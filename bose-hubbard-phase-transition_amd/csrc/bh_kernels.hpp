@@ -23,15 +23,23 @@ template <typename XT, int B, int G, int FMT>
 __device__ __forceinline__ double bh_row_partial(int32_t rs, int32_t re, int g, int b, const uint32_t* __restrict__ words,
                                                  const int32_t* __restrict__ col64, const double* __restrict__ val64,
                                                  const double* __restrict__ tab, int cb, const XT* __restrict__ x) {
+    // two entries per iteration, both loaded unconditionally (the second repeats the row's last entry past the
+    // end and adds a zero), in the order of k
     double acc = 0.0;
-    if constexpr (FMT == BH_FMT_PACKED) {
-        const uint32_t mask = (1u << cb) - 1u;
-        for (int32_t k = rs + g; k < re; k += G) {
-            const uint32_t w = words[k];
-            acc += tab[w & mask] * to_d(x[static_cast<int64_t>(w >> cb) * B + b]);
+    for (int32_t k = rs + g; k < re; k += 2 * G) {
+        const int32_t k1 = min(k + G, re - 1);
+        double p0, p1;
+        if constexpr (FMT == BH_FMT_PACKED) {
+            const uint32_t mask = (1u << cb) - 1u;
+            const uint32_t w0 = words[k], w1 = words[k1];
+            p0 = tab[w0 & mask] * to_d(x[static_cast<int64_t>(w0 >> cb) * B + b]);
+            p1 = tab[w1 & mask] * to_d(x[static_cast<int64_t>(w1 >> cb) * B + b]);
+        } else {
+            p0 = val64[k] * to_d(x[static_cast<int64_t>(col64[k]) * B + b]);
+            p1 = val64[k1] * to_d(x[static_cast<int64_t>(col64[k1]) * B + b]);
         }
-    } else {
-        for (int32_t k = rs + g; k < re; k += G) acc += val64[k] * to_d(x[static_cast<int64_t>(col64[k]) * B + b]);
+        acc += p0;
+        acc += k + G < re ? p1 : 0.0;
     }
     return acc;
 }
@@ -122,14 +130,16 @@ __device__ __forceinline__ void bh_rowblock_run(int64_t t0, int64_t t1, const in
         const int2 b0 = *reinterpret_cast<const int2*>(rb + 2 * t), b1 = *reinterpret_cast<const int2*>(rb + 2 * t + 2);
         const int32_t r0 = b0.x, ks = b0.y, r1 = b1.x, ke = b1.y;
         const int np = (ke - ks) * B;
-        double v[PER];
+        if (np > 0) {  // uniform
+            // Products past the block's end repeat its last one and go to LDS slots that phase 2 never reads
+            // (i < PER * WG = CAP always): every load and store is unconditional, so that all of a thread's word
+            // loads, then all of its gathers, are in flight together.
+            double v[PER];
 #pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            const int i = threadIdx.x + u * WG;
-            v[u] = 0.0;
-            if (i < np) {
-                const int32_t k = ks + i / B;
-                const int bb = i % B;
+            for (int u = 0; u < PER; ++u) {
+                const int ic = min(static_cast<int>(threadIdx.x) + u * WG, np - 1);
+                const int32_t k = ks + ic / B;
+                const int bb = ic % B;
                 if constexpr (FMT == BH_FMT_PACKED) {
                     const uint32_t w = words[k];
                     v[u] = tab[w & mask] * to_d(x[static_cast<int64_t>(w >> cb) * B + bb]);
@@ -137,11 +147,8 @@ __device__ __forceinline__ void bh_rowblock_run(int64_t t0, int64_t t1, const in
                     v[u] = val64[k] * to_d(x[static_cast<int64_t>(col64[k]) * B + bb]);
                 }
             }
-        }
 #pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            const int i = threadIdx.x + u * WG;
-            if (i < np) prod[i] = v[u];
+            for (int u = 0; u < PER; ++u) prod[threadIdx.x + u * WG] = v[u];
         }
         __syncthreads();
         const int nq = (r1 - r0) * B;
@@ -203,26 +210,24 @@ __device__ __forceinline__ void bh_rowlane_run(int64_t r0, int64_t r1, const int
             uint32_t w[U];
             double val[U];
             BhVec<XT, V> xv[U];
+            // Entries past the row's end read the row's last entry again (a cache hit) with a zero value, so
+            // every word load and gather of the unrolled step is unconditional and all of them are in flight
+            // together (a load under a branch is waited for before the next one is issued).  Such an entry
+            // adds a zero product (the x it gathers is one the row already uses).
             if constexpr (FMT == BH_FMT_PACKED) {
 #pragma unroll
-                for (int u = 0; u < U; ++u) w[u] = k + u < ke ? words[k + u] : 0u;
+                for (int u = 0; u < U; ++u) w[u] = words[min(k + u, ke - 1)];
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    if (k + u < ke) {
-                        xv[u] = bh_vload<V>(x + static_cast<int64_t>(w[u] >> cb) * B + b0);
-                    } else {
+                for (int u = 0; u < U; ++u) xv[u] = bh_vload<V>(x + static_cast<int64_t>(w[u] >> cb) * B + b0);
 #pragma unroll
-                        for (int v = 0; v < V; ++v) xv[u].v[v] = XT(0);
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u) val[u] = tab[w[u] & mask];
+                for (int u = 0; u < U; ++u) val[u] = k + u < ke ? tab[w[u] & mask] : 0.0;
             } else {
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    const bool ok = k + u < ke;
-                    w[u] = ok ? static_cast<uint32_t>(col64[k + u]) : 0u;
-                    val[u] = ok ? val64[k + u] : 0.0;
+                    const int32_t kk = min(k + u, ke - 1);
+                    w[u] = static_cast<uint32_t>(col64[kk]);
+                    const double vv = val64[kk];
+                    val[u] = k + u < ke ? vv : 0.0;
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) xv[u] = bh_vload<V>(x + static_cast<int64_t>(w[u]) * B + b0);

@@ -359,24 +359,36 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_b(LzArgs<XT> a) {
     if (blockIdx.x == 0 && threadIdx.x < B && act) a.st.alpha[j * B + b] = al;
     // acc[0]: ||v_{j+1}||^2;  acc[1 + l]: e_l = <y_l, w - alpha v_j>, from which the reduction forms
     // <y_l, v_{j+1}> = (e_l - c_l) / beta_{j+1} (the locked vectors are orthonormal)
-    double cl[LL], acc[1 + LM];
+    double acc[1 + LM];
 #pragma unroll
     for (int q = 0; q <= LM; ++q) acc[q] = 0.0;
-#pragma unroll
-    for (int l = 0; l < LL; ++l) {
-        cl[l] = 0.0;
-        if (l < nl) {
-            if (mode == 1) {
-                cl[l] = a.st.sums[(1 + l) * B + b] - al * a.st.sums[(1 + LM + l) * B + b];
-            } else {
-                const double bj = j > 0 ? a.st.beta[j * B + b] : 0.0;
-                cl[l] = (a.st.ltheta[l * B + b] - al) * a.st.dso[l * B + b] - bj * a.st.dso[(LZ_LMAX + l) * B + b];
+    // c_l of every (l, lane), one thread each, staged in LDS and read from there where it is used (registers
+    // stay free for the accumulators; workgroup 0 also records them)
+    __shared__ double cls[LL * B];
+    if constexpr (LM > 0) {
+        for (int q = threadIdx.x; q < LM * B; q += LZ_WG) {
+            const int bb = q % B, l = q / B, jj = a.st.step[bb];
+            const bool aa = a.st.active[bb] != 0 && jj < a.maxit;
+            const int nll = (aa && mode != 0) ? min(a.st.nlock[bb], LM) : 0;
+            double c = 0.0;
+            if (l < nll) {
+                const double all = a.st.sums[bb];
+                if (mode == 1) {
+                    c = a.st.sums[(1 + l) * B + bb] - all * a.st.sums[(1 + LM + l) * B + bb];
+                } else {
+                    const double bj = jj > 0 ? a.st.beta[jj * B + bb] : 0.0;
+                    c = (a.st.ltheta[l * B + bb] - all) * a.st.dso[l * B + bb] - bj * a.st.dso[(LZ_LMAX + l) * B + bb];
+                }
             }
+            cls[q] = c;
+            if (blockIdx.x == 0 && aa) a.st.dso[(2 * LZ_LMAX + l) * B + bb] = c;
         }
-        if (LM > 0 && blockIdx.x == 0 && threadIdx.x < B && act && l < LM) a.st.dso[(2 * LZ_LMAX + l) * B + b] = cl[l];
+        __syncthreads();
     }
     constexpr int ROWS = LZ_WG / B;
-    constexpr int UN = LM > 0 ? 2 : 4;  // rows per thread in flight
+    // rows per thread in flight: one with sixteen locked vectors (their accumulators and a group of loads in
+    // flight already take the registers; two rows per thread were slower at the chain m=12 sector)
+    constexpr int UN = LM > 4 ? 1 : (LM > 0 ? 2 : 4);
     const int64_t ntile = (a.D + ROWS - 1) / ROWS;
     const int64_t per = (ntile + gridDim.x - 1) / gridDim.x;
     const int64_t L = bh_xcd_tile(blockIdx.x, gridDim.x);
@@ -390,9 +402,14 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_b(LzArgs<XT> a) {
             for (int u = 0; u < UN; ++u) {
                 const int64_t r = (t + u) * ROWS + threadIdx.x / B;
                 ok[u] = (t + u < t1) && r < a.D;
-                idx[u] = ok[u] ? r * B + b : 0;
-                w[u] = ok[u] ? a.W[idx[u]] : 0.0;
-                v[u] = ok[u] ? to_d(vj[idx[u]]) : 0.0;
+                idx[u] = ok[u] ? r * B + b : 0;  // a valid element either way: the loads below are unconditional
+            }
+#pragma unroll
+            for (int u = 0; u < UN; ++u) {
+                const double wl = a.W[idx[u]];
+                const double vl = to_d(vj[idx[u]]);
+                w[u] = ok[u] ? wl : 0.0;
+                v[u] = ok[u] ? vl : 0.0;
             }
 #pragma unroll
             for (int u = 0; u < UN; ++u) w[u] -= al * v[u] * sj;
@@ -400,14 +417,29 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_b(LzArgs<XT> a) {
                 double w0[UN];
 #pragma unroll
                 for (int u = 0; u < UN; ++u) w0[u] = w[u];
+                // the locked vectors in groups of LG, the group's loads issued together (a vector l >= nl of a
+                // group reads slot nl - 1 again, a cache hit, and is discarded); the subtractions run over l in order
+                constexpr int LG = LL < 4 ? LL : 4;
 #pragma unroll
-                for (int l = 0; l < LL; ++l) {
-                    if (l >= nl) continue;
+                for (int l0 = 0; l0 < LL; l0 += LG) {
+                    if (l0 >= nl) break;
+                    XT y[LG][UN];
 #pragma unroll
-                    for (int u = 0; u < UN; ++u) {
-                        const double y = ok[u] ? to_d(a.Y[static_cast<int64_t>(l) * a.vs + idx[u]]) : 0.0;
-                        acc[1 + l] += y * w0[u];
-                        w[u] -= cl[l] * y;
+                    for (int q = 0; q < LG; ++q) {
+                        const int64_t off = static_cast<int64_t>(min(l0 + q, nl - 1)) * a.vs;
+#pragma unroll
+                        for (int u = 0; u < UN; ++u) y[q][u] = a.Y[off + idx[u]];
+                    }
+#pragma unroll
+                    for (int q = 0; q < LG; ++q) {
+#pragma unroll
+                        for (int u = 0; u < UN; ++u) {
+                            // every vector of the group is consumed (a value selected to zero for l >= nl adds
+                            // nothing), so that no load is sunk into a branch
+                            const double yd = (ok[u] && l0 + q < nl) ? to_d(y[q][u]) : 0.0;
+                            acc[1 + l0 + q] += yd * w0[u];
+                            w[u] -= cls[(l0 + q) * B + b] * yd;
+                        }
                     }
                 }
             }
@@ -511,55 +543,21 @@ __global__ void __launch_bounds__(LZ_RED_WG) k_lz_reduce(const double* __restric
 
 // R2: beta_{j+1} = ||v_{j+1}||, 1/beta, each active lane's step advanced; the carried products
 // <y_l, v_{j+1}> = (e_l - c_l) / beta_{j+1} and the Parlett-Scott estimates of the locked vectors; the next
-// step's orthogonalisation flag.  One workgroup: phase 1 reduces every value (coalesced reads, a fixed
-// order) into LDS; phase 2 updates the locked-vector
+// step's orthogonalisation flag.  K2's partial sums are first reduced by k_lz_reduce_wide (one workgroup per
+// value, fixed order) into `pre`; this one workgroup stages them in LDS, then updates the locked-vector
 // quantities with a thread per (vector, lane), then each lane's beta and step counter with a thread per lane
 // (the step counters are read by all of phase 2a before any is written).
 constexpr int LZ_BETA_MAXV = (1 + LZ_LMAX) * 64;
-constexpr int LZ_BETA_ACC = 8;  // independent partial sums per thread in the reduction
 
 template <int B>
-__global__ void __launch_bounds__(LZ_RED_WG) k_lz_beta(LzState st, const double* __restrict__ part, int nwg, int nv,
-                                                       int maxit, double eps, int so_always) {
+__global__ void __launch_bounds__(LZ_RED_WG) k_lz_beta(LzState st, const double* __restrict__ pre, int nv, int maxit,
+                                                       double eps, int so_always) {
     __shared__ double sums[LZ_BETA_MAXV];
-    __shared__ double red[LZ_RED_WG];
     __shared__ int want;
     const int so_cur = st.so[0], so_prev = st.so[1];
     const int nvv = so_cur ? nv : B;
     if (threadIdx.x == 0) want = 0;
-    // Values v0 .. v0 + nvc - 1 at a time: thread t sums value v0 + t % nvc over the rows g, g + G, ... with
-    // g = t / nvc, G = LZ_RED_WG / nvc, so that consecutive threads read consecutive values of one row
-    // (coalesced); the G partial sums of a value are then added in the order of g.
-    for (int v0 = 0; v0 < nvv; v0 += LZ_RED_WG) {
-        const int nvc = min(LZ_RED_WG, nvv - v0);
-        const int G = LZ_RED_WG / nvc;
-        const int t = threadIdx.x;
-        if (t < G * nvc) {
-            const int v = v0 + t % nvc, g = t / nvc;
-            double acc[LZ_BETA_ACC];
-#pragma unroll
-            for (int q = 0; q < LZ_BETA_ACC; ++q) acc[q] = 0.0;
-            for (int w0 = g; w0 < nwg; w0 += LZ_BETA_ACC * G) {
-#pragma unroll
-                for (int q = 0; q < LZ_BETA_ACC; ++q) {
-                    const int w = w0 + q * G;
-                    if (w < nwg) acc[q] += part[static_cast<size_t>(w) * nv + v];
-                }
-            }
-#pragma unroll
-            for (int h = LZ_BETA_ACC / 2; h >= 1; h >>= 1)
-#pragma unroll
-                for (int q = 0; q < h; ++q) acc[q] += acc[q + h];
-            red[t] = acc[0];
-        }
-        __syncthreads();
-        if (t < nvc) {
-            double s = 0.0;
-            for (int g = 0; g < G; ++g) s += red[g * nvc + t];
-            sums[v0 + t] = s;
-        }
-        __syncthreads();
-    }
+    for (int v = threadIdx.x; v < nvv; v += LZ_RED_WG) sums[v] = pre[v];
     __syncthreads();
     // phase 2a: one thread per (locked vector l, lane b), all in parallel
     for (int q = threadIdx.x; q < LZ_LMAX * B; q += LZ_RED_WG) {
@@ -608,10 +606,10 @@ __global__ void __launch_bounds__(LZ_RED_WG) k_lz_beta(LzState st, const double*
 // (v < nmin) unless the step is exact (then also the locked-vector sums).
 template <int B>
 __global__ void __launch_bounds__(LZ_RED_WG) k_lz_reduce_wide(LzState st, const double* __restrict__ part, int nwg,
-                                                              int nv, int nmin, double* __restrict__ out) {
+                                                              int nv, int nmin, int so_any, double* __restrict__ out) {
     __shared__ double red[LZ_RED_WG / 64];
     const int v = blockIdx.x;
-    const int nvv = st.so[0] == 1 ? nv : nmin;
+    const int nvv = (so_any ? st.so[0] != 0 : st.so[0] == 1) ? nv : nmin;
     if (v >= nvv) return;  // uniform over the workgroup
     double s = 0.0;
     for (int w = threadIdx.x; w < nwg; w += LZ_RED_WG) s += part[static_cast<size_t>(w) * nv + v];
@@ -715,8 +713,11 @@ __global__ void __launch_bounds__(256) k_lz_ritz(const XT* __restrict__ V, int64
             for (int q = 0; q < LZ_RZ_T; ++q)
 #pragma unroll
                 for (int u = 0; u < LZ_RZ_R; ++u) {
+                    // unconditional loads of a valid element (selected to zero past the end), so that all of them
+                    // are in flight together
                     const int64_t idx = base + u * 256;
-                    v[q][u] = idx < vs ? V[static_cast<int64_t>(t + q) * vs + idx] : XT(0);
+                    const XT x = V[static_cast<int64_t>(t + q) * vs + min(idx, vs - 1)];
+                    v[q][u] = idx < vs ? x : XT(0);
                 }
 #pragma unroll
             for (int q = 0; q < LZ_RZ_T; ++q) {
@@ -734,7 +735,8 @@ __global__ void __launch_bounds__(256) k_lz_ritz(const XT* __restrict__ V, int64
 #pragma unroll
             for (int u = 0; u < LZ_RZ_R; ++u) {
                 const int64_t idx = base + u * 256;
-                v[u] = idx < vs ? to_d(V[static_cast<int64_t>(t) * vs + idx]) : 0.0;
+                const XT x = V[static_cast<int64_t>(t) * vs + min(idx, vs - 1)];
+                v[u] = idx < vs ? to_d(x) : 0.0;
             }
             const double* ct = cs + (t - t0) * per_t + b;
 #pragma unroll
@@ -1188,8 +1190,11 @@ class LzEngine final : public LzEngineBase {
         mult = !o.distinct;
         ymax = lz_ymax(D, o);
         {
+            // Step graphs are off under rocprofv3 (it exports ROCPROF_OUTPUT_PATH to the application): its kernel
+            // tracing of graph launches ended two runs with a host SIGSEGV inside hipGraphLaunch.  BHX_GRAPH=0/1
+            // forces them off/on.
             const char* e = std::getenv("BHX_GRAPH");
-            use_graphs = !(e && std::strcmp(e, "0") == 0);
+            use_graphs = e ? std::strcmp(e, "0") != 0 : std::getenv("ROCPROF_OUTPUT_PATH") == nullptr;
         }
         chunk = o.chunk > 0 ? o.chunk : 16;
         tol = o.tol > 0 ? o.tol : 1e-10;
@@ -1249,6 +1254,7 @@ class LzEngine final : public LzEngineBase {
             BH_TRY(dalloc(ctx, part, std::max(n1, n2)));
         }
         BH_TRY(dalloc(ctx, dnorm, static_cast<size_t>(2 * std::max(LZ_CMAX, ymax) * B)));
+        BH_TRY(dalloc(ctx, sums2, static_cast<size_t>(LZ_BETA_MAXV)));
         BH_TRY(dalloc(ctx, dnj, B));
         BH_TRY(dalloc(ctx, lock_map, static_cast<size_t>(std::max(LZ_CMAX, ymax)) * B));
         BH_TRY(dalloc(ctx, dno, B));
@@ -1598,7 +1604,7 @@ class LzEngine final : public LzEngineBase {
                     const double th = an.theta[lev[q]];
                     const bool below = th <= kappa + kmarg;
                     double thr = cthr[q];
-                    if (!below && q == 0 && R.pass > 0) thr = std::fmax(thr, 1e-4 * (th - kappa));
+                    if (!below && q == 0 && R.pass > 0) thr = std::fmax(thr, 1e-2 * (th - kappa));
                     if ((below || q == 0) && lbound[q] >= thr && !exhausted) conv = false;
                 }
             }
@@ -1635,6 +1641,9 @@ class LzEngine final : public LzEngineBase {
                 R.status = conv && !capped && (mult ? (nmerged >= k || exhausted) : static_cast<int>(lev.size()) == k)
                                ? BH_OK
                                : BH_ENOCONV;
+                if (lz_debug())
+                    std::fprintf(stderr, "[lz] done lane=%d tot=%d conv=%d capped=%d nmerged=%d exhausted=%d kappa=%.10f\n", b,
+                                 tj, conv ? 1 : 0, capped ? 1 : 0, nmerged, exhausted ? 1 : 0, kappa);
                 R.theta.clear();
                 R.s.clear();
                 for (int i : cand) {
@@ -2005,6 +2014,12 @@ class LzEngine final : public LzEngineBase {
             if (MIXED && dropped && L.status == BH_OK) L.status = BH_ENOCONV;
             // fewer than k levels is an answer only when the Krylov space was exhausted
             if (L.status == BH_OK && static_cast<int>(lv.size()) < k && !R.exhausted) L.status = BH_ENOCONV;
+            if (lz_debug()) {
+                std::fprintf(stderr, "[lz] finish lane=%d run_status=%d status=%d dropped=%d levels=%d frozen=%d pass=%d:", b,
+                             R.status, L.status, dropped ? 1 : 0, static_cast<int>(lv.size()), R.nfz, R.pass);
+                for (const auto& qq : lv0) std::fprintf(stderr, " (%.10f r=%.1e)", qq.first, rs_of(qq));
+                std::fprintf(stderr, "\n");
+            }
             if (want_vectors) L.vecs.assign(static_cast<size_t>(k) * D, 0.0);
             for (int i = 0; i < k && i < static_cast<int>(lv.size()); ++i) {
                 const double ev = lv[i].first;
@@ -2322,7 +2337,7 @@ class LzEngine final : public LzEngineBase {
         {
             KTimer kt(ctx, "lanczos_reduce", s);
             const int nv = (1 + 2 * LM) * B;
-            hipLaunchKernelGGL((k_lz_reduce_wide<B>), dim3(nv), dim3(LZ_RED_WG), 0, s, st, part.p, grid_k1, nv, B, st.sums);
+            hipLaunchKernelGGL((k_lz_reduce_wide<B>), dim3(nv), dim3(LZ_RED_WG), 0, s, st, part.p, grid_k1, nv, B, 0, st.sums);
         }
         {
             KTimer kt(ctx, "lanczos_update", s);
@@ -2341,8 +2356,11 @@ class LzEngine final : public LzEngineBase {
         }
         {
             KTimer kt(ctx, "lanczos_reduce", s);
-            hipLaunchKernelGGL((k_lz_beta<B>), dim3(1), dim3(LZ_RED_WG), 0, s, st, part.p, grid_k2, (1 + LM) * B, cap,
-                               eps_store, lz_so_always());
+            const int nv2 = (1 + LM) * B;
+            hipLaunchKernelGGL((k_lz_reduce_wide<B>), dim3(nv2), dim3(LZ_RED_WG), 0, s, st, part.p, grid_k2, nv2, B, 1,
+                               sums2.p);
+            hipLaunchKernelGGL((k_lz_beta<B>), dim3(1), dim3(LZ_RED_WG), 0, s, st, sums2.p, nv2, cap, eps_store,
+                               lz_so_always());
         }
     }
     // C holds rows t < max(njb) of [(t * nv + i) * B + b]
@@ -2354,6 +2372,7 @@ class LzEngine final : public LzEngineBase {
         BH_TRY(dC.alloc(ctx, C.size()));
         BH_HIP(ctx, hipMemcpyAsync(dC.p, C.data(), C.size() * 8, hipMemcpyHostToDevice, s));
         BH_HIP(ctx, hipMemcpyAsync(dnj.p, njb.data(), B * sizeof(int), hipMemcpyHostToDevice, s));
+        if (lz_debug()) std::fprintf(stderr, "[lz] ritz njmax=%d outputs=%d out_bytes=%d\n", njmax, nv, static_cast<int>(sizeof(OT)));
         {
             KTimer kt(ctx, "lanczos_ritz", s);
             const int64_t per_wg = 256 * LZ_RZ_R;
@@ -2710,7 +2729,7 @@ class LzEngine final : public LzEngineBase {
     double* W = nullptr;
     double* r0 = nullptr;
     DevArray<XT> V_own, Y_own, Xc_own;
-    DevArray<double> W_own, r0_own, st_d, part, dC, dnorm, dd;
+    DevArray<double> W_own, r0_own, st_d, part, sums2, dC, dnorm, dd;
     DevArray<int> st_i, dnj, dno, lock_map;
     size_t n_alpha = 0, n_beta = 0, bytes = 0;
     LzState st{};
