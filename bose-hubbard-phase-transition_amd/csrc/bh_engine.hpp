@@ -19,6 +19,8 @@ struct LzProfile {
     double wait_ms = 0.0;    // host waiting for the device at the analyses
     double host_ms = 0.0;    // host side of the analyses: decisions, locks, restarts, retirements
     double analysis_ms = 0.0;  // of which the tridiagonal analyses of the lanes
+    double deflate_ms = 0.0;   // of which the deflations (Ritz vectors frozen, fresh starts)
+    double lock_ms = 0.0;      // of which the locks (locked Ritz vectors formed)
     double finish_ms = 0.0;  // FP64 finish, wall time
     double run_ms = 0.0;     // enqueueing steps, graph captures included
     double capture_ms = 0.0; // capturing and instantiating step graphs

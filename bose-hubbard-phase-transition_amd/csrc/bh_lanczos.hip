@@ -5,7 +5,8 @@
 // sparse product per step.  Here, per Lanczos step j, four launches on one stream and no host
 // round trip:
 //   K1  w = H v_j - beta_j v_{j-1} (one CSR pass for all B vectors, fused with the diagonal and the
-//       recurrence), written to W in FP64; per-workgroup partials of alpha_j = <v_j, w> and, for the
+//       recurrence), written to W shifted by alpha_{j-1} v_j in storage precision (lz_sigma); per-workgroup
+//       partials of alpha_j = <v_j, w> (FP64, unshifted) and, for the
 //       locked Ritz vectors y_l, of <y_l, w> and <y_l, v_j>
 //   R1  the partials summed in a fixed order (one wave per value)
 //   K2  w'' = w - alpha_j v_j - sum_l c_l y_l with c_l = <y_l, w> - alpha_j <y_l, v_j>, stored as
@@ -22,7 +23,7 @@
 // Start vector: Spectra's Park-Miller LCG r (SimpleRandom.h:30-96) normalised, v0 = r / ||r||.
 // Spectra projects it first (v0 = A r / ||A r||, Arnoldi.h:147-154); that projection removes the null
 // space of H, and at J = 0 the unit-filling Mott state has E = 0 exactly, so it is not applied.
-// Mixed precision: Krylov and locked vectors stored FP32, W and every reduction FP64; the matrix
+// Mixed precision: Krylov and locked vectors and the shifted W stored FP32, every reduction FP64; the matrix
 // values come exactly from the FP64 table of the packed format.
 #include <algorithm>
 #include <chrono>
@@ -94,11 +95,21 @@ struct LzArgs {
     XT* V;         // [slots][D][B] Krylov basis; lane b's vector t at V[t][r][b]
     XT* Xc;        // [D][B] current Lanczos vector of every lane (the operand of the gathers)
     int64_t vs;    // D * B
-    double* W;     // [D][B]
+    XT* W;         // [D][B] w - sigma_b x_b in storage precision (sigma_b = alpha_{j-1}, see lz_sigma)
     XT* Y;         // [LZ_LMAX][D][B] locked Ritz vectors (unit norm)
     int maxit;     // slots - 1: a lane stops at this step until it is restarted or retired
     LzState st;
 };
+
+// Shift of the stored operator image: K1 stores w - sigma_b x_b (x_b = s_j v_j) with sigma_b = alpha_{j-1} (0 at
+// j = 0), which is of the order of the next beta where w itself is of the order of |alpha_j|, so that the
+// image loses no more to its storage rounding than v_{j+1} does; K2 and the exact-step dot products add
+// sigma_b x_b back (alpha_j itself is summed from the unshifted w).  Any sigma would do, as long as K1, K2 and
+// the dot products of one step read the same one.
+template <int B>
+__device__ __forceinline__ double lz_sigma(const LzState& st, int b, int j) {
+    return j > 0 ? st.alpha[(j - 1) * B + b] : 0.0;
+}
 
 // Per-workgroup sum of one value per vector b (b = lane % B), stored at part[wg * nv + q * B + b].
 template <int B>
@@ -172,7 +183,7 @@ __device__ __forceinline__ void lz_block_put_v(const double (&x)[V], int q, doub
 // whether the lane takes the step.
 template <int B>
 struct K1Shared {
-    double sj[B], jbs[B], ub[B], bj[B];
+    double sj[B], jbs[B], ub[B], bj[B], sg[B];
     int64_t om[B];
     int act[B];
 };
@@ -201,17 +212,17 @@ struct K1Epi {
 #pragma unroll
             for (int v = 0; v < V; ++v) xm.v[v] = a.V[ls.om[b0 + v] + idx + v];
         }
-        BhVec<double, V> wv;
+        BhVec<XT, V> wv;
 #pragma unroll
         for (int v = 0; v < V; ++v) {
             const int b = b0 + v;
             const double hx = (FMT == BH_FMT_PACKED) ? -acc[v] : acc[v];
             const double xr = to_d(xv.v[v]) * ls.sj[b];
             const double w = ls.jbs[b] * hx + ls.ub[b] * d * xr - ls.bj[b] * to_d(xm.v[v]);
-            wv.v[v] = w;
+            wv.v[v] = static_cast<XT>(w - ls.sg[b] * xr);
             if (ls.act[b]) pa[v] += xr * w;
         }
-        *reinterpret_cast<BhVec<double, V>*>(a.W + idx) = wv;
+        *reinterpret_cast<BhVec<XT, V>*>(a.W + idx) = wv;
     }
 };
 
@@ -226,6 +237,7 @@ __device__ __forceinline__ void k1_setup(const LzArgs<XT>& a, K1Shared<B>& ls, K
         ls.jbs[b] = a.st.J[b] * sj;
         ls.ub[b] = a.st.U[b];
         ls.bj[b] = j > 0 ? a.st.beta[j * B + b] * a.st.scale[(j - 1) * B + b] : 0.0;
+        ls.sg[b] = lz_sigma<B>(a.st, b, j);
         ls.om[b] = static_cast<int64_t>(j > 0 ? j - 1 : 0) * a.vs;
         ls.act[b] = a.st.active[b] != 0 && j < a.maxit;
     }
@@ -309,7 +321,7 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_sodots(LzArgs<XT> a, int64_t rows_
     const int j = a.st.step[b];
     const bool act = a.st.active[b] != 0 && j < a.maxit;
     const int nl = act ? min(a.st.nlock[b], LM) : 0;
-    const double sj = a.st.scale[j * B + b];
+    const double sj = a.st.scale[j * B + b], sg = lz_sigma<B>(a.st, b, j);
     double acc[2 * LM];
 #pragma unroll
     for (int q = 0; q < 2 * LM; ++q) acc[q] = 0.0;
@@ -318,7 +330,7 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_sodots(LzArgs<XT> a, int64_t rows_
     if (nl > 0) {
         for (int64_t r = r0 + threadIdx.x / B; r < r1; r += LZ_WG / B) {
             const int64_t idx = r * B + b;
-            const double w = a.W[idx], xr = to_d(a.Xc[idx]) * sj;
+            const double xr = to_d(a.Xc[idx]) * sj, w = to_d(a.W[idx]) + sg * xr;
 #pragma unroll
             for (int l = 0; l < LM; ++l) {
                 if (l < nl) {
@@ -353,7 +365,7 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_b(LzArgs<XT> a) {
     XT* vn = a.V + static_cast<int64_t>(j + 1) * a.vs;
     const bool act = a.st.active[b] != 0 && j < a.maxit;
     const double sj = a.st.scale[j * B + b];
-    const double al = a.st.sums[b];
+    const double al = a.st.sums[b], sg = lz_sigma<B>(a.st, b, j);
     const int mode = a.st.so[0];
     const int nl = (act && LM > 0 && mode != 0) ? min(a.st.nlock[b], LM) : 0;
     if (blockIdx.x == 0 && threadIdx.x < B && act) a.st.alpha[j * B + b] = al;
@@ -406,13 +418,13 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_b(LzArgs<XT> a) {
             }
 #pragma unroll
             for (int u = 0; u < UN; ++u) {
-                const double wl = a.W[idx[u]];
+                const double wl = to_d(a.W[idx[u]]);
                 const double vl = to_d(vj[idx[u]]);
                 w[u] = ok[u] ? wl : 0.0;
                 v[u] = ok[u] ? vl : 0.0;
             }
 #pragma unroll
-            for (int u = 0; u < UN; ++u) w[u] -= al * v[u] * sj;
+            for (int u = 0; u < UN; ++u) w[u] -= (al - sg) * v[u] * sj;
             if constexpr (LM > 0) {
                 double w0[UN];
 #pragma unroll
@@ -1309,7 +1321,7 @@ class LzEngine final : public LzEngineBase {
                        V,
                        Xc,
                        vs,
-                       W,
+                       reinterpret_cast<XT*>(W),  // the FP64 buffer holds the narrower W of the steps
                        Y,
                        cap,
                        st};
@@ -2460,6 +2472,12 @@ class LzEngine final : public LzEngineBase {
     // forms the Ritz vectors new_locks[b] and stores them in Y: appended (lock_slot -1) or replacing slot
     int lock(const std::vector<std::vector<int>>& new_locks, const std::vector<std::vector<int>>& lock_slot,
              const std::vector<Analysis>& ans) {
+        const auto tl0 = std::chrono::steady_clock::now();
+        struct Clock {
+            double& acc;
+            std::chrono::steady_clock::time_point t;
+            ~Clock() { acc += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count(); }
+        } clk{prof.lock_ms, tl0};
         int nv = 0, jmax = 0;
         for (int b = 0; b < B; ++b)
             if (!new_locks[b].empty()) {
@@ -2575,6 +2593,11 @@ class LzEngine final : public LzEngineBase {
     // the lane's locked set at the start of the pass (the pass's own locks are appended after them); under
     // full reorthogonalisation every step projects them out (k_lz_fr_dots / k_lz_fr_axpy).
     int deflate(const std::vector<int>& flag) {
+        struct Clock {
+            double& acc;
+            std::chrono::steady_clock::time_point t;
+            ~Clock() { acc += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count(); }
+        } clk{prof.deflate_ms, std::chrono::steady_clock::now()};
         int nv = 0, jmax = 0;
         for (int b = 0; b < B; ++b)
             if (flag[b]) {
@@ -2874,17 +2897,24 @@ static int lanczos_batch(bh_ctx* ctx, bh_ham* h, int prec, const double* J, cons
     const auto tf0 = std::chrono::steady_clock::now();
     BH_TRY(e->finish(all, out, vecs != nullptr));
     e->prof.finish_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tf0).count();
+    const bool used_fp64 = e->fp64();
     static const bool prof_on = [] {
         const char* v = std::getenv("BHX_LZ_PROFILE");
         return v && std::strcmp(v, "1") == 0;
     }();
-    if (prof_on)
+    if (prof_on) {
+        const LzProfile pr = e->prof;
+        const bool f64 = used_fp64;
+        const double total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts0).count();
+        const auto td0 = std::chrono::steady_clock::now();
+        e.reset();
+        const double teardown = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - td0).count();
         std::fprintf(stderr,
                      "[lz-profile] D=%lld lanes=%d fp64=%d checks=%d setup=%.1f ms run=%.1f ms (capture %.1f) wait=%.1f ms "
-                     "host=%.1f ms (analysis %.1f) finish=%.1f ms\n",
-                     static_cast<long long>(h->D), nreal, e->fp64() ? 1 : 0, e->prof.checks, e->prof.setup_ms,
-                     e->prof.run_ms, e->prof.capture_ms, e->prof.wait_ms, e->prof.host_ms, e->prof.analysis_ms,
-                     e->prof.finish_ms);
+                     "host=%.1f ms (analysis %.1f, deflate %.1f, lock %.1f) finish=%.1f ms total=%.1f ms teardown=%.1f ms\n",
+                     static_cast<long long>(h->D), nreal, f64 ? 1 : 0, pr.checks, pr.setup_ms, pr.run_ms, pr.capture_ms,
+                     pr.wait_ms, pr.host_ms, pr.analysis_ms, pr.deflate_ms, pr.lock_ms, pr.finish_ms, total, teardown);
+    }
     const int k = o.k;
     if (vecs) vecs->assign(static_cast<size_t>(nreal) * k * h->D, 0.0);
     for (int b = 0; b < nreal; ++b) {
@@ -2895,7 +2925,7 @@ static int lanczos_batch(bh_ctx* ctx, bh_ham* h, int prec, const double* J, cons
         }
         if (res->iters) res->iters[b] = out[b].iters;
         if (res->status) res->status[b] = out[b].status;
-        if (res->used_fp64) res->used_fp64[b] = e->fp64() ? 1 : 0;
+        if (res->used_fp64) res->used_fp64[b] = used_fp64 ? 1 : 0;
         if (vecs && !out[b].vecs.empty())
             std::copy(out[b].vecs.begin(), out[b].vecs.end(), vecs->begin() + static_cast<int64_t>(b) * k * h->D);
     }
@@ -2928,6 +2958,7 @@ namespace {
 // Krylov convention: the start vector's component in each level's eigenspace, normalised.
 int solve_diagonal(bh_ctx* ctx, bh_ham* h, const std::vector<int>& runs, const double* U, const bh_lanczos_opts& o,
                    bh_lanczos_result* res, std::vector<double>* vecs) {
+    if (runs.empty()) return BH_OK;  // (the diagonal's copy to the host below took ~95 ms at D = 1.35e6)
     const int k = o.k;
     std::vector<float> d(static_cast<size_t>(h->D));
     BH_HIP(ctx, hipMemcpyAsync(d.data(), h->diag, d.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
