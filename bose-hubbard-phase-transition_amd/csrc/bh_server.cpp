@@ -122,9 +122,42 @@ int lanes_for(bh_server* S, int64_t D, int prec) {
     return L;
 }
 
-// evicts idle sectors (no queue, no busy lane) least recently used first until `need` more bytes fit
-void evict_for(bh_server* S, size_t need, Sector* keep) {
+// releases one sector engine: its stream is drained first (a check may have left speculative steps on it)
+void release_engine(bh_server* S, Sector& sc, int p) {
+    if (sc.stream[p]) (void)hipStreamSynchronize(sc.stream[p]);
+    if (sc.eng[p]) S->engines -= 1;
+    sc.eng[p].reset();
+    sc.eng_bytes[p] = 0;
+    sc.inflight[p] = false;
+    if (sc.stream[p]) (void)hipStreamDestroy(sc.stream[p]);
+    sc.stream[p] = nullptr;
+}
+
+// Frees memory until `need` more bytes fit: first idle engines (no busy lane) with nothing queued for them,
+// any sector's, least recently used first (an engine is rebuilt on demand; without this step two sectors
+// whose FP64 reruns wait for memory while each holds an idle mixed engine would wait for each other
+// forever), then idle sectors (no queue, no busy lane) whole.  `keep` / `keep_p` name the engine being made.
+void evict_for(bh_server* S, size_t need, Sector* keep, int keep_p = -1) {
     const size_t budget = S->opts.hbm_budget;
+    while (server_used(S) + need > budget) {
+        Sector* victim = nullptr;
+        int vp = -1;
+        for (auto& kv : S->sectors) {
+            Sector* sc = kv.second.get();
+            for (int p = 0; p < 2; ++p) {
+                if ((sc == keep && p == keep_p) || !sc->eng[p] || !sc->queue[p].empty()) continue;
+                bool busy = false;
+                for (char c : sc->lane_busy[p]) busy |= c != 0;
+                if (busy) continue;
+                if (!victim || sc->last_use < victim->last_use) {
+                    victim = sc;
+                    vp = p;
+                }
+            }
+        }
+        if (!victim) break;
+        release_engine(S, *victim, vp);
+    }
     while (server_used(S) + need > budget) {
         Sector* victim = nullptr;
         for (auto& kv : S->sectors) {
@@ -136,19 +169,29 @@ void evict_for(bh_server* S, size_t need, Sector* keep) {
             if (busy) continue;
             if (!victim || sc->last_use < victim->last_use) victim = sc;
         }
-        if (!victim) return;
-        for (int p = 0; p < 2; ++p) {
-            // a check may have left speculative steps on the stream: drain it before the engine's graphs,
-            // its buffers and the stream itself are released
-            if (victim->stream[p]) (void)hipStreamSynchronize(victim->stream[p]);
-            if (victim->eng[p]) S->engines -= 1;
-            victim->eng[p].reset();
-            if (victim->stream[p]) (void)hipStreamDestroy(victim->stream[p]);
-            victim->stream[p] = nullptr;
-        }
+        if (!victim) break;
+        for (int p = 0; p < 2; ++p) release_engine(S, *victim, p);
         if (victim->ham) bh_ham_destroy(victim->ham);
         if (victim->basis) bh_basis_destroy(victim->basis);
         S->sectors.erase(victim->key);
+    }
+    // Still short and no lane anywhere busy (nothing would ever free memory): the other sectors' CSRs are
+    // released too, their queues kept (a sector builds its CSR again when it is served), so that the sector
+    // being served always progresses.
+    if (server_used(S) + need <= budget) return;
+    for (auto& kv : S->sectors)
+        for (int p = 0; p < 2; ++p)
+            for (char c : kv.second->lane_busy[p])
+                if (c) return;
+    for (auto& kv : S->sectors) {
+        Sector* sc = kv.second.get();
+        if (sc == keep || !sc->ham || server_used(S) + need <= budget) continue;
+        for (int p = 0; p < 2; ++p) release_engine(S, *sc, p);
+        bh_ham_destroy(sc->ham);
+        sc->ham = nullptr;
+        if (sc->basis) bh_basis_destroy(sc->basis);
+        sc->basis = nullptr;
+        sc->ham_bytes = 0;
     }
 }
 
@@ -201,7 +244,7 @@ int ensure_engine(bh_server* S, Sector& sc, int p) {
     const int L = lanes_wanted(S, sc, p);
     const size_t need = bh_engine_bytes(sc.ham->D, prec, L, lo) +
                         (prec == BH_PREC_FP64 ? static_cast<size_t>(sc.ham->nnz) * 12 : 0);
-    evict_for(S, need, &sc);
+    evict_for(S, need, &sc, p);
     if (server_used(S) + need > S->opts.hbm_budget) {
         if (need + sc.ham_bytes > S->opts.hbm_budget) sc.never_fits = true;
         return BH_EBUDGET;  // wait for lanes elsewhere to drain
@@ -284,12 +327,34 @@ void admit_queued(bh_server* S, Sector& sc, int p) {
 
 // Analyses a sector engine whose chunk has completed: retires converged lanes, completes their requests or
 // queues their reruns (FP64, then FP64 with full reorthogonalisation).
+// An engine call failed (a device allocation of its analysis or finish over the budget, a HIP error): every
+// request on its lanes completes with that status and the engine is released (its state is not trusted), so
+// that the worker never spins on an engine that cannot advance.
+void fail_engine(bh_server* S, Sector& sc, int p, int rc) {
+    const int k = S->opts.lanczos.k;
+    for (size_t l = 0; l < sc.lane_busy[p].size(); ++l) {
+        if (!sc.lane_busy[p][l]) continue;
+        sc.lane_busy[p][l] = 0;
+        complete(S, sc.lane_req[p][l], rc, nullptr, nullptr, k, 0, p, sc.lane_start[p][l]);
+    }
+    release_engine(S, sc, p);
+}
+
 void harvest(bh_server* S, Sector& sc, int p) {
     const int k = S->opts.lanczos.k;
     std::vector<int> fin;
-    if (sc.eng[p]->check(fin) != BH_OK || fin.empty()) return;
+    int rc = sc.eng[p]->check(fin);
+    if (rc != BH_OK) {
+        fail_engine(S, sc, p, rc);
+        return;
+    }
+    if (fin.empty()) return;
     std::vector<LaneResult> res;
-    if (sc.eng[p]->finish(fin, res, false) != BH_OK) return;
+    rc = sc.eng[p]->finish(fin, res, false);
+    if (rc != BH_OK) {
+        fail_engine(S, sc, p, rc);
+        return;
+    }
     for (size_t i = 0; i < fin.size(); ++i) {
         const int lane = fin[i];
         Pending pd = sc.lane_req[p][lane];
@@ -369,8 +434,13 @@ void worker_loop(bh_server* S) {
                         harvest(S, sc, p);
                     }
                     admit_queued(S, sc, p);
-                    if (!(sc.eng[p] && sc.eng[p]->active() > 0 && sc.eng[p]->run(sc.eng[p]->suggest_chunk()) == BH_OK))
+                    if (!(sc.eng[p] && sc.eng[p]->active() > 0)) break;
+                    const int rc = sc.eng[p]->run(sc.eng[p]->suggest_chunk());
+                    if (rc != BH_OK) {
+                        fail_engine(S, sc, p, rc);
+                        progressed = true;
                         break;
+                    }
                     sc.inflight[p] = true;
                     progressed = true;
                     if (!small || std::chrono::duration<double, std::milli>(Clock::now() - t0).count() > SRV_SMALL_MS) break;
@@ -399,7 +469,12 @@ int bh_server_create(bh_ctx* ctx, const bh_server_opts* opts, bh_server** out) {
         return BH_EHIP;
     }
     if (!S->opts.hbm_budget) S->opts.hbm_budget = static_cast<size_t>(0.8 * static_cast<double>(freeb));
-    int rc = bh_ctx_create(ctx->device, S->opts.hbm_budget + (static_cast<size_t>(1) << 30), &S->ctx);
+    // The budget admits CSRs and engines; the context also holds the transient buffers of an engine's
+    // analysis, deflations and finish (one at a time: the worker is one thread), up to ~2 (2k + 8) vectors of
+    // the largest engine -- a few GB at the engine cap -- so it gets that much on top of the budget.
+    const size_t slack = std::min<size_t>(static_cast<size_t>(16) << 30,
+                                          std::max<size_t>(static_cast<size_t>(1) << 30, freeb / 10));
+    int rc = bh_ctx_create(ctx->device, S->opts.hbm_budget + slack, &S->ctx);
     if (rc != BH_OK) {
         delete S;
         return rc;

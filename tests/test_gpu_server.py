@@ -133,3 +133,29 @@ def test_alone_equals_in_mix_mixed_lane(ctx):
     a, b = np.array(alone.evals[:4]), np.array(mixed.evals[:4])
     assert abs(a[0] - b[0]) <= 1e-10 * abs(a[0])
     assert np.all(np.abs(a - b) <= 1e-8 * np.maximum(np.abs(a), 1.0)), (a, b)
+
+
+def test_server_budget_contention(ctx):
+    """Three sectors under a budget that holds one sector's engines at a time, every mixed result rejected (an
+    acceptance bound no solve meets), so that each sector's FP64 reruns wait for memory while the other sectors
+    hold idle mixed engines and CSRs: the server must release those (an idle engine with nothing queued for
+    it, and when nothing runs, other sectors' CSRs) instead of waiting for memory that never frees.  Every
+    request completes, from an FP64 solve, with E0 within 1e-9 of a direct FP64 solve."""
+    sectors = [("chain", 9, 9), ("chain", 10, 8), ("chain", 8, 10)]  # D = 24,310 / 24,310 / 19,448
+    rng = np.random.default_rng(5)
+    reqs = []
+    with bhx.Server(ctx, lanes=4, hbm_budget=160 << 20, ncv=64, accept_rel=1e-300) as srv:
+        for g, m, n in sectors:
+            for _ in range(4):
+                J = float(10 ** rng.uniform(-1.2, -0.5))
+                reqs.append((srv.submit(g, n, J, 1.0, 0.0, m=m), m, n, J))
+        assert srv.drain(300.0), srv.stats().in_flight
+        for rid, m, n, J in reqs:
+            r = srv.poll(rid)
+            assert r is not None and r.status == 0 and r.used_fp64 == 1
+            b = ctx.basis(m, n)
+            h = b.hamiltonian(*bhx.lattice("chain", m=m))
+            want = h.lanczos(J, 1.0, k=4, prec=bhx.PREC_FP64).evals[0]
+            assert abs(r.evals[0] - want[0]) <= 1e-9 * abs(want[0])
+            h.close()
+            b.close()
