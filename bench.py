@@ -332,13 +332,26 @@ def stream(srv, reqs, rate, rng):
     return res, span, t_sub, qs, ok
 
 
-def stream_summary(res, span, rate, t_sub, qs, ok):
+# sectors up to this dimension are solved in FP64 with full reorthogonalisation by design (bh_lanczos_opts.fullre_d)
+FULLRE_D = 16384
+
+
+def request_dim(q):
+    g, m, Lx, Ly, n = q[:5]
+    return math.comb((m if g == "chain" else Lx * Ly) + n - 1, n)
+
+
+def stream_summary(res, span, rate, t_sub, qs, ok, reqs):
     lat = np.array([r.t_done - r.t_submit for r in res if r is not None])
+    big = [request_dim(q) > FULLRE_D for q in reqs]
     return {"requests": len(res), "offered_rps": rate, "completed": int(sum(r is not None for r in res)),
             "failed": int(sum(1 for r in res if r is None or r.status != 0)), "drained": bool(ok),
             "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
             "solves_per_s": float(len(lat) / span), "submit_s": t_sub, "span_s": span,
             "fp64_results": int(sum(1 for r in res if r is not None and r.used_fp64)),
+            "fp64_by_design": int(sum(1 for r, bg in zip(res, big) if r is not None and not bg)),
+            "mixed_lane_requests": int(sum(big)),
+            "fp64_reruns_of_mixed": int(sum(1 for r, bg in zip(res, big) if r is not None and bg and r.used_fp64)),
             "queue_first_half_mean": float(np.mean([q for t, q in qs if t < t_sub / 2])) if qs else None,
             "queue_second_half_mean": float(np.mean([q for t, q in qs if t >= t_sub / 2])) if qs else None}
 
@@ -351,14 +364,14 @@ def serving(ctx, nreq, rate, sustained_n, seed=7):
     reqs, rng = request_stream(nreq, seed)
     with bhx.Server(ctx, lanes=16, k=4, ncv=160) as srv:
         res, span, t_sub, qs, ok = stream(srv, reqs, rate, rng)
-        cold = stream_summary(res, span, rate, t_sub, qs, ok)
+        cold = stream_summary(res, span, rate, t_sub, qs, ok, reqs)
         log("serving cold:", json.dumps(cold))
         sus = None
         if sustained_n > 0:
             r2 = 0.7 * cold["solves_per_s"]
             reqs2, rng2 = request_stream(sustained_n, seed + 1)
             res2, span2, t_sub2, qs2, ok2 = stream(srv, reqs2, r2, rng2)
-            sus = stream_summary(res2, span2, r2, t_sub2, qs2, ok2)
+            sus = stream_summary(res2, span2, r2, t_sub2, qs2, ok2, reqs2)
             log("serving sustained:", json.dumps(sus))
     return {"cold_stream": cold, "sustained": sus, "sectors": len(serving_sectors()),
             "mix": "chain m=6..13, square 3x3 and 3x4, n in [m-2, m+2], D <= 5e6; J/U log-uniform [0.01, 1], "
@@ -535,9 +548,9 @@ def main():
 
     # dominant kernel: the fused Lanczos operator pass K1 (row-lane engine at B = 16).  Algorithmic bytes per
     # launch: packed matrix 4 B/entry + row pointers 4 B/row + diagonal 4 B/row, and per vector and row
-    # v_j read (4 B), v_{j-1} read (4 B), w written in FP64 (8 B).
+    # v_j read (4 B), v_{j-1} read (4 B), the shifted w written in FP32 (4 B).
     D, nnz = ham.D, ham.nnz
-    k1_bytes = 4.0 * nnz + 4.0 * (D + 1) + 4.0 * D + B * D * (4 + 4 + 8)
+    k1_bytes = 4.0 * nnz + 4.0 * (D + 1) + 4.0 * D + B * D * (4 + 4 + 4)
     k1_ms = k1_total / max(k1_count, 1)
     achieved = k1_bytes / (k1_ms * 1e-3) / 1e9
     k1_tuned = ham.tuned(0, bhx.PREC_MIXED, B)
@@ -583,7 +596,7 @@ def main():
         t3 = time.perf_counter() - t3
         ctx.set_timing(False)
         c3, tot3 = ctx.kernel_stats("lanczos_spmv")
-        k1b3 = 4.0 * h3.nnz + 4.0 * (h3.D + 1) + 4.0 * h3.D + 2 * h3.D * (4 + 4 + 8)
+        k1b3 = 4.0 * h3.nnz + 4.0 * (h3.D + 1) + 4.0 * h3.D + 2 * h3.D * (4 + 4 + 4)
         k1us3 = tot3 / max(c3, 1) * 1e3
         spmv3["lanczos_mixed_B2"] = {"s": t3, "iters": r3.iters.tolist(), "E0": r3.evals[:, 0].tolist(),
                                      "evals": r3.evals.tolist(), "evals_fp64": E3_64.tolist(),

@@ -331,12 +331,19 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_sodots(LzArgs<XT> a, int64_t rows_
         for (int64_t r = r0 + threadIdx.x / B; r < r1; r += LZ_WG / B) {
             const int64_t idx = r * B + b;
             const double xr = to_d(a.Xc[idx]) * sj, w = to_d(a.W[idx]) + sg * xr;
+            // the locked vectors in groups of four, the group's loads issued together (a vector l >= nl reads
+            // slot nl - 1 again and adds zeros)
 #pragma unroll
-            for (int l = 0; l < LM; ++l) {
-                if (l < nl) {
-                    const double y = to_d(a.Y[static_cast<int64_t>(l) * a.vs + idx]);
-                    acc[l] += y * w;
-                    acc[LM + l] += y * xr;
+            for (int l0 = 0; l0 < LM; l0 += 4) {
+                if (l0 >= nl) break;
+                XT y[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) y[q] = a.Y[static_cast<int64_t>(min(l0 + q, nl - 1)) * a.vs + idx];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const double yd = l0 + q < nl ? to_d(y[q]) : 0.0;
+                    acc[l0 + q] += yd * w;
+                    acc[LM + l0 + q] += yd * xr;
                 }
             }
         }
