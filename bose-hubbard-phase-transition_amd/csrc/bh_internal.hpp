@@ -42,6 +42,9 @@ struct bh_ctx {
     std::vector<double> last_vecs;
     int last_vecs_B = 0, last_vecs_k = 0;
     int64_t last_vecs_D = 0;
+    // device allocations made through the context: calls and host time (hipFree waits for the device)
+    int64_t n_alloc = 0, n_free = 0;
+    double alloc_ms = 0.0, free_ms = 0.0;
 };
 
 int bh_set_error(const bh_ctx* ctx, int code, const std::string& msg);

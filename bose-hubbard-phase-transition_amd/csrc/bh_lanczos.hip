@@ -2885,6 +2885,8 @@ static int lanczos_batch(bh_ctx* ctx, bh_ham* h, int prec, const double* J, cons
                          const bh_lanczos_opts& o, bh_lanczos_result* res, std::vector<double>* vecs) {
     int rc = BH_OK;
     const auto ts0 = std::chrono::steady_clock::now();
+    const int64_t na0 = ctx->n_alloc, nf0 = ctx->n_free;
+    const double am0 = ctx->alloc_ms, fm0 = ctx->free_ms;
     auto e = bh_make_engine(ctx, h, prec, nreal, o, ctx->stream, true, &rc);
     if (!e) return rc;
     // the lanes of one batch finish together: their deflation passes start together too (one formation of the
@@ -2918,9 +2920,12 @@ static int lanczos_batch(bh_ctx* ctx, bh_ham* h, int prec, const double* J, cons
         const double teardown = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - td0).count();
         std::fprintf(stderr,
                      "[lz-profile] D=%lld lanes=%d fp64=%d checks=%d setup=%.1f ms run=%.1f ms (capture %.1f) wait=%.1f ms "
-                     "host=%.1f ms (analysis %.1f, deflate %.1f, lock %.1f) finish=%.1f ms total=%.1f ms teardown=%.1f ms\n",
+                     "host=%.1f ms (analysis %.1f, deflate %.1f, lock %.1f) finish=%.1f ms total=%.1f ms teardown=%.1f ms "
+                     "hipMalloc %lld (%.1f ms) hipFree %lld (%.1f ms)\n",
                      static_cast<long long>(h->D), nreal, f64 ? 1 : 0, pr.checks, pr.setup_ms, pr.run_ms, pr.capture_ms,
-                     pr.wait_ms, pr.host_ms, pr.analysis_ms, pr.deflate_ms, pr.lock_ms, pr.finish_ms, total, teardown);
+                     pr.wait_ms, pr.host_ms, pr.analysis_ms, pr.deflate_ms, pr.lock_ms, pr.finish_ms, total, teardown,
+                     static_cast<long long>(ctx->n_alloc - na0), ctx->alloc_ms - am0,
+                     static_cast<long long>(ctx->n_free - nf0), ctx->free_ms - fm0);
     }
     const int k = o.k;
     if (vecs) vecs->assign(static_cast<size_t>(nreal) * k * h->D, 0.0);

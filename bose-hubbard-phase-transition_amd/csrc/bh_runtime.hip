@@ -1,4 +1,5 @@
 // bh_runtime.hip -- context, errors, budgeted device memory, kernel timing, lattices.
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -19,7 +20,10 @@ int bh_dev_alloc(bh_ctx* ctx, void** p, size_t bytes) {
         return bh_set_error(ctx, BH_EBUDGET,
                             "HBM budget exceeded: need " + std::to_string(bytes) + " B, used " +
                                 std::to_string(ctx->used) + " of " + std::to_string(ctx->budget));
+    const auto t0 = std::chrono::steady_clock::now();
     hipError_t e = hipMalloc(p, bytes);
+    ctx->n_alloc += 1;
+    ctx->alloc_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (e != hipSuccess) {
         *p = nullptr;
         return bh_set_error(ctx, BH_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
@@ -30,7 +34,10 @@ int bh_dev_alloc(bh_ctx* ctx, void** p, size_t bytes) {
 
 void bh_dev_free(bh_ctx* ctx, void* p, size_t bytes) {
     if (!p) return;
+    const auto t0 = std::chrono::steady_clock::now();
     (void)hipFree(p);
+    ctx->n_free += 1;
+    ctx->free_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     ctx->used = ctx->used >= bytes ? ctx->used - bytes : 0;
 }
 
