@@ -311,15 +311,17 @@ def stream(srv, reqs, rate, rng):
         if len(ids) % 500 == 0:
             log(f"  submitted {len(ids)}/{len(reqs)}, {srv.stats().completed - done0} served")
     t_sub = time.perf_counter() - t0
-    # drain, reporting each further tenth of the stream served, then each request of the last tenth (a line
-    # only when requests have completed)
+    # drain, reporting each further tenth of the stream served, each request of the last tenth, and otherwise
+    # the count every 20 s while requests keep completing (a line only when requests have completed)
     ok, step = False, max(len(reqs) // 10, 1)
     last = srv.stats().completed - done0
     nxt = last + step
-    while time.perf_counter() - t0 < 900.0:
+    t_last = time.perf_counter()
+    while time.perf_counter() - t0 < 1800.0:
         ok = srv.drain(2.0)
         served = srv.stats().completed - done0
-        if ok or served >= nxt or (served > last and len(reqs) - served < step):
+        if ok or served >= nxt or (served > last and (len(reqs) - served < step or time.perf_counter() - t_last > 20.0)):
+            t_last = time.perf_counter()
             st = srv.stats()
             log(f"  served {served}/{len(reqs)} at {time.perf_counter() - t0:.1f} s (in flight {st.in_flight}, "
                 f"engines {st.engines}, HBM {st.hbm_used / 2**30:.1f} GiB)")

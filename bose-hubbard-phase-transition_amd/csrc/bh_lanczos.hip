@@ -51,6 +51,8 @@ constexpr int LZ_SO_KMAX = 6;      // multiplicities: larger k runs with full re
 constexpr int LZ_PASS_MAX = 64;    // deflation passes per solve
 constexpr int LZ_GRAM_VALS = 2 * 2 * LZ_NVMAX;  // partial sums per lane of one Gram-tile launch (NI 2 x NV 16, or 4 x 8)
 constexpr int LZ_RED_WG = 1024;    // reduction workgroup: 16 waves, one value per wave
+constexpr int LZ_WS_A = 5, LZ_WS_B = 6, LZ_WS_C = 7;  // context workspace slots of the engine's scratch (0-4: basis,
+                                                       // W, locked vectors, start vector, current vectors)
 constexpr int LZ_GRAPH_STEPS = 8;  // Lanczos steps per captured HIP graph
 constexpr int LZ_K2_GRID = 1024;   // K2 workgroups at most (rows of partial sums for R2; 512 and 2048 measured slower)
 constexpr int LZ_K1_TILES = 4;         // row groups per K1 workgroup on the row-lane engine
@@ -1892,14 +1894,15 @@ class LzEngine final : public LzEngineBase {
                 for (int i = 0; i < nc[b]; ++i)
                     for (int t = 0; t < static_cast<int>(rh[b].s[i].size()); ++t)
                         C[(static_cast<size_t>(t) * nvf + nf[b] + i) * B + b] = rh[b].s[i][t] * hsc[static_cast<size_t>(t) * B + b];
-            DevArray<double> Yf, HY;
-            BH_TRY(Yf.alloc(ctx, static_cast<size_t>(nvf) * vs));
-            BH_TRY(HY.alloc(ctx, static_cast<size_t>(nvf) * vs));
-            BH_TRY(form_vectors(njb, nvf, C, Yf.p, vs));
+            DevArray<double> Yf, HY;  // engines without the context workspace own their scratch
+            double *Yfp = nullptr, *HYp = nullptr;
+            BH_TRY(scratch(LZ_WS_A, Yf, static_cast<size_t>(nvf) * vs, &Yfp));
+            BH_TRY(scratch(LZ_WS_B, HY, static_cast<size_t>(nvf) * vs, &HYp));
+            BH_TRY(form_vectors(njb, nvf, C, Yfp, vs));
             if (nfmax > 0) {
                 BH_HIP(ctx, hipMemcpyAsync(dnj.p, nf.data(), B * sizeof(int), hipMemcpyHostToDevice, s));
                 hipLaunchKernelGGL((k_lz_copy_frozen<XT, B>), dim3(static_cast<unsigned>((vs + 255) / 256)), dim3(256), 0, s, Y,
-                                   Yf.p, vs, dnj.p, nfmax);
+                                   Yfp, vs, dnj.p, nfmax);
                 BH_HIP(ctx, hipStreamSynchronize(s));
             }
             // exact FP64 operator on each candidate (packed words, FP64 table, FP64 vectors)
@@ -1911,18 +1914,18 @@ class LzEngine final : public LzEngineBase {
             for (int i = 0; i < nvf; ++i) {
                 KTimer kt(ctx, "lanczos_finish_spmv", s);
                 SpmvArgs<double> sa{D, h->row_ptr, h->words, nullptr, nullptr, h->diag, nullptr, h->table, h->code_bits,
-                                    dJ.p, dU.p, Yf.p + static_cast<int64_t>(i) * vs, HY.p + static_cast<int64_t>(i) * vs, 1};
+                                    dJ.p, dU.p, Yfp + static_cast<int64_t>(i) * vs, HYp + static_cast<int64_t>(i) * vs, 1};
                 BH_TRY(bh_apply(ctx, h, &sa, nullptr, BH_FMT_PACKED, B, s));
             }
             BH_HIP(ctx, hipGetLastError());
-            BH_TRY(rayleigh_ritz(lanes, nvf, nt, Yf.p, HY.p));
+            BH_TRY(rayleigh_ritz(lanes, nvf, nt, Yfp, HYp));
             // Rayleigh quotients and residuals, LZ_NVMAX vectors per launch
             dots.assign(static_cast<size_t>(2 * nvf) * B, 0.0);
             std::vector<double> rr(static_cast<size_t>(nvf) * B);
             for (int i0 = 0; i0 < nvf; i0 += LZ_NVMAX) {
                 const int nq = std::min(LZ_NVMAX, nvf - i0);
-                hipLaunchKernelGGL((k_lz_dots<double, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yf.p + static_cast<int64_t>(i0) * vs,
-                                   HY.p + static_cast<int64_t>(i0) * vs, D, nq, 0, nullptr, part.p);
+                hipLaunchKernelGGL((k_lz_dots<double, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yfp + static_cast<int64_t>(i0) * vs,
+                                   HYp + static_cast<int64_t>(i0) * vs, D, nq, 0, nullptr, part.p);
                 reduce_plain(grid_k2, 2 * nq * B, dnorm.p);
                 BH_HIP(ctx, hipMemcpyAsync(dots.data() + static_cast<size_t>(2 * i0) * B, dnorm.p, static_cast<size_t>(2 * nq) * B * 8,
                                            hipMemcpyDeviceToHost, s));
@@ -1941,8 +1944,8 @@ class LzEngine final : public LzEngineBase {
             BH_HIP(ctx, hipMemcpyAsync(dth.p, th0.data(), th0.size() * 8, hipMemcpyHostToDevice, s));
             for (int i0 = 0; i0 < nvf; i0 += LZ_NVMAX) {
                 const int nq = std::min(LZ_NVMAX, nvf - i0);
-                hipLaunchKernelGGL((k_lz_dots<double, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yf.p + static_cast<int64_t>(i0) * vs,
-                                   HY.p + static_cast<int64_t>(i0) * vs, D, nq, 1, dth.p + static_cast<size_t>(i0) * B, part.p);
+                hipLaunchKernelGGL((k_lz_dots<double, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yfp + static_cast<int64_t>(i0) * vs,
+                                   HYp + static_cast<int64_t>(i0) * vs, D, nq, 1, dth.p + static_cast<size_t>(i0) * B, part.p);
                 reduce_plain(grid_k2, nq * B, dnorm.p);
                 BH_HIP(ctx, hipMemcpyAsync(rr.data() + static_cast<size_t>(i0) * B, dnorm.p, static_cast<size_t>(nq) * B * 8,
                                            hipMemcpyDeviceToHost, s));
@@ -1955,7 +1958,7 @@ class LzEngine final : public LzEngineBase {
                 }
             if (want_vectors) {
                 hy.resize(static_cast<size_t>(nvf) * vs);
-                BH_HIP(ctx, hipMemcpyAsync(hy.data(), Yf.p, hy.size() * 8, hipMemcpyDeviceToHost, s));
+                BH_HIP(ctx, hipMemcpyAsync(hy.data(), Yfp, hy.size() * 8, hipMemcpyDeviceToHost, s));
                 BH_HIP(ctx, hipStreamSynchronize(s));
             }
         }
@@ -2194,10 +2197,11 @@ class LzEngine final : public LzEngineBase {
             hipLaunchKernelGGL((k_lz_combine<B>), dim3(grid), dim3(256), 0, s, HYp, vs, dz.p, nvf, nvf);
         } else {
             DevArray<double> tmp;
-            BH_TRY(tmp.alloc(ctx, static_cast<size_t>(nvf) * vs));
+            double* tmpp = nullptr;
+            BH_TRY(scratch(LZ_WS_C, tmp, static_cast<size_t>(nvf) * vs, &tmpp));
             for (double* X : {Yp, HYp}) {
-                hipLaunchKernelGGL((k_lz_combine_wide<B>), dim3(grid), dim3(256), 0, s, X, tmp.p, vs, dz.p, nvf, nvf);
-                BH_HIP(ctx, hipMemcpyAsync(X, tmp.p, static_cast<size_t>(nvf) * vs * 8, hipMemcpyDeviceToDevice, s));
+                hipLaunchKernelGGL((k_lz_combine_wide<B>), dim3(grid), dim3(256), 0, s, X, tmpp, vs, dz.p, nvf, nvf);
+                BH_HIP(ctx, hipMemcpyAsync(X, tmpp, static_cast<size_t>(nvf) * vs * 8, hipMemcpyDeviceToDevice, s));
             }
             BH_HIP(ctx, hipStreamSynchronize(s));  // tmp is freed on return
         }
@@ -2382,6 +2386,22 @@ class LzEngine final : public LzEngineBase {
                                lz_so_always());
         }
     }
+    // Scratch of the finish, the deflations and the locks: a grow-only slot of the context workspace when the
+    // engine uses it (batch solves: no allocation per call, and no hipFree, which waits for the device), else
+    // the caller's own array (server engines).  LZ_WS_A: the finish's candidates / a deflation's Ritz vectors;
+    // LZ_WS_B: the finish's H Y; LZ_WS_C: the Rayleigh-Ritz combination / a lock's Ritz vectors.
+    template <typename T>
+    int scratch(int slot, DevArray<T>& own, size_t n, T** out) {
+        if (ws) {
+            void* q = nullptr;
+            BH_TRY(bh_ws_get(ctx, slot, n * sizeof(T), &q));
+            *out = static_cast<T*>(q);
+            return BH_OK;
+        }
+        BH_TRY(own.alloc(ctx, n));
+        *out = own.p;
+        return BH_OK;
+    }
     // C holds rows t < max(njb) of [(t * nv + i) * B + b]
     template <typename OT>
     int form_vectors(const std::vector<int>& njb, int nv, const std::vector<double>& C, OT* out, int64_t os) {
@@ -2507,17 +2527,18 @@ class LzEngine final : public LzEngineBase {
             }
         }
         DevArray<XT> tmp;
-        BH_TRY(tmp.alloc(ctx, static_cast<size_t>(nv) * vs));
-        BH_TRY(form_vectors(njb, nv, C, tmp.p, vs));
+        XT* tmpp = nullptr;
+        BH_TRY(scratch(LZ_WS_C, tmp, static_cast<size_t>(nv) * vs, &tmpp));
+        BH_TRY(form_vectors(njb, nv, C, tmpp, vs));
         // normalise: ||y||^2 per (i, b)
-        hipLaunchKernelGGL((k_lz_dots<XT, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, tmp.p, nullptr, D, nv, 2, nullptr, part.p);
+        hipLaunchKernelGGL((k_lz_dots<XT, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, tmpp, nullptr, D, nv, 2, nullptr, part.p);
         reduce_plain(grid_k2, nv * B, dnorm.p);
         std::vector<double> nrm(static_cast<size_t>(nv) * B);
         BH_HIP(ctx, hipMemcpyAsync(nrm.data(), dnorm.p, nrm.size() * 8, hipMemcpyDeviceToHost, s));
         BH_HIP(ctx, hipStreamSynchronize(s));
         for (auto& v : nrm) v = v > 0 ? 1.0 / std::sqrt(v) : 0.0;
         BH_HIP(ctx, hipMemcpyAsync(dnorm.p, nrm.data(), nrm.size() * 8, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL((k_lz_scale<XT, B>), dim3(static_cast<unsigned>((vs * nv + 255) / 256)), dim3(256), 0, s, tmp.p, vs,
+        hipLaunchKernelGGL((k_lz_scale<XT, B>), dim3(static_cast<unsigned>((vs * nv + 255) / 256)), dim3(256), 0, s, tmpp, vs,
                            nv, dnorm.p);
         // vector i of lane b goes to Y slot nlock[b] + i (strided copy of one interleaved column); its Ritz
         // value and a fresh orthogonality estimate go to the device state
@@ -2553,7 +2574,7 @@ class LzEngine final : public LzEngineBase {
         }
         // vector i of lane b goes to Y slot dst[i][b] (one interleaved column each), in one launch
         BH_HIP(ctx, hipMemcpyAsync(lock_map.p, dst.data(), dst.size() * sizeof(int), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL((k_lz_put_locked<XT, B>), dim3(static_cast<unsigned>((vs + 255) / 256)), dim3(256), 0, s, tmp.p,
+        hipLaunchKernelGGL((k_lz_put_locked<XT, B>), dim3(static_cast<unsigned>((vs + 255) / 256)), dim3(256), 0, s, tmpp,
                            Y, vs, nv, lock_map.p);
         const int so2[2] = {1, 0};  // orthogonalise the next two Lanczos vectors against the locked set
         BH_HIP(ctx, hipMemcpyAsync(st.ltheta, lth.data(), lth.size() * 8, hipMemcpyHostToDevice, s));
@@ -2625,8 +2646,9 @@ class LzEngine final : public LzEngineBase {
             }
         }
         DevArray<double> Zf;
-        BH_TRY(Zf.alloc(ctx, static_cast<size_t>(nv) * vs));
-        BH_TRY(form_vectors(njb, nv, C, Zf.p, vs));
+        double* Zfp = nullptr;
+        BH_TRY(scratch(LZ_WS_A, Zf, static_cast<size_t>(nv) * vs, &Zfp));
+        BH_TRY(form_vectors(njb, nv, C, Zfp, vs));
         const unsigned g256 = static_cast<unsigned>((vs + 255) / 256);
         // drop the frozen vectors the plans do not keep: kept slots move down, in ascending order
         {
@@ -2660,7 +2682,7 @@ class LzEngine final : public LzEngineBase {
                 cnt[b] = on ? rh[b].nfz + i : -1;
                 slot[b] = on ? rh[b].nfz + i : -1;
             }
-            double* Zi = Zf.p + static_cast<int64_t>(i) * vs;
+            double* Zi = Zfp + static_cast<int64_t>(i) * vs;
             BH_TRY(ortho_frozen(Zi, cnt));
             BH_TRY(inv_norms(static_cast<const double*>(Zi), f));
             BH_HIP(ctx, hipMemcpyAsync(lock_map.p, slot.data(), B * sizeof(int), hipMemcpyHostToDevice, s));
