@@ -11,7 +11,7 @@ CSR, including the FP64 finish (Ritz vectors, Rayleigh quotients with the exact 
 residuals).  value = eigensolves per second.  The solver layout is the lexicographic basis order
 (same spectrum as the reference order; the CSR is rebuilt in it, 3 ms).
 
-Beside it: the plain FP64 GPU Lanczos (FP64 CSR and FP64 vectors, one solve at a time, its fastest
+Beside it: the plain FP64 GPU Lanczos (FP64 vectors, exact FP64 matrix values, one solve at a time, its fastest
 engine), the error of every mixed value against the FP64 solve of its J point, the dominant kernel's HBM
 roofline (HIP events on the library stream over the timed region), and the reference's own CPU solver
 (Op::IRLM_eigen = Spectra GenEigs, compiled from the reference's sources into oracle/_ref) on the host
@@ -511,7 +511,7 @@ def main():
     build_s = time.perf_counter() - t0
     log(f"build D={ham.D} nnz={ham.nnz} {build_s * 1e3:.1f} ms")
 
-    # plain FP64 GPU Lanczos: FP64 CSR and vectors, unbatched (accuracy comparator and baseline rate), and the
+    # plain FP64 GPU Lanczos: FP64 vectors and matrix values, unbatched (accuracy comparator and baseline rate), and the
     # FP64 values of every J point of the batch (the comparator of each mixed lane's four values)
     fp64_s, r64, E64 = float("nan"), None, None
     if not args.quick:

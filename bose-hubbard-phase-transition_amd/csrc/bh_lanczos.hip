@@ -1194,7 +1194,9 @@ static int lz_ymax(int64_t D, const bh_lanczos_opts& o) {
 template <typename XT, int B>
 class LzEngine final : public LzEngineBase {
     static constexpr bool MIXED = std::is_same<XT, float>::value;
-    static constexpr int FMT = MIXED ? BH_FMT_PACKED : BH_FMT_CSR64;
+    // Both precisions read the packed words: the FP64 table gives the exact matrix values (4 bytes per entry
+    // instead of the 12 of an FP64 CSR), and the diagonal V (small integers) is exact in FP32.
+    static constexpr int FMT = BH_FMT_PACKED;
 
    public:
     LzEngine(bh_ctx* c, bh_ham* ham, const bh_lanczos_opts& opts, hipStream_t stream, bool use_ctx_ws)
@@ -1321,10 +1323,10 @@ class LzEngine final : public LzEngineBase {
         a = LzArgs<XT>{D,
                        h->row_ptr,
                        h->words,
-                       h->col64,
-                       h->val64,
-                       MIXED ? h->diag : nullptr,
-                       MIXED ? nullptr : h->diag64,
+                       nullptr,
+                       nullptr,
+                       h->diag,
+                       nullptr,
                        h->table,
                        h->code_bits,
                        V,
@@ -2820,10 +2822,6 @@ std::unique_ptr<LzEngineBase> bh_make_engine(bh_ctx* ctx, bh_ham* h, int prec, i
     std::unique_ptr<LzEngineBase> e;
     *rc = BH_OK;
     if (bh_lz_fullre(h->D, o)) prec = BH_PREC_FP64;
-    if (prec == BH_PREC_FP64) {
-        *rc = bh_ham_prepare_locked(h, BH_PREC_FP64);
-        if (*rc != BH_OK) return nullptr;
-    }
 #define MK(XT, BB)                                                                  \
     {                                                                               \
         auto* p = new LzEngine<XT, BB>(ctx, h, o, s, use_ctx_ws);                   \

@@ -54,7 +54,9 @@ extern "C" {
 #define BH_ORDER_LEX 2
 
 /* ---- precisions --------------------------------------------------------------------------------- */
-/* BH_PREC_FP64: FP64 CSR (double value + int32 column) and FP64 vectors -- the plain comparator.
+/* BH_PREC_FP64: FP64 vectors and arithmetic -- the plain comparator.  The SpMV entry points read an FP64
+ *   CSR (double value + int32 column); the Lanczos engine reads the packed words below, whose FP64 table
+ *   gives the same exact values in 4 bytes per entry.
  * BH_PREC_MIXED: one 32-bit word per entry (column << code_bits | code), value = -sqrt(code) read
  *   from an exact FP64 table, FP32-stored vectors, FP64 accumulation and reductions, FP64 finish. */
 #define BH_PREC_FP64 0
