@@ -22,6 +22,8 @@ struct LzProfile {
     double deflate_ms = 0.0;   // of which the deflations (Ritz vectors frozen, fresh starts)
     double lock_ms = 0.0;      // of which the locks (locked Ritz vectors formed)
     double finish_ms = 0.0;  // FP64 finish, wall time
+    int refine_steps = 0;    // Davidson steps of the FP64 refinement run in the finishes (one per batch of lanes)
+    int refined_lanes = 0;   // lane-steps of the refinement
     double run_ms = 0.0;     // enqueueing steps, graph captures included
     double capture_ms = 0.0; // capturing and instantiating step graphs
     double setup_ms = 0.0;   // engine construction and admission

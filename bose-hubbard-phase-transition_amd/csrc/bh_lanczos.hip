@@ -61,6 +61,8 @@ constexpr int LZ_CHECK_MIN = 4;        // fewest steps between two analyses of a
 constexpr int64_t LZ_SPEC_MIN_WORK = 4ll << 20;  // D B from which analyses overlap speculative steps
 constexpr int64_t LZ_PART_BYTES = 64ll << 20;  // partial sums of the staged K1 (more workgroups than grid_rl)
 constexpr int LZ_K1_GRID_MAX = 32768;  // K1 workgroups at most (rows of partial sums)
+constexpr int LZ_REFINE_STEPS = 3;         // Davidson steps of the FP64 refinement of a rejected mixed lane
+constexpr double LZ_REFINE_CLAMP = 0.1;    // refinement: |diag(H) - theta| held at least this x max(|U|, |J|)
 constexpr double LZ_ACCEPT_LEVEL = 1e-6;  // mixed: bound on ||r||^2/gap of the levels above E0, relative
 
 struct LzState {
@@ -911,6 +913,29 @@ __global__ void k_lz_scale(YT* __restrict__ Y, int64_t vs, int nv, const double*
     Y[idx] = static_cast<YT>(to_d(Y[idx]) * f[i * B + b]);
 }
 
+// Davidson correction of the FP64 refinement, lanes with on[b]: T[i][r][b] = (HY - theta Y)[i][r][b] / (U_b V_r - theta_ib),
+// the denominator held at least dl[b] away from zero (a diagonal entry close to theta is a direction the diagonal
+// does not resolve); T = 0 on the other lanes
+template <int B>
+__global__ void __launch_bounds__(256) k_lz_dav_t(const double* __restrict__ Y, const double* __restrict__ HY, double* __restrict__ T,
+                                                  int64_t D, int nv, const double* __restrict__ theta, const double* __restrict__ U,
+                                                  const double* __restrict__ dl, const float* __restrict__ diag,
+                                                  const int* __restrict__ on) {
+    const int64_t vs = D * B;
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= vs * nv) return;
+    const int64_t i = idx / vs, e = idx - i * vs;
+    const int b = static_cast<int>(e % B);
+    double t = 0.0;
+    if (on[b]) {
+        const double th = theta[i * B + b], d = dl[b];
+        double den = U[b] * static_cast<double>(diag[e / B]) - th;
+        if (fabs(den) < d) den = den < 0.0 ? -d : d;
+        t = (HY[idx] - th * Y[idx]) / den;
+    }
+    T[idx] = t;
+}
+
 // out-of-place form of k_lz_combine for more than LZ_NVMAX inputs: T[c][r][b] = sum_{i < nin} X[i][r][b] Z[(i nout + c) B + b]
 template <int B>
 __global__ void __launch_bounds__(256) k_lz_combine_wide(const double* __restrict__ X, double* __restrict__ T, int64_t vs,
@@ -1088,6 +1113,16 @@ int lz_so_always() {
     return d;
 }
 
+// Davidson steps of the FP64 refinement that a mixed lane failing its acceptance test gets in the finish
+// before it is reported (and rerun in FP64): BHX_REFINE=<n>, 0 turns the refinement off.
+int lz_refine_steps() {
+    static int d = [] {
+        const char* e = std::getenv("BHX_REFINE");
+        return e ? std::max(0, std::atoi(e)) : LZ_REFINE_STEPS;
+    }();
+    return d;
+}
+
 // Steps enqueued at an analysis before the host analyses (the device runs them meanwhile): one captured
 // graph's worth by default, BHX_SPEC=0 turns the overlap off.
 int lz_spec_steps() {
@@ -1132,6 +1167,7 @@ struct RunHost {
     int pass = 0;
     int nfz = 0;
     std::vector<double> fz_theta;
+    int refined = 0;  // Davidson steps of the FP64 refinement in the finish
 };
 
 struct Analysis {
@@ -1898,8 +1934,19 @@ class LzEngine final : public LzEngineBase {
                         C[(static_cast<size_t>(t) * nvf + nf[b] + i) * B + b] = rh[b].s[i][t] * hsc[static_cast<size_t>(t) * B + b];
             DevArray<double> Yf, HY;  // engines without the context workspace own their scratch
             double *Yfp = nullptr, *HYp = nullptr;
-            BH_TRY(scratch(LZ_WS_A, Yf, static_cast<size_t>(nvf) * vs, &Yfp));
-            BH_TRY(scratch(LZ_WS_B, HY, static_cast<size_t>(nvf) * vs, &HYp));
+            // mixed lanes get slots for the corrections of the FP64 refinement of their lowest k + 1 candidates
+            // (dropped when the memory is not there: such a lane is then rerun in FP64 as without the refinement)
+            int pref = MIXED && lz_refine_steps() > 0 ? std::min(nvf, k + 1) : 0;
+            {
+                int rc0 = scratch(LZ_WS_A, Yf, static_cast<size_t>(nvf + pref) * vs, &Yfp);
+                if (rc0 == BH_OK) rc0 = scratch(LZ_WS_B, HY, static_cast<size_t>(nvf + pref) * vs, &HYp);
+                if (rc0 != BH_OK && pref > 0) {
+                    pref = 0;
+                    rc0 = scratch(LZ_WS_A, Yf, static_cast<size_t>(nvf) * vs, &Yfp);
+                    if (rc0 == BH_OK) rc0 = scratch(LZ_WS_B, HY, static_cast<size_t>(nvf) * vs, &HYp);
+                }
+                BH_TRY(rc0);
+            }
             BH_TRY(form_vectors(njb, nvf, C, Yfp, vs));
             if (nfmax > 0) {
                 BH_HIP(ctx, hipMemcpyAsync(dnj.p, nf.data(), B * sizeof(int), hipMemcpyHostToDevice, s));
@@ -1921,43 +1968,44 @@ class LzEngine final : public LzEngineBase {
             }
             BH_HIP(ctx, hipGetLastError());
             BH_TRY(rayleigh_ritz(lanes, nvf, nt, Yfp, HYp));
-            // Rayleigh quotients and residuals, LZ_NVMAX vectors per launch
             dots.assign(static_cast<size_t>(2 * nvf) * B, 0.0);
-            std::vector<double> rr(static_cast<size_t>(nvf) * B);
-            for (int i0 = 0; i0 < nvf; i0 += LZ_NVMAX) {
-                const int nq = std::min(LZ_NVMAX, nvf - i0);
-                hipLaunchKernelGGL((k_lz_dots<double, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yfp + static_cast<int64_t>(i0) * vs,
-                                   HYp + static_cast<int64_t>(i0) * vs, D, nq, 0, nullptr, part.p);
-                reduce_plain(grid_k2, 2 * nq * B, dnorm.p);
-                BH_HIP(ctx, hipMemcpyAsync(dots.data() + static_cast<size_t>(2 * i0) * B, dnorm.p, static_cast<size_t>(2 * nq) * B * 8,
-                                           hipMemcpyDeviceToHost, s));
-                BH_HIP(ctx, hipStreamSynchronize(s));
-            }
-            for (int i = 0; i < nvf; ++i)
-                for (int b = 0; b < B; ++b) {
-                    const double yy = dots[(2 * i) * B + b], yhy = dots[(2 * i + 1) * B + b];
-                    theta_f[i * B + b] = yy > 0 ? yhy / yy : NAN;
+            BH_TRY(measure(nvf, Yfp, HYp, nullptr, dots, theta_f, resid_f));
+            // lanes whose FP64 Rayleigh-Ritz values fail the acceptance test get up to lz_refine_steps() Davidson
+            // steps in FP64 (refine); a lane that still fails is reported as before and rerun in FP64
+            for (int it = 0; pref > 0 && it < lz_refine_steps(); ++it) {
+                std::vector<int> on(B, 0);
+                bool any = false;
+                for (size_t q = 0; q < lanes.size(); ++q) {
+                    const int b = lanes[q];
+                    if (nt[b] == 0 || rh[b].status != BH_OK) continue;
+                    std::vector<std::pair<double, int>> lv;
+                    LaneResult L;
+                    select_levels(b, nt[b], theta_f, resid_f, L, lv);
+                    // a lane short of k levels is left to the rerun: the corrections of the levels it has do
+                    // not supply a missing copy
+                    if (static_cast<int>(lv.size()) < k && !rh[b].exhausted) continue;
+                    if (!bh_lane_accepted(L.status, L.evals.data(), L.errest.data(), k, o.accept_rel)) {
+                        on[b] = 1;
+                        any = true;
+                    }
                 }
-            DevArray<double> dth;
-            BH_TRY(dth.alloc(ctx, theta_f.size()));
-            std::vector<double> th0(theta_f);
-            for (auto& v : th0)
-                if (!std::isfinite(v)) v = 0.0;
-            BH_HIP(ctx, hipMemcpyAsync(dth.p, th0.data(), th0.size() * 8, hipMemcpyHostToDevice, s));
-            for (int i0 = 0; i0 < nvf; i0 += LZ_NVMAX) {
-                const int nq = std::min(LZ_NVMAX, nvf - i0);
-                hipLaunchKernelGGL((k_lz_dots<double, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yfp + static_cast<int64_t>(i0) * vs,
-                                   HYp + static_cast<int64_t>(i0) * vs, D, nq, 1, dth.p + static_cast<size_t>(i0) * B, part.p);
-                reduce_plain(grid_k2, nq * B, dnorm.p);
-                BH_HIP(ctx, hipMemcpyAsync(rr.data() + static_cast<size_t>(i0) * B, dnorm.p, static_cast<size_t>(nq) * B * 8,
-                                           hipMemcpyDeviceToHost, s));
-                BH_HIP(ctx, hipStreamSynchronize(s));
+                if (!any) break;
+                BH_TRY(refine(on, nvf, pref, Yfp, HYp, theta_f));
+                prof.refine_steps += 1;
+                std::vector<double> d2(dots.size()), t2(theta_f.size()), r2(resid_f.size());
+                BH_TRY(measure(nvf, Yfp, HYp, &on, d2, t2, r2));
+                for (int b = 0; b < B; ++b)
+                    if (on[b]) {
+                        for (int i = 0; i < nvf; ++i) {
+                            theta_f[i * B + b] = t2[i * B + b];
+                            resid_f[i * B + b] = r2[i * B + b];
+                            dots[(2 * i) * B + b] = d2[(2 * i) * B + b];
+                            dots[(2 * i + 1) * B + b] = d2[(2 * i + 1) * B + b];
+                        }
+                        rh[b].refined = it + 1;
+                        prof.refined_lanes += 1;
+                    }
             }
-            for (int i = 0; i < nvf; ++i)
-                for (int b = 0; b < B; ++b) {
-                    const double yy = dots[(2 * i) * B + b];
-                    resid_f[i * B + b] = yy > 0 ? std::sqrt(rr[i * B + b] / yy) : NAN;
-                }
             if (want_vectors) {
                 hy.resize(static_cast<size_t>(nvf) * vs);
                 BH_HIP(ctx, hipMemcpyAsync(hy.data(), Yfp, hy.size() * 8, hipMemcpyDeviceToHost, s));
@@ -1968,13 +2016,41 @@ class LzEngine final : public LzEngineBase {
             const int b = lanes[q];
             RunHost& R = rh[b];
             LaneResult& L = out[q];
+            std::vector<std::pair<double, int>> lv;
+            select_levels(b, nt[b], theta_f, resid_f, L, lv);
+            if (lz_debug()) {
+                std::fprintf(stderr, "[lz] finish lane=%d run_status=%d status=%d refined=%d levels=%d frozen=%d pass=%d:", b,
+                             R.status, L.status, R.refined, static_cast<int>(lv.size()), R.nfz, R.pass);
+                for (int i = 0; i < nt[b]; ++i) std::fprintf(stderr, " (%.10f r=%.1e)", theta_f[i * B + b], resid_f[i * B + b]);
+                std::fprintf(stderr, "\n");
+            }
+            if (want_vectors) L.vecs.assign(static_cast<size_t>(k) * D, 0.0);
+            for (int i = 0; i < k && i < static_cast<int>(lv.size()); ++i) {
+                if (want_vectors && !hy.empty()) {
+                    const int iv = lv[i].second;
+                    const double yy = dots[(2 * iv) * B + b];
+                    const double f = yy > 0 ? 1.0 / std::sqrt(yy) : 0.0;
+                    for (int64_t r = 0; r < D; ++r)
+                        L.vecs[static_cast<size_t>(i) * D + r] = hy[static_cast<size_t>(iv) * vs + r * B + b] * f;
+                }
+            }
+        }
+        return BH_OK;
+    }
+
+   private:
+    // Levels of lane b from its nl candidates (FP64 Rayleigh quotients theta_f, residual norms resid_f):
+    // L.status, evals, resid, errest and the chosen candidates lv (value, slot).
+    void select_levels(int b, int nl, const std::vector<double>& theta_f, const std::vector<double>& resid_f, LaneResult& L,
+                       std::vector<std::pair<double, int>>& lv) {
+            const RunHost& R = rh[b];
             L.iters = R.iters;
             L.status = R.status;
             L.evals.assign(k, NAN);
             L.resid.assign(k, NAN);
             L.errest.assign(k, NAN);
-            const int nl = nt[b];
-            std::vector<std::pair<double, int>> lv0, lv;
+            std::vector<std::pair<double, int>> lv0;
+            lv.clear();
             for (int i = 0; i < nl; ++i) lv0.push_back({theta_f[i * B + b], i});
             std::sort(lv0.begin(), lv0.end());
             const double mtol = dedup * std::fmax(R.tnorm, 1e-300);
@@ -2038,13 +2114,6 @@ class LzEngine final : public LzEngineBase {
             if (MIXED && dropped && L.status == BH_OK) L.status = BH_ENOCONV;
             // fewer than k levels is an answer only when the Krylov space was exhausted
             if (L.status == BH_OK && static_cast<int>(lv.size()) < k && !R.exhausted) L.status = BH_ENOCONV;
-            if (lz_debug()) {
-                std::fprintf(stderr, "[lz] finish lane=%d run_status=%d status=%d dropped=%d levels=%d frozen=%d pass=%d:", b,
-                             R.status, L.status, dropped ? 1 : 0, static_cast<int>(lv.size()), R.nfz, R.pass);
-                for (const auto& qq : lv0) std::fprintf(stderr, " (%.10f r=%.1e)", qq.first, rs_of(qq));
-                std::fprintf(stderr, "\n");
-            }
-            if (want_vectors) L.vecs.assign(static_cast<size_t>(k) * D, 0.0);
             for (int i = 0; i < k && i < static_cast<int>(lv.size()); ++i) {
                 const double ev = lv[i].first;
                 const double rs = rs_of(lv[i]);
@@ -2054,15 +2123,116 @@ class LzEngine final : public LzEngineBase {
                 L.evals[i] = ev;
                 L.resid[i] = rs;
                 L.errest[i] = std::isfinite(gap) ? rs * rs / gap : rs;
-                if (want_vectors && !hy.empty()) {
-                    const int iv = lv[i].second;
-                    const double yy = dots[(2 * iv) * B + b];
-                    const double f = yy > 0 ? 1.0 / std::sqrt(yy) : 0.0;
-                    for (int64_t r = 0; r < D; ++r)
-                        L.vecs[static_cast<size_t>(i) * D + r] = hy[static_cast<size_t>(iv) * vs + r * B + b] * f;
-                }
             }
+    }
+    // FP64 Rayleigh quotients and residual norms of slots 0 .. nvf - 1 (LZ_NVMAX vectors per launch): dots
+    // (<y,y>, <y,Hy>), theta and resid, for every lane
+    int measure(int nvf, const double* Yfp, const double* HYp, const std::vector<int>* /*unused*/, std::vector<double>& dots,
+                std::vector<double>& theta_f, std::vector<double>& resid_f) {
+        std::vector<double> rr(static_cast<size_t>(nvf) * B);
+        for (int i0 = 0; i0 < nvf; i0 += LZ_NVMAX) {
+            const int nq = std::min(LZ_NVMAX, nvf - i0);
+            hipLaunchKernelGGL((k_lz_dots<double, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yfp + static_cast<int64_t>(i0) * vs,
+                               HYp + static_cast<int64_t>(i0) * vs, D, nq, 0, nullptr, part.p);
+            reduce_plain(grid_k2, 2 * nq * B, dnorm.p);
+            BH_HIP(ctx, hipMemcpyAsync(dots.data() + static_cast<size_t>(2 * i0) * B, dnorm.p, static_cast<size_t>(2 * nq) * B * 8,
+                                       hipMemcpyDeviceToHost, s));
+            BH_HIP(ctx, hipStreamSynchronize(s));
         }
+        for (int i = 0; i < nvf; ++i)
+            for (int b = 0; b < B; ++b) {
+                const double yy = dots[(2 * i) * B + b], yhy = dots[(2 * i + 1) * B + b];
+                theta_f[i * B + b] = yy > 0 ? yhy / yy : NAN;
+            }
+        DevArray<double> dth;
+        BH_TRY(dth.alloc(ctx, static_cast<size_t>(nvf) * B));
+        std::vector<double> th0(theta_f.begin(), theta_f.begin() + static_cast<size_t>(nvf) * B);
+        for (auto& v : th0)
+            if (!std::isfinite(v)) v = 0.0;
+        BH_HIP(ctx, hipMemcpyAsync(dth.p, th0.data(), th0.size() * 8, hipMemcpyHostToDevice, s));
+        for (int i0 = 0; i0 < nvf; i0 += LZ_NVMAX) {
+            const int nq = std::min(LZ_NVMAX, nvf - i0);
+            hipLaunchKernelGGL((k_lz_dots<double, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, Yfp + static_cast<int64_t>(i0) * vs,
+                               HYp + static_cast<int64_t>(i0) * vs, D, nq, 1, dth.p + static_cast<size_t>(i0) * B, part.p);
+            reduce_plain(grid_k2, nq * B, dnorm.p);
+            BH_HIP(ctx, hipMemcpyAsync(rr.data() + static_cast<size_t>(i0) * B, dnorm.p, static_cast<size_t>(nq) * B * 8,
+                                       hipMemcpyDeviceToHost, s));
+            BH_HIP(ctx, hipStreamSynchronize(s));  // dth is freed on return
+        }
+        for (int i = 0; i < nvf; ++i)
+            for (int b = 0; b < B; ++b) {
+                const double yy = dots[(2 * i) * B + b];
+                resid_f[i * B + b] = yy > 0 ? std::sqrt(rr[i * B + b] / yy) : NAN;
+            }
+        return BH_OK;
+    }
+    // One Davidson step in FP64 for the lanes with on[b] = 1, over their nvf candidates in slots 0 .. nvf - 1 of
+    // Y (and HY = H Y), lowest first: the corrections t_i = (H y_i - theta_i y_i) / (diag(H) - theta_i) of the
+    // lowest np, each normalised, go to slots nvf .. nvf + np - 1 with their images under the exact FP64
+    // operator, and a Rayleigh-Ritz over the nvf + np vectors leaves the lowest nvf Ritz vectors in slots
+    // 0 .. nvf - 1 (the other lanes keep theirs).
+    // The candidates are combinations of FP32-stored Krylov vectors, so their residuals stop near the FP32
+    // rounding of H y; where that fails the certificate (small J/U: tiny |E0| or close levels), the diagonal
+    // U V dominates the operator and the preconditioned corrections remove most of the residual.
+    int refine(const std::vector<int>& on, int nvf, int np, double* Yp, double* HYp, const std::vector<double>& theta_f) {
+        double* Tp = Yp + static_cast<int64_t>(nvf) * vs;
+        double* HTp = HYp + static_cast<int64_t>(nvf) * vs;
+        std::vector<double> th0(theta_f.begin(), theta_f.begin() + static_cast<size_t>(np) * B), dl(B);
+        for (auto& v : th0)
+            if (!std::isfinite(v)) v = 0.0;
+        for (int b = 0; b < B; ++b) dl[b] = LZ_REFINE_CLAMP * std::fmax(std::fmax(std::fabs(Ub[b]), std::fabs(Jb[b])), 1e-300);
+        DevArray<double> dth, ddl, dU, dJ, df;
+        DevArray<int> don;
+        BH_TRY(dth.alloc(ctx, th0.size()));
+        BH_TRY(ddl.alloc(ctx, B));
+        BH_TRY(dU.alloc(ctx, B));
+        BH_TRY(dJ.alloc(ctx, B));
+        BH_TRY(df.alloc(ctx, static_cast<size_t>(np) * B));
+        BH_TRY(don.alloc(ctx, B));
+        BH_HIP(ctx, hipMemcpyAsync(dth.p, th0.data(), th0.size() * 8, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(ddl.p, dl.data(), B * 8, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(dU.p, Ub.data(), B * 8, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(dJ.p, Jb.data(), B * 8, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(don.p, on.data(), B * sizeof(int), hipMemcpyHostToDevice, s));
+        {
+            KTimer kt(ctx, "lanczos_refine", s);
+            const int64_t n = static_cast<int64_t>(np) * vs;
+            hipLaunchKernelGGL((k_lz_dav_t<B>), dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, Yp, HYp, Tp, D, np,
+                               dth.p, dU.p, ddl.p, h->diag, don.p);
+        }
+        // unit corrections (a zero correction stays zero)
+        std::vector<double> nrm(static_cast<size_t>(np) * B), f(nrm.size());
+        for (int i0 = 0; i0 < np; i0 += LZ_NVMAX) {
+            const int nq = std::min(LZ_NVMAX, np - i0);
+            hipLaunchKernelGGL((k_lz_dots<double, B>), dim3(grid_k2), dim3(LZ_WG), 0, s, Tp + static_cast<int64_t>(i0) * vs, nullptr,
+                               D, nq, 2, nullptr, part.p);
+            reduce_plain(grid_k2, nq * B, dnorm.p);
+            BH_HIP(ctx, hipMemcpyAsync(nrm.data() + static_cast<size_t>(i0) * B, dnorm.p, static_cast<size_t>(nq) * B * 8,
+                                       hipMemcpyDeviceToHost, s));
+            BH_HIP(ctx, hipStreamSynchronize(s));
+        }
+        for (size_t i = 0; i < nrm.size(); ++i) f[i] = nrm[i] > 0 && std::isfinite(nrm[i]) ? 1.0 / std::sqrt(nrm[i]) : 0.0;
+        BH_HIP(ctx, hipMemcpyAsync(df.p, f.data(), f.size() * 8, hipMemcpyHostToDevice, s));
+        {
+            const int64_t n = static_cast<int64_t>(np) * vs;
+            hipLaunchKernelGGL((k_lz_scale<double, B>), dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, Tp, vs, np,
+                               df.p);
+        }
+        for (int i = 0; i < np; ++i) {
+            KTimer kt(ctx, "lanczos_finish_spmv", s);
+            SpmvArgs<double> sa{D, h->row_ptr, h->words, nullptr, nullptr, h->diag, nullptr, h->table, h->code_bits,
+                                dJ.p, dU.p, Tp + static_cast<int64_t>(i) * vs, HTp + static_cast<int64_t>(i) * vs, 1};
+            BH_TRY(bh_apply(ctx, h, &sa, nullptr, BH_FMT_PACKED, B, s));
+        }
+        BH_HIP(ctx, hipGetLastError());
+        std::vector<int> lr, n2(B, 0);
+        for (int b = 0; b < B; ++b)
+            if (on[b]) {
+                lr.push_back(b);
+                n2[b] = nvf + np;
+            }
+        BH_TRY(rayleigh_ritz(lr, nvf + np, n2, Yp, HYp, nvf));
+        BH_HIP(ctx, hipStreamSynchronize(s));  // the small arrays above are freed on return
         return BH_OK;
     }
 
@@ -2105,8 +2275,11 @@ class LzEngine final : public LzEngineBase {
     // A = Y^T H Y, the pencil (A, G) is solved on the host after dropping the numerically dependent
     // directions of G (copies of one level that finite-precision Lanczos produces), and Y, HY are replaced
     // in place by the Ritz vectors of the span and their images.  A candidate that mixes two eigenvectors
-    // is separated when the other one is in the span.
-    int rayleigh_ritz(const std::vector<int>& lanes, int nvf, const std::vector<int>& nt, double* Yp, double* HYp) {
+    // is separated when the other one is in the span.  The lowest nout (<= nvf, default nvf) Ritz vectors are
+    // kept; lanes not listed keep their first nout vectors unchanged (identity coefficients, exact).
+    int rayleigh_ritz(const std::vector<int>& lanes, int nvf, const std::vector<int>& nt, double* Yp, double* HYp,
+                      int nout = -1) {
+        if (nout < 0 || nout > nvf) nout = nvf;
         DevArray<double> dg, dz;
         std::vector<double> g(static_cast<size_t>(nvf) * 2 * nvf * B);
         if (nvf <= LZ_NVMAX) {
@@ -2156,7 +2329,14 @@ class LzEngine final : public LzEngineBase {
                                 gg[((static_cast<size_t>(i / NI) * nqi + q / NV) * per_blk) +
                                    ((static_cast<size_t>(i % NI) * 2 + w) * NV + q % NV) * B + b];
         }
-        std::vector<double> Z(static_cast<size_t>(nvf) * nvf * B, 0.0);
+        std::vector<double> Z(static_cast<size_t>(nvf) * nout * B, 0.0);
+        {
+            std::vector<char> listed(B, 0);
+            for (int b : lanes) listed[b] = 1;
+            for (int b = 0; b < B; ++b)
+                if (!listed[b])
+                    for (int i = 0; i < nout; ++i) Z[(static_cast<size_t>(i) * nout + i) * B + b] = 1.0;
+        }
         for (int b : lanes) {
             const int n = std::min(nt[b], nvf);
             if (n == 0) continue;
@@ -2185,25 +2365,25 @@ class LzEngine final : public LzEngineBase {
                 }
             bhtri::jacobi_eig(nk, Ah.data(), th.data(), Wv.data());
             for (int i = 0; i < n; ++i)
-                for (int c = 0; c < nk; ++c) {
+                for (int c = 0; c < nk && c < nout; ++c) {
                     double v = 0.0;
                     for (int d = 0; d < nk; ++d) v += S[i * nk + d] * Wv[d * nk + c];
-                    Z[(static_cast<size_t>(i) * nvf + c) * B + b] = v;
+                    Z[(static_cast<size_t>(i) * nout + c) * B + b] = v;
                 }
         }
         BH_TRY(dz.alloc(ctx, Z.size()));
         BH_HIP(ctx, hipMemcpyAsync(dz.p, Z.data(), Z.size() * 8, hipMemcpyHostToDevice, s));
         const unsigned grid = static_cast<unsigned>((vs + 255) / 256);
         if (nvf <= LZ_NVMAX) {
-            hipLaunchKernelGGL((k_lz_combine<B>), dim3(grid), dim3(256), 0, s, Yp, vs, dz.p, nvf, nvf);
-            hipLaunchKernelGGL((k_lz_combine<B>), dim3(grid), dim3(256), 0, s, HYp, vs, dz.p, nvf, nvf);
+            hipLaunchKernelGGL((k_lz_combine<B>), dim3(grid), dim3(256), 0, s, Yp, vs, dz.p, nvf, nout);
+            hipLaunchKernelGGL((k_lz_combine<B>), dim3(grid), dim3(256), 0, s, HYp, vs, dz.p, nvf, nout);
         } else {
             DevArray<double> tmp;
             double* tmpp = nullptr;
-            BH_TRY(scratch(LZ_WS_C, tmp, static_cast<size_t>(nvf) * vs, &tmpp));
+            BH_TRY(scratch(LZ_WS_C, tmp, static_cast<size_t>(nout) * vs, &tmpp));
             for (double* X : {Yp, HYp}) {
-                hipLaunchKernelGGL((k_lz_combine_wide<B>), dim3(grid), dim3(256), 0, s, X, tmpp, vs, dz.p, nvf, nvf);
-                BH_HIP(ctx, hipMemcpyAsync(X, tmpp, static_cast<size_t>(nvf) * vs * 8, hipMemcpyDeviceToDevice, s));
+                hipLaunchKernelGGL((k_lz_combine_wide<B>), dim3(grid), dim3(256), 0, s, X, tmpp, vs, dz.p, nvf, nout);
+                BH_HIP(ctx, hipMemcpyAsync(X, tmpp, static_cast<size_t>(nout) * vs * 8, hipMemcpyDeviceToDevice, s));
             }
             BH_HIP(ctx, hipStreamSynchronize(s));  // tmp is freed on return
         }
@@ -2940,10 +3120,11 @@ static int lanczos_batch(bh_ctx* ctx, bh_ham* h, int prec, const double* J, cons
         const double teardown = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - td0).count();
         std::fprintf(stderr,
                      "[lz-profile] D=%lld lanes=%d fp64=%d checks=%d setup=%.1f ms run=%.1f ms (capture %.1f) wait=%.1f ms "
-                     "host=%.1f ms (analysis %.1f, deflate %.1f, lock %.1f) finish=%.1f ms total=%.1f ms teardown=%.1f ms "
-                     "hipMalloc %lld (%.1f ms) hipFree %lld (%.1f ms)\n",
+                     "host=%.1f ms (analysis %.1f, deflate %.1f, lock %.1f) finish=%.1f ms (refinement steps %d, lane-steps %d) "
+                     "total=%.1f ms teardown=%.1f ms hipMalloc %lld (%.1f ms) hipFree %lld (%.1f ms)\n",
                      static_cast<long long>(h->D), nreal, f64 ? 1 : 0, pr.checks, pr.setup_ms, pr.run_ms, pr.capture_ms,
-                     pr.wait_ms, pr.host_ms, pr.analysis_ms, pr.deflate_ms, pr.lock_ms, pr.finish_ms, total, teardown,
+                     pr.wait_ms, pr.host_ms, pr.analysis_ms, pr.deflate_ms, pr.lock_ms, pr.finish_ms, pr.refine_steps,
+                     pr.refined_lanes, total, teardown,
                      static_cast<long long>(ctx->n_alloc - na0), ctx->alloc_ms - am0,
                      static_cast<long long>(ctx->n_free - nf0), ctx->free_ms - fm0);
     }

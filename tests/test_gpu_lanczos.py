@@ -146,6 +146,26 @@ def test_forced_fallback_and_nonconvergence(ctx):
     b.close()
 
 
+def test_refinement_certifies_small_hopping(ctx):
+    """Chain m=10, n=9, J/U in [0.01, 0.023]: |E0| is 0.02-0.07 while the FP32-stored Krylov vectors leave Ritz
+    residuals near 3e-7, so ||r||^2/gap of E0 sits above 1e-9 |E0| after the Lanczos run.  The FP64 Davidson
+    refinement of the finish certifies every lane with the rerun switched off: E0 within 1e-9 of the FP64 solve
+    and of the oracle's FP64 Lanczos, every level within 1e-6."""
+    m, off, sites, b, h = build(ctx, ("chain", 10, True), 9)
+    Js = 0.01 * 40.0 ** (np.arange(8) / 31.0)
+    r = h.lanczos(Js, 1.0, k=4, prec=bhx.PREC_MIXED, fallback=-1)
+    ref = h.lanczos(Js, 1.0, k=4, prec=bhx.PREC_FP64)
+    assert np.all(r.status == 0) and np.all(r.used_fp64 == 0)
+    assert np.all(r.errest[:, 0] <= 1e-9 * np.abs(r.evals[:, 0]))
+    assert np.all(rel(r.evals[:, 0], ref.evals[:, 0]) <= 1e-9)
+    assert np.all(rel(r.evals, ref.evals) <= 1e-6)
+    oh = O.hamiltonian(m, 9, off, sites)
+    rc, ev, _, _ = O.lanczos_mult(oh, Js[0], 1.0, 4, tol=1e-12)
+    assert rc == 0 and rel(r.evals[0, 0], ev[0]) <= 1e-9
+    h.close()
+    b.close()
+
+
 @pytest.mark.parametrize("prec,bound", [(bhx.PREC_FP64, 1e-8), (bhx.PREC_MIXED, 3e-4)])
 def test_ritz_vectors(ctx, prec, bound):
     """Returned Ritz vectors are unit vectors with small residuals against the oracle's dense H
