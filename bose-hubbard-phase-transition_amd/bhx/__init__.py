@@ -26,6 +26,8 @@ CONV_REF_INTERACTION = 2
 ORDER_REFERENCE = 0
 ORDER_ASCENDING = 1
 ORDER_LEX = 2
+ORDER_CUSTOM = 3
+ORDER_BAND = 4
 PREC_FP64 = 0
 PREC_MIXED = 1
 
@@ -111,6 +113,8 @@ def lib():
             "bh_basis_copy_states": (C.c_int, [_vp, _vp]),
             "bh_basis_copy_tags": (C.c_int, [_vp, _vp]),
             "bh_basis_copy_perm": (C.c_int, [_vp, _vp, _vp]),
+            "bh_basis_permute": (C.c_int, [_vp, _vp, _vp]),
+            "bh_basis_reorder_band": (C.c_int, [_vp, _vp, _vp, _vp]),
             "bh_hamiltonian_build": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint, C.POINTER(_vp)]),
             "bh_ham_destroy": (C.c_int, [_vp]),
             "bh_ham_dim": (C.c_int64, [_vp]),
@@ -313,6 +317,27 @@ class Basis:
         b = np.zeros(self.D, np.int32)
         self.ctx.check(lib().bh_basis_copy_perm(self.h, _ptr(a), _ptr(b)))
         return a, b
+
+    def permute(self, lex_of_pos):
+        """Re-orders the basis: position k holds the state of lexicographic rank lex_of_pos[k] (bh_basis_permute).
+        Build Hamiltonians after this call."""
+        lp = np.ascontiguousarray(lex_of_pos, np.int32)
+        if lp.shape != (self.D,):
+            raise BHError(EINVAL, f"permute: lex_of_pos must have {self.D} entries")
+        self.ctx.check(lib().bh_basis_permute(self.ctx.h, self.h, _ptr(lp)))
+        self.order = ORDER_CUSTOM
+
+    def reorder_band(self, off, sites):
+        """Re-orders the basis into the reverse Cuthill-McKee order of the lattice's hopping graph
+        (bh_basis_reorder_band). Build Hamiltonians after this call."""
+        off = np.ascontiguousarray(off, np.int32)
+        sites = np.ascontiguousarray(sites, np.int32)
+        if off.shape != (self.m + 1,):
+            raise BHError(EINVAL, f"reorder_band: off must have {self.m + 1} entries")
+        if sites.size == 0:
+            sites = np.zeros(1, np.int32)
+        self.ctx.check(lib().bh_basis_reorder_band(self.ctx.h, self.h, _ptr(off), _ptr(sites)))
+        self.order = ORDER_BAND
 
     def close(self):
         if self.h:

@@ -295,6 +295,7 @@ int bh_basis_build(bh_ctx* ctx, int m, int n, int order, bh_basis** out) {
     b->n = n;
     b->order = order;
     b->D = D;
+    b->layout_id = bh_next_layout_id();
     bh_state_layout(m, n, &b->nw, &b->bits);
 
     // rank table g[l][R] = C(R + m-l-2, m-l-1), l in [0, m-2], R in [0, n]; values < D

@@ -328,6 +328,7 @@ static int free_ham(bh_ham* h) {
     bh_dev_free(ctx, h->col64, static_cast<size_t>(h->nnz ? h->nnz : 1) * sizeof(int32_t));
     bh_dev_free(ctx, h->val64, static_cast<size_t>(h->nnz ? h->nnz : 1) * sizeof(double));
     bh_dev_free(ctx, h->diag64, static_cast<size_t>(h->D) * sizeof(double));
+    bh_dev_free(ctx, h->lex_of_pos, static_cast<size_t>(h->D) * sizeof(int32_t));
     delete h;
     return BH_OK;
 }
@@ -482,6 +483,12 @@ int bh_hamiltonian_build(bh_ctx* ctx, const bh_basis* b, const int* nbr_off, con
         std::memcpy(&f, &v, 4);
         return static_cast<double>(f);
     };
+    DevArray<int32_t> lop;
+    if (bh_basis_lex_start(b)) {
+        BH_TRY(lop.alloc(ctx, static_cast<size_t>(D)));
+        BH_HIP(ctx, hipMemcpyAsync(lop.p, b->lex_of_pos, static_cast<size_t>(D) * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+        BH_HIP(ctx, hipStreamSynchronize(s));
+    }
     h->nnz = nnz;
     h->max_row = stats[0];
     h->tmax = as_f(stats[1]);
@@ -494,6 +501,11 @@ int bh_hamiltonian_build(bh_ctx* ctx, const bh_basis* b, const int* nbr_off, con
     h->table32 = t32.release();
     h->bytes = static_cast<size_t>(D + 1) * 4 + static_cast<size_t>(nnz ? nnz : 1) * 4 + static_cast<size_t>(D) * 4 +
                tab * 12;
+    if (bh_basis_lex_start(b)) {
+        h->lex_of_pos = lop.release();
+        h->layout_id = b->layout_id;
+        h->bytes += static_cast<size_t>(D) * 4;
+    }
     *out = hp.release();
     return BH_OK;
 }

@@ -37,6 +37,7 @@ struct bh_ctx {
     int64_t r0_D = -1;
     uint64_t r0_seed = 0;
     int r0_mixed = -1;
+    uint64_t r0_layout = 0;  // layout_id of the basis for the lexicographically indexed start vectors, else 0
     double r0_scale = 0.0;
     // Ritz vectors kept by the last bh_lanczos(want_vectors=1): [B][k][D] doubles on the host
     std::vector<double> last_vecs;
@@ -125,7 +126,16 @@ struct bh_basis {
     int32_t* pos_of_lex = nullptr;  // [D]
     uint32_t* rank_tab = nullptr;   // [(m-1) * (n+1)] g[l][R] = C(R + m-l-2, m-l-1)
     size_t bytes = 0;
+    uint64_t layout_id = 0;         // new at every build and permutation (keys host caches of the layout)
 };
+
+// the caller-chosen orders index the solver's start vector by lexicographic rank, so that a solve runs the
+// lexicographic order's Krylov sequence whatever the permutation
+inline bool bh_basis_lex_start(const bh_basis* b) {
+    return b && (b->order == BH_ORDER_BAND || b->order == BH_ORDER_CUSTOM);
+}
+inline bool bh_basis_lex_start(int order) { return order == BH_ORDER_BAND || order == BH_ORDER_CUSTOM; }
+uint64_t bh_next_layout_id();
 
 // Autotuner result for one (kernel, precision, batch width) of a sector: the chosen variant and the
 // measured time of every candidate (variant, microseconds per launch); no candidates when not timed.
@@ -136,7 +146,11 @@ struct BhTuned {
 
 struct bh_ham {
     bh_ctx* ctx = nullptr;
-    const bh_basis* basis = nullptr;
+    const bh_basis* basis = nullptr;  // the basis at build time (not used after it: the basis may be destroyed)
+    // caller-chosen orders (bh_basis_lex_start): a copy of the basis's lex_of_pos at build time, which indexes the
+    // solver's start vectors, and the basis's layout_id then; nullptr and 0 otherwise
+    int32_t* lex_of_pos = nullptr;  // [D]
+    uint64_t layout_id = 0;
     int64_t D = 0, nnz = 0;
     int code_bits = 0;
     unsigned flags = 0;

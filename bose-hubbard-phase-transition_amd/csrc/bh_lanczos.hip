@@ -992,14 +992,17 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_gram_blk(const double* __restrict_
 // frozen before) and restarts from a fresh start vector orthogonal to every frozen vector; the kernels below
 // form that vector and keep frozen sets orthonormal.  cnt[b] < 0 leaves lane b untouched.
 
-// Z[r][b] = x_{off + r + 1} / (2^31 - 1) - 0.5 for lane b, where x_i = 16807^i x_0 mod (2^31 - 1): the Park-Miller
-// sequence of the start vector continued past its first `off` values (jumped to by modular powers)
+// Z[r][b] = x_{off + i(r) + 1} / (2^31 - 1) - 0.5 for lane b, where x_i = 16807^i x_0 mod (2^31 - 1): the
+// Park-Miller sequence of the start vector continued past its first `off` values (jumped to by modular powers);
+// i(r) = r, or lex_of_pos[r] when given (the caller-chosen orders, as for the start vector)
 template <int B>
-__global__ void __launch_bounds__(256) k_lz_pm_fill(double* __restrict__ Z, int64_t D, int b, uint64_t x0, uint64_t off) {
+__global__ void __launch_bounds__(256) k_lz_pm_fill(double* __restrict__ Z, int64_t D, int b, uint64_t x0, uint64_t off,
+                                                    const int32_t* __restrict__ lex_of_pos) {
     const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (r >= D) return;
     const uint64_t p = 2147483647ull;
-    uint64_t e = off + static_cast<uint64_t>(r) + 1, base = 16807ull, pw = 1;
+    const uint64_t ir = lex_of_pos ? static_cast<uint64_t>(lex_of_pos[r]) : static_cast<uint64_t>(r);
+    uint64_t e = off + ir + 1, base = 16807ull, pw = 1;
     while (e) {
         if (e & 1) pw = (pw * base) % p;
         base = (base * base) % p;
@@ -1134,16 +1137,28 @@ int lz_spec_steps() {
 }
 
 // start vector of the solver (Park-Miller sequence x <- 16807 x mod (2^31 - 1), seed 0 -> 1, values
-// x / (2^31 - 1) - 0.5), shared by every lane
-std::vector<double> lz_start_vector(int64_t D, uint64_t seed) {
-    std::vector<double> r(static_cast<size_t>(D));
+// x / (2^31 - 1) - 0.5), shared by every lane: position q takes value number q + 1 of the sequence, or, in the
+// caller-chosen orders (the Hamiltonian keeps lex_of_pos), value number lex(q) + 1 for the lexicographic rank
+// lex(q) of its state
+int lz_start_vector(bh_ctx* ctx, const bh_ham* h, uint64_t seed, std::vector<double>& r) {
+    const int64_t D = h->D;
+    r.assign(static_cast<size_t>(D), 0.0);
     const uint64_t p = 2147483647ull;
     uint64_t x = seed ? (seed & p) : 1;
     for (int64_t i = 0; i < D; ++i) {
         x = (16807ull * x) % p;
         r[i] = static_cast<double>(x) / static_cast<double>(p) - 0.5;
     }
-    return r;
+    if (h->lex_of_pos) {
+        std::vector<int32_t> lop(static_cast<size_t>(D));
+        BH_HIP(ctx, hipMemcpyAsync(lop.data(), h->lex_of_pos, lop.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        BH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        std::vector<double> q(static_cast<size_t>(D));
+        for (int64_t i = 0; i < D; ++i) q[i] = r[lop[i]];
+        r.swap(q);
+    }
+    return BH_OK;
 }
 
 struct RunHost {
@@ -1337,8 +1352,11 @@ class LzEngine final : public LzEngineBase {
         BH_HIP(ctx, hipMemsetAsync(st_i.p, 0, st_i.bytes(), s));
         // start vector and the inverse norm of its stored copy (cached in the context workspace per
         // (D, seed, storage precision) when the workspace is used)
-        if (!ws || ctx->r0_D != D || ctx->r0_seed != o.seed || ctx->r0_mixed != static_cast<int>(MIXED)) {
-            std::vector<double> r = lz_start_vector(D, o.seed);
+        const uint64_t layout = h->lex_of_pos ? h->layout_id : 0;
+        if (!ws || ctx->r0_D != D || ctx->r0_seed != o.seed || ctx->r0_mixed != static_cast<int>(MIXED) ||
+            ctx->r0_layout != layout) {
+            std::vector<double> r;
+            BH_TRY(lz_start_vector(ctx, h, o.seed, r));
             BH_HIP(ctx, hipMemcpyAsync(r0, r.data(), D * 8, hipMemcpyHostToDevice, s));
             double nrm = 0.0;
             for (int64_t i = 0; i < D; ++i) {
@@ -1351,6 +1369,7 @@ class LzEngine final : public LzEngineBase {
                 ctx->r0_D = D;
                 ctx->r0_seed = o.seed;
                 ctx->r0_mixed = static_cast<int>(MIXED);
+                ctx->r0_layout = layout;
                 ctx->r0_scale = scale0;
             }
         } else {
@@ -2881,7 +2900,8 @@ class LzEngine final : public LzEngineBase {
             const uint64_t p = 2147483647ull;
             const uint64_t x0 = o.seed ? (o.seed & p) : 1;
             hipLaunchKernelGGL((k_lz_pm_fill<B>), dim3(static_cast<unsigned>((D + 255) / 256)), dim3(256), 0, s, W, D, b, x0,
-                               static_cast<uint64_t>(rh[b].pass + 1) * static_cast<uint64_t>(D));
+                               static_cast<uint64_t>(rh[b].pass + 1) * static_cast<uint64_t>(D),
+                               h->lex_of_pos);
         }
         BH_TRY(ortho_frozen(W, cnt));
         BH_TRY(inv_norms(static_cast<const double*>(W), f));
@@ -3183,7 +3203,7 @@ int solve_diagonal(bh_ctx* ctx, bh_ham* h, const std::vector<int>& runs, const d
         h->vlevels = v;
     }
     std::vector<double> r;
-    if (vecs && o.distinct) r = lz_start_vector(h->D, o.seed);
+    if (vecs && o.distinct) BH_TRY(lz_start_vector(ctx, h, o.seed, r));
     // with multiplicity: the k lowest diagonal entries, ties in position order, eigenvectors e_q
     std::vector<int64_t> order;
     if (!o.distinct) {

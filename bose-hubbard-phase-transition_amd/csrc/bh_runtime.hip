@@ -1,3 +1,4 @@
+#include <atomic>
 // bh_runtime.hip -- context, errors, budgeted device memory, kernel timing, lattices.
 #include <chrono>
 #include <cmath>
@@ -39,6 +40,11 @@ void bh_dev_free(bh_ctx* ctx, void* p, size_t bytes) {
     ctx->n_free += 1;
     ctx->free_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     ctx->used = ctx->used >= bytes ? ctx->used - bytes : 0;
+}
+
+uint64_t bh_next_layout_id() {
+    static std::atomic<uint64_t> next{1};
+    return next.fetch_add(1);
 }
 
 int bh_ws_get(bh_ctx* ctx, int slot, size_t bytes, void** p) {

@@ -52,6 +52,11 @@ extern "C" {
 /* BH_ORDER_LEX: descending lexicographic enumeration order itself (the states before any sort); no
  * reference function returns it, it is offered as a solver layout whose hops stay closer together. */
 #define BH_ORDER_LEX 2
+/* BH_ORDER_CUSTOM: a permutation set by bh_basis_permute.  BH_ORDER_BAND: the reverse Cuthill-McKee order of a
+ * lattice's hopping graph (bh_basis_reorder_band); no reference function returns either, they are solver
+ * layouts like BH_ORDER_LEX.  bh_basis_build accepts neither. */
+#define BH_ORDER_CUSTOM 3
+#define BH_ORDER_BAND 4
 
 /* ---- precisions --------------------------------------------------------------------------------- */
 /* BH_PREC_FP64: FP64 vectors and arithmetic -- the plain comparator.  The SpMV entry points read an FP64
@@ -102,6 +107,16 @@ int bh_basis_copy_states(const bh_basis* b, uint8_t* states);
 int bh_basis_copy_tags(const bh_basis* b, double* tags);
 /* lex_of_pos[k]: lexicographic rank of the state at position k; pos_of_lex: the inverse */
 int bh_basis_copy_perm(const bh_basis* b, int32_t* lex_of_pos, int32_t* pos_of_lex);
+/* Re-orders a basis: the state at position k becomes the state of lexicographic rank lex_of_pos[k] (D entries, a
+ * permutation of 0 .. D-1, else BH_EINVAL); the order becomes BH_ORDER_CUSTOM.  Hamiltonians built on the basis
+ * before the call keep the former order: build them after it.  No reference counterpart: the reference has one
+ * order (BH::fixed_set_basis, src/hamiltonian.cpp:60-122) and the spectra do not depend on it. */
+int bh_basis_permute(bh_ctx* ctx, bh_basis* b, const int32_t* lex_of_pos);
+/* Re-orders a basis into BH_ORDER_BAND for the lattice (nbr_off, nbr_sites as for bh_hamiltonian_build): the
+ * reverse Cuthill-McKee order of the hopping graph, computed on the host from a Hamiltonian built in the current
+ * order, so that every hop lies close to the diagonal and the gathers of the operator pass stay in L2-sized
+ * windows.  Deterministic; same conditions as bh_basis_permute. */
+int bh_basis_reorder_band(bh_ctx* ctx, bh_basis* b, const int* nbr_off, const int* nbr_sites);
 
 /* Replaces Analysis::SPDM and Analysis::braket (include/analysis.hpp:56-57, src/analysis.cpp:497-512,
  * 562-594): the single-particle density matrix rho[i*m + j] = <phi| b_i^+ b_j |phi> of a real state phi
