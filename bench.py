@@ -343,6 +343,25 @@ def request_dim(q):
     return math.comb((m if g == "chain" else Lx * Ly) + n - 1, n)
 
 
+def rerun_breakdown(res, big, reqs):
+    """FP64 reruns of mixed-lane requests against the mixed-lane requests, by J/U band and by filling (n against
+    the number of sites): "band": [reruns, requests]."""
+    bands = [("J<0.03", 0.0, 0.03), ("0.03<=J<0.1", 0.03, 0.1), ("0.1<=J<0.3", 0.1, 0.3), ("J>=0.3", 0.3, 2.0)]
+    out = {}
+    for r, bg, q in zip(res, big, reqs):
+        if r is None or not bg:
+            continue
+        g, m, Lx, Ly, n, J = q[:6]
+        sites = m if g == "chain" else Lx * Ly
+        keys = [name for name, lo, hi in bands if lo <= J < hi]
+        keys.append("n<sites" if n < sites else ("n=sites" if n == sites else "n>sites"))
+        for key in keys:
+            c = out.setdefault(key, [0, 0])
+            c[0] += int(bool(r.used_fp64))
+            c[1] += 1
+    return out
+
+
 def stream_summary(res, span, rate, t_sub, qs, ok, reqs):
     lat = np.array([r.t_done - r.t_submit for r in res if r is not None])
     big = [request_dim(q) > FULLRE_D for q in reqs]
@@ -354,6 +373,7 @@ def stream_summary(res, span, rate, t_sub, qs, ok, reqs):
             "fp64_by_design": int(sum(1 for r, bg in zip(res, big) if r is not None and not bg)),
             "mixed_lane_requests": int(sum(big)),
             "fp64_reruns_of_mixed": int(sum(1 for r, bg in zip(res, big) if r is not None and bg and r.used_fp64)),
+            "reruns_by": rerun_breakdown(res, big, reqs),
             "queue_first_half_mean": float(np.mean([q for t, q in qs if t < t_sub / 2])) if qs else None,
             "queue_second_half_mean": float(np.mean([q for t, q in qs if t >= t_sub / 2])) if qs else None}
 

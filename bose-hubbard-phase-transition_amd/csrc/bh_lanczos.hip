@@ -1604,6 +1604,10 @@ class LzEngine final : public LzEngineBase {
                 if (an.theta.front() < hlo[b] - slack || an.theta.back() > hhi[b] + slack) broken = true;
             }
             if (broken) {
+                if (lz_debug())
+                    std::fprintf(stderr, "[lz] broken lane=%d j=%d tot=%d tnorm=%.3e tnorm0=%.3e bnext=%.3e lo=%.6f hi=%.6f th0=%.6f thn=%.6f\n",
+                                 b, j, tj, an.tnorm, R.tnorm0, bnext, hlo[b], hhi[b], an.theta.empty() ? 0.0 : an.theta.front(),
+                                 an.theta.empty() ? 0.0 : an.theta.back());
                 R.done = true;
                 R.iters = tj;
                 R.status = BH_ENOCONV;
@@ -2625,22 +2629,28 @@ class LzEngine final : public LzEngineBase {
         return BH_OK;
     }
     // restarts the flagged lanes (plans made by the analysis): kept Ritz vectors in the tridiagonal basis of
-    // arrow_to_tridiag in slots 0..p-1, the normalised next Lanczos vector in slot p, T rewritten to match
+    // arrow_to_tridiag in slots 0..p-1, the normalised next Lanczos vector in slot p, T rewritten to match.
+    // Rows 0..p of alpha, beta and scale are written back for every lane, so they are read from the device
+    // here, after the speculative steps: another lane early in a pass (just admitted or deflated) has written
+    // some of those rows since the analysis copied them, and a stale copy would overwrite its recurrence.
     int restart(const std::vector<int>& flag) {
-        int jmax = 0, nv = 0;
+        int jmax = 0, nv = 0, pmax = 0;
         for (int b = 0; b < B; ++b)
             if (flag[b]) {
                 jmax = std::max(jmax, plan[b].j);
                 nv = std::max(nv, static_cast<int>(plan[b].theta.size()) + 1);
+                pmax = std::max(pmax, static_cast<int>(plan[b].theta.size()));
             }
-        std::vector<double> hsc(static_cast<size_t>(jmax + 1) * B);
+        std::vector<double> hsc(static_cast<size_t>(std::max(jmax, pmax) + 1) * B);
+        std::vector<double> fa(static_cast<size_t>(std::max(pmax, 1)) * B), fb(static_cast<size_t>(pmax + 1) * B);
         std::vector<int> hstep(B);
         BH_HIP(ctx, hipMemcpyAsync(hsc.data(), st.scale, hsc.size() * 8, hipMemcpyDeviceToHost, s));
         BH_HIP(ctx, hipMemcpyAsync(hstep.data(), st.step, B * 4, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipMemcpyAsync(fa.data(), st.alpha, fa.size() * 8, hipMemcpyDeviceToHost, s));
+        BH_HIP(ctx, hipMemcpyAsync(fb.data(), st.beta, fb.size() * 8, hipMemcpyDeviceToHost, s));
         BH_HIP(ctx, hipStreamSynchronize(s));
         std::vector<double> C(static_cast<size_t>(jmax + 1) * nv * B, 0.0);
         std::vector<int> nin(B, 0), nout(B, 0);
-        int pmax = 0;
         for (int b = 0; b < B; ++b) {
             if (!flag[b]) continue;
             const Restart& P = plan[b];
@@ -2664,17 +2674,16 @@ class LzEngine final : public LzEngineBase {
             C[(static_cast<size_t>(j) * nv + p) * B + b] = hsc[static_cast<size_t>(j) * B + b];
             nin[b] = j + 1;
             nout[b] = p + 1;
-            // T of the restarted lane; rows 0..p of the host copies are written back for every lane
-            for (int c = 0; c < p; ++c) ha[static_cast<size_t>(c) * B + b] = ta[c];
-            hb[b] = 0.0;
-            for (int c = 1; c < p; ++c) hb[static_cast<size_t>(c) * B + b] = tb[c];
-            hb[static_cast<size_t>(p) * B + b] = ns;
+            // T of the restarted lane, in the fresh copies of rows 0..p
+            for (int c = 0; c < p; ++c) fa[static_cast<size_t>(c) * B + b] = ta[c];
+            fb[b] = 0.0;
+            for (int c = 1; c < p; ++c) fb[static_cast<size_t>(c) * B + b] = tb[c];
+            fb[static_cast<size_t>(p) * B + b] = ns;
             for (int c = 0; c <= p; ++c) hsc[static_cast<size_t>(c) * B + b] = 1.0;
             hstep[b] = p;
             jl[b] = p;
             kept[b] = p;
             nextc[b] = p + chunk;
-            pmax = std::max(pmax, p);
             if (lz_debug()) std::fprintf(stderr, "[lz] restart lane=%d j=%d tot=%d keep=%d sigma=%.3e\n", b, j, tot[b], p, ns);
         }
         BH_TRY(dC.alloc(ctx, C.size()));
@@ -2687,8 +2696,8 @@ class LzEngine final : public LzEngineBase {
                                dnj.p, dno.p, dC.p, nv);
         }
         BH_HIP(ctx, hipGetLastError());
-        BH_HIP(ctx, hipMemcpyAsync(st.alpha, ha.data(), static_cast<size_t>(pmax) * B * 8, hipMemcpyHostToDevice, s));
-        BH_HIP(ctx, hipMemcpyAsync(st.beta, hb.data(), static_cast<size_t>(pmax + 1) * B * 8, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(st.alpha, fa.data(), static_cast<size_t>(pmax) * B * 8, hipMemcpyHostToDevice, s));
+        BH_HIP(ctx, hipMemcpyAsync(st.beta, fb.data(), static_cast<size_t>(pmax + 1) * B * 8, hipMemcpyHostToDevice, s));
         BH_HIP(ctx, hipMemcpyAsync(st.scale, hsc.data(), static_cast<size_t>(pmax + 1) * B * 8, hipMemcpyHostToDevice, s));
         BH_HIP(ctx, hipMemcpyAsync(st.step, hstep.data(), B * 4, hipMemcpyHostToDevice, s));
         // the first two steps after a restart orthogonalise against the locked vectors
@@ -3111,7 +3120,7 @@ static int lanczos_batch(bh_ctx* ctx, bh_ham* h, int prec, const double* J, cons
     if (!e) return rc;
     // the lanes of one batch finish together: their deflation passes start together too (one formation of the
     // frozen vectors for all of them; a lane's arithmetic does not depend on when its pass starts)
-    e->set_deflation_barrier(true);
+    e->set_deflation_barrier(!(std::getenv("BHX_DEFL_BARRIER") && std::strcmp(std::getenv("BHX_DEFL_BARRIER"), "0") == 0));
     for (int b = 0; b < nreal; ++b) BH_TRY(e->admit(b, J[b], U[b]));
     e->prof.setup_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts0).count();
     std::vector<int> fin, all;

@@ -166,6 +166,25 @@ def test_refinement_certifies_small_hopping(ctx):
     b.close()
 
 
+def test_deflation_without_barrier(ctx, monkeypatch):
+    """Chain m=12, n=10, J/U in [0.3, 1], sixteen lanes with the deflation barrier off (BHX_DEFL_BARRIER=0, as
+    in the server): the lanes of large J start their second pass while the others still run and restart.
+    Every lane converges and agrees with the barrier run (E0 within 1e-9, levels within 1e-6)."""
+    off, sites = O.chain_neighbours(12, True)
+    b = ctx.basis(12, 10, bhx.ORDER_LEX)
+    h = b.hamiltonian(off, sites)
+    rng = np.random.default_rng(5)
+    Js = np.sort(10 ** rng.uniform(np.log10(0.3), 0, 16))
+    ref = h.lanczos(Js, 1.0, k=4, prec=bhx.PREC_MIXED, fallback=-1, ncv=160)
+    monkeypatch.setenv("BHX_DEFL_BARRIER", "0")
+    r = h.lanczos(Js, 1.0, k=4, prec=bhx.PREC_MIXED, fallback=-1, ncv=160)
+    assert np.all(ref.status == 0) and np.all(r.status == 0)
+    assert np.all(rel(r.evals[:, 0], ref.evals[:, 0]) <= 1e-9)
+    assert np.all(rel(r.evals, ref.evals) <= 1e-6)
+    h.close()
+    b.close()
+
+
 @pytest.mark.parametrize("prec,bound", [(bhx.PREC_FP64, 1e-8), (bhx.PREC_MIXED, 3e-4)])
 def test_ritz_vectors(ctx, prec, bound):
     """Returned Ritz vectors are unit vectors with small residuals against the oracle's dense H

@@ -159,3 +159,27 @@ def test_server_budget_contention(ctx):
             assert abs(r.evals[0] - want[0]) <= 1e-9 * abs(want[0])
             h.close()
             b.close()
+
+
+def test_lanes_at_different_passes(ctx):
+    """Chain m=12, n=10, sixteen requests at J/U in [0.3, 1] on one engine: the lanes of large J finish their
+    first pass and start the next (deflation) while the others still iterate and restart.  A thick restart
+    writes rows 0..p of the recurrence back for every lane; those rows are read after the speculative steps,
+    so a lane early in its pass keeps its own.  Every request is certified in mixed precision (no FP64 rerun)
+    and E0 matches the batch solve within 1e-9."""
+    rng = np.random.default_rng(5)
+    Js = np.sort(10 ** rng.uniform(np.log10(0.3), 0, 16))
+    with bhx.Server(ctx, lanes=16, k=4, ncv=160) as srv:
+        ids = [srv.submit("chain", 10, float(J), 1.0, 0.0, m=12) for J in Js]
+        assert srv.drain(300)
+        res = [srv.wait(i) for i in ids]
+    off, sites = O.chain_neighbours(12, True)
+    b = ctx.basis(12, 10, bhx.ORDER_LEX)
+    h = b.hamiltonian(off, sites)
+    ref = h.lanczos(Js, 1.0, k=4, prec=bhx.PREC_MIXED, ncv=160)
+    h.close()
+    b.close()
+    assert all(r.status == 0 for r in res)
+    assert [int(r.used_fp64) for r in res] == [0] * 16
+    E0 = np.array([r.evals[0] for r in res])
+    assert np.all(np.abs(E0 - ref.evals[:, 0]) <= 1e-9 * np.abs(ref.evals[:, 0]))
