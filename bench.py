@@ -381,7 +381,7 @@ def stream_summary(res, span, rate, t_sub, qs, ok, reqs):
 def serving(ctx, nreq, rate, sustained_n, seed=7):
     """The serving stream: nreq requests at `rate` req/s offered load (Poisson) from a cold server (sector
     CSRs built and the operator tuned on first use), then, on the same warm server, sustained_n requests at
-    0.7 of the rate the first stream completed (the sustained operating point: the queue is sampled to show it
+    0.4 of the rate the first stream completed (the sustained operating point: the queue is sampled to show it
     does not grow).  k = 4, mixed precision with FP64 rerun on rejection, values with multiplicity."""
     reqs, rng = request_stream(nreq, seed)
     with bhx.Server(ctx, lanes=16, k=4, ncv=160) as srv:
@@ -390,7 +390,8 @@ def serving(ctx, nreq, rate, sustained_n, seed=7):
         log("serving cold:", json.dumps(cold))
         sus = None
         if sustained_n > 0:
-            r2 = 0.7 * cold["solves_per_s"]
+            # the cold rate comes from full batches over a deep backlog; at 0.7 of it the queue still grew
+            r2 = 0.4 * cold["solves_per_s"]
             reqs2, rng2 = request_stream(sustained_n, seed + 1)
             res2, span2, t_sub2, qs2, ok2 = stream(srv, reqs2, r2, rng2)
             sus = stream_summary(res2, span2, r2, t_sub2, qs2, ok2, reqs2)
