@@ -1513,7 +1513,7 @@ class LzEngine final : public LzEngineBase {
     // step of any lane.  With timing on, the launches are issued one by one so that each carries its events.
     int run(int steps) override {
         if (!tuned) BH_TRY(tune_k1());
-        const int LM = lmax_used == 0 ? 0 : (lmax_used <= 4 ? 4 : 16);
+        const int LM = step_lm();
         const int nsteps = steps;
         if (use_graphs && !ctx->timing) {
             while (steps >= LZ_GRAPH_STEPS) {
@@ -2432,6 +2432,16 @@ class LzEngine final : public LzEngineBase {
     // hundreds of microseconds; smaller engines analyse at the scheduled steps only.  The rule depends on
     // the sizes alone, so the sequence of analyses and its results are reproducible.
     bool speculate() const { return lz_spec_steps() > 0 && vs >= LZ_SPEC_MIN_WORK; }
+    // orthogonalisation variant of the next steps: the smallest of 0, 4, 8, 16 locked vectors that covers every
+    // iterating lane's current set (a lane's set shrinks back to its frozen vectors at a deflation); none under
+    // full reorthogonalisation, whose steps project out the frozen vectors themselves
+    int step_lm() const {
+        if (fullre) return 0;
+        int mx = 0;
+        for (int b = 0; b < B; ++b)
+            if (act[b]) mx = std::max(mx, nlock[b]);
+        return mx == 0 ? 0 : (mx <= 4 ? 4 : (mx <= 8 ? 8 : 16));
+    }
     // K1 variants: 0 = staged row blocks; V + 16 u (V = 1, 2, 4, 8 vectors per thread, u = 0, 1, 2 for
     // U = 4, 8, 16 entries of a row in flight) = row-lane engine
     static int k1_V(int var) { return var & 15; }
@@ -2491,6 +2501,7 @@ class LzEngine final : public LzEngineBase {
         if (LM > 0) {
             KTimer kt(ctx, "lanczos_sodots", s);
             if (LM == 4) hipLaunchKernelGGL((k_lz_sodots<XT, B, 4>), dim3(grid_k1), dim3(LZ_WG), 0, s, a, rpw_so, nv);
+            else if (LM == 8) hipLaunchKernelGGL((k_lz_sodots<XT, B, 8>), dim3(grid_k1), dim3(LZ_WG), 0, s, a, rpw_so, nv);
             else hipLaunchKernelGGL((k_lz_sodots<XT, B, 16>), dim3(grid_k1), dim3(LZ_WG), 0, s, a, rpw_so, nv);
         }
     }
@@ -2571,6 +2582,7 @@ class LzEngine final : public LzEngineBase {
             KTimer kt(ctx, "lanczos_update", s);
             if (LM == 0) hipLaunchKernelGGL((k_lz_b<XT, B, 0>), dim3(grid_k2), dim3(LZ_WG), 0, s, a);
             else if (LM == 4) hipLaunchKernelGGL((k_lz_b<XT, B, 4>), dim3(grid_k2), dim3(LZ_WG), 0, s, a);
+            else if (LM == 8) hipLaunchKernelGGL((k_lz_b<XT, B, 8>), dim3(grid_k2), dim3(LZ_WG), 0, s, a);
             else hipLaunchKernelGGL((k_lz_b<XT, B, 16>), dim3(grid_k2), dim3(LZ_WG), 0, s, a);
         }
         if (fullre) {
