@@ -405,7 +405,7 @@ class Hamiltonian:
         return Y[:, 0] if one else Y
 
     def prepare(self, prec: int):
-        """Materialises what `prec` needs (the FP64 CSR) ahead of the first operator application."""
+        """Materialises what `prec` needs (the FP64 CSR of the FP64 SpMV) ahead of the first operator application."""
         self.ctx.check(lib().bh_ham_prepare(self.h, prec))
 
     def spmv_device(self, J, U, x_ptr: int, y_ptr: int, B: int, prec: int = PREC_FP64, noalloc: bool = False,

@@ -158,7 +158,8 @@ int64_t bh_ham_device_bytes(const bh_ham* h);
 int bh_ham_copy_csr(const bh_ham* h, int64_t* row_ptr, int32_t* col, double* val, double* diag);
 /* the packed words (column << code_bits | code) and code_bits of the mixed format */
 int bh_ham_copy_packed(const bh_ham* h, uint32_t* words, int* code_bits);
-/* Materialise the FP64 CSR (needed once before BH_PREC_FP64 use; done lazily otherwise). */
+/* Materialise the FP64 CSR of the FP64 SpMV entry points (needed once before their BH_PREC_FP64 use, and before
+ * bh_spmv_noalloc in FP64; done lazily by bh_spmv otherwise; the Lanczos engines do not use it). */
 int bh_ham_prepare(bh_ham* h, int prec);
 /* No counterpart in the reference (its product has one fixed kernel): the operator-kernel variant the
  * autotuner chose for this sector at (kind, prec, B), once the sector has been used that way (kind 0: the
