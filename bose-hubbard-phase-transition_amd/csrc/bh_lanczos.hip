@@ -365,8 +365,10 @@ __global__ void __launch_bounds__(LZ_WG) k_lz_sodots(LzArgs<XT> a, int64_t rows_
 // <y_l, v_{j-1}>, dropping <r_l, v_j> for the locked vector's residual r_l (in exact arithmetic r_l lies along
 // one earlier Lanczos vector, so the term is at the level of the basis' loss of orthogonality).  A one-read
 // step reads each locked vector once instead of twice (K1 skips them).
+// Eight locked vectors: held to 64 VGPRs (eight waves per SIMD, the 1024-workgroup grid in one round; 20 bytes
+// of spills per lane) -- 131 us per launch at the chain m=12 sector against 138 us at 66 VGPRs and seven waves.
 template <typename XT, int B, int LM>
-__global__ void __launch_bounds__(LZ_WG) k_lz_b(LzArgs<XT> a) {
+__global__ void __launch_bounds__(LZ_WG) __attribute__((amdgpu_waves_per_eu(LM == 8 ? 8 : 1))) k_lz_b(LzArgs<XT> a) {
     constexpr int LL = LM > 0 ? LM : 1;
     constexpr int NV2 = (1 + LM) * B;
     __shared__ double lds[LZ_PUT_LDS];
