@@ -894,9 +894,7 @@ __global__ void __launch_bounds__(256) k_lz_combine(double* X, int64_t vs, const
     double x[LZ_NVMAX];
 #pragma unroll
     for (int i = 0; i < LZ_NVMAX; ++i) x[i] = i < nin ? X[static_cast<int64_t>(i) * vs + idx] : 0.0;
-#pragma unroll
-    for (int c = 0; c < LZ_NVMAX; ++c) {
-        if (c >= nout) break;
+    for (int c = 0; c < nout; ++c) {
         double acc = 0.0;
 #pragma unroll
         for (int i = 0; i < LZ_NVMAX; ++i)
@@ -2794,7 +2792,6 @@ class LzEngine final : public LzEngineBase {
             }
             if (!new_locks[b].empty()) anv[b] = ans[b].tnorm;
             nlock[b] += appended;
-            lmax_used = std::max(lmax_used, nlock[b]);
         }
         // vector i of lane b goes to Y slot dst[i][b] (one interleaved column each), in one launch
         BH_HIP(ctx, hipMemcpyAsync(lock_map.p, dst.data(), dst.size() * sizeof(int), hipMemcpyHostToDevice, s));
@@ -2970,7 +2967,6 @@ class LzEngine final : public LzEngineBase {
                     tau[static_cast<size_t>(l) * B + b] = eps_store;
                     tau[static_cast<size_t>(LZ_LMAX + l) * B + b] = eps_store;
                 }
-                lmax_used = std::max(lmax_used, R.nfz);
             }
             if (lz_debug())
                 std::fprintf(stderr, "[lz] deflate lane=%d tot=%d pass=%d frozen=%d\n", b, tot[b], R.pass, R.nfz);
@@ -3034,7 +3030,6 @@ class LzEngine final : public LzEngineBase {
     Deflation dplan[B];
     std::vector<int> waiting;   // lanes whose pass is complete, waiting for the deflation barrier
     bool defl_barrier = false;  // batch solves: lanes deflate together once none is iterating
-    int lmax_used = 0;
     bool mult = true;  // eigenvalues with multiplicity (deflation passes); false: distinct levels
     int ymax = LZ_LMAX;
 };
